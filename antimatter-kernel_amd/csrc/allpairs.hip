@@ -1,0 +1,281 @@
+// Exact all-pairs Coulomb force and potential (no cutoff): the theta = 0 limit of the reference's
+// Barnes-Hut (src/interaction/barnes-hut.h:22-24) with the AddPointCharge pair term
+// (barnes-hut.cc:19-27), and the all-pairs workload's hot kernel.
+//
+// FP32 path (AllPairsF32Kernel): every lane owns one target; the wave walks the source array in
+// lockstep, so source loads are wave-uniform scalar loads (s_load) and the pair math runs on the
+// FP32 VALU with SGPR operands.  Coordinates are hi/lo-split floats of x * 2^20 (units of
+// ~0.95 um): dx = (xh_i - xh_j) + (xl_i - xl_j) keeps ~2^-24 relative accuracy on dx even for
+// pairs much closer than the cloud size.  Each tile of kTile sources is summed in FP32 and the
+// tile total is added to an FP64 accumulator (blocked compensated summation).  A softening of
+// 1e-30 m^2 (the reference's pair floor) on r^2 makes coincident pairs, including i == j,
+// contribute exactly zero without a branch; annihilated particles carry charge 0 in the
+// mirror and drop out the same way.  Grid = (target blocks) x (source chunks); chunk partials
+// are reduced in a fixed order, so results are deterministic.
+//
+// FP64 path (AllPairsF64Kernel): one thread per target, sources j = 0..n-1 in ascending order
+// with the reference pair term, no FMA contraction: the same operations in the same order as
+// the CPU restatement, so results are expected to match it bit for bit.
+#include <algorithm>
+
+#include "am_internal.hpp"
+
+namespace am {
+
+constexpr int kAllPairsThreads = 256;
+constexpr int kTile = 128;  // sources per FP32 partial sum
+
+// Pair interaction in FP32 on hi/lo coordinates.  Returns s = q_j / (r^2 + eps^2)^(3/2) and
+// the separation d = x_i - x_j (units).
+__device__ __forceinline__ void PairF32(const float4 th, const float4 tl, const float4 sh, const float4 sl,
+                                        float& px, float& py, float& pz) {
+  const float dx = (th.x - sh.x) + (tl.x - sl.x);
+  const float dy = (th.y - sh.y) + (tl.y - sl.y);
+  const float dz = (th.z - sh.z) + (tl.z - sl.z);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float s = (sh.w * ri) * (ri * ri);
+  px = fmaf(s, dx, px);
+  py = fmaf(s, dy, py);
+  pz = fmaf(s, dz, pz);
+}
+
+// part layout: [chunk][component][n_pad] doubles (acc in e/unit^2 before scaling).
+__global__ __launch_bounds__(kAllPairsThreads) void AllPairsF32Kernel(const float4* __restrict__ src, int n_pad,
+                                                                       int chunk, double* __restrict__ part) {
+  const int i = blockIdx.x * kAllPairsThreads + threadIdx.x;  // < n_pad by construction
+  const float4 th = src[2 * i];
+  const float4 tl = src[2 * i + 1];
+  const int j0 = blockIdx.y * chunk;
+  const int j1 = min(n_pad, j0 + chunk);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int jt = j0; jt < j1; jt += kTile) {
+    float px = 0.f, py = 0.f, pz = 0.f;
+    const float4* s = src + 2 * jt;
+#pragma unroll 8
+    for (int k = 0; k < kTile; ++k) PairF32(th, tl, s[2 * k], s[2 * k + 1], px, py, pz);
+    ax += (double)px;
+    ay += (double)py;
+    az += (double)pz;
+  }
+  double* out = part + (size_t)blockIdx.y * 3 * n_pad;
+  out[i] = ax;
+  out[n_pad + i] = ay;
+  out[2 * (size_t)n_pad + i] = az;
+}
+
+// Sum chunk partials in order and scale to newtons: F = k_C q_i e * 2^40 * acc.
+__global__ void AllPairsReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
+                                     const double* __restrict__ q, const uint8_t* __restrict__ flags,
+                                     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const double* p = part + (size_t)c * 3 * n_pad;
+    sx += p[i];
+    sy += p[n_pad + i];
+    sz += p[2 * (size_t)n_pad + i];
+  }
+  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0;
+  const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
+  fx[i] = scale * sx;
+  fy[i] = scale * sy;
+  fz[i] = scale * sz;
+}
+
+// FP64 source record: {x, y, z, q} with q = 0 for annihilated particles.
+__global__ void PackF64SourceKernel(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                    const double* __restrict__ z, const double* __restrict__ q,
+                                    const uint8_t* __restrict__ flags, double4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = make_double4(x[i], y[i], z[i], (flags[i] & kFlagAnnihilated) ? 0.0 : q[i]);
+}
+
+// AddPointCharge (barnes-hut.cc:19-27) in FP64, sources in ascending index order.
+__global__ __launch_bounds__(256) void AllPairsF64Kernel(int n, const double4* __restrict__ src,
+                                                          double* __restrict__ fx, double* __restrict__ fy,
+                                                          double* __restrict__ fz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int it = min(i, n - 1);
+  const double4 t = src[it];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (t.w != 0.0) {
+    for (int j = 0; j < n; ++j) {
+      const double4 s = src[j];
+      if (j == it || s.w == 0.0) continue;
+      const double rx = __dsub_rn(t.x, s.x), ry = __dsub_rn(t.y, s.y), rz = __dsub_rn(t.z, s.z);
+      const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
+      if (d2 < kMinSquaredDistance) continue;
+      const double inv = 1.0 / sqrt(d2);
+      const double mag = __ddiv_rn(__dmul_rn(__dmul_rn(kCoulomb, t.w), s.w), d2);
+      const double c = __dmul_rn(mag, inv);
+      ax = __dadd_rn(ax, __dmul_rn(rx, c));
+      ay = __dadd_rn(ay, __dmul_rn(ry, c));
+      az = __dadd_rn(az, __dmul_rn(rz, c));
+    }
+  }
+  if (i < n) {
+    fx[i] = ax;
+    fy[i] = ay;
+    fz[i] = az;
+  }
+}
+
+// Potential per particle (AccumulatePotential leaf term, barnes-hut.cc:223-237): phi_i =
+// sum_j q_j / r_ij over alive j != i with r^2 >= 1e-30; out[i] = q_i * phi_i (0 if dead).
+__global__ __launch_bounds__(256) void AllPairsPotentialF64Kernel(int n, const double4* __restrict__ src,
+                                                                   double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int it = min(i, n - 1);
+  const double4 t = src[it];
+  double phi = 0.0;
+  if (t.w != 0.0) {
+    for (int j = 0; j < n; ++j) {
+      const double4 s = src[j];
+      if (j == it || s.w == 0.0) continue;
+      const double rx = __dsub_rn(t.x, s.x), ry = __dsub_rn(t.y, s.y), rz = __dsub_rn(t.z, s.z);
+      const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
+      if (d2 < kMinSquaredDistance) continue;
+      phi = __dadd_rn(phi, __ddiv_rn(s.w, sqrt(d2)));
+    }
+  }
+  if (i < n) out[i] = __dmul_rn(t.w, phi);
+}
+
+// FP32 potential: part[chunk][i] = sum_{j != i} q_j / sqrt(r^2 + eps^2) in e/unit.
+__global__ __launch_bounds__(kAllPairsThreads) void AllPairsPotentialF32Kernel(const float4* __restrict__ src,
+                                                                                int n_pad, int chunk,
+                                                                                double* __restrict__ part) {
+  const int i = blockIdx.x * kAllPairsThreads + threadIdx.x;
+  const float4 th = src[2 * i];
+  const float4 tl = src[2 * i + 1];
+  const int j0 = blockIdx.y * chunk;
+  const int j1 = min(n_pad, j0 + chunk);
+  double acc = 0.0;
+  for (int jt = j0; jt < j1; jt += kTile) {
+    float p = 0.f;
+    const float4* s = src + 2 * jt;
+#pragma unroll 8
+    for (int k = 0; k < kTile; ++k) {
+      const float4 sh = s[2 * k], sl = s[2 * k + 1];
+      const float dx = (th.x - sh.x) + (tl.x - sl.x);
+      const float dy = (th.y - sh.y) + (tl.y - sl.y);
+      const float dz = (th.z - sh.z) + (tl.z - sl.z);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+      const float w = (jt + k == i) ? 0.f : sh.w;  // the self term q_i / eps would swamp FP32
+      p = fmaf(w, __builtin_amdgcn_rsqf(r2), p);
+    }
+    acc += (double)p;
+  }
+  part[(size_t)blockIdx.y * n_pad + i] = acc;
+}
+
+__global__ void AllPairsPotentialReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
+                                              const float4* __restrict__ src, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < chunks; ++c) s += part[(size_t)c * n_pad + i];
+  const float qe = src[2 * i].w;
+  out[i] = (double)qe * s;  // q_i phi_i in e^2/unit
+}
+
+static int ChunkCount(am_ctx* ctx, int target_blocks, int n_pad) {
+  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+  const int want = 8 * cus;  // >= 8 workgroups per CU
+  int chunks = (want + target_blocks - 1) / target_blocks;
+  chunks = std::max(1, std::min(chunks, n_pad / kTile));
+  return chunks;
+}
+
+
+int AllPairsForcesF32(am_store* s) {
+  am_ctx* ctx = s->ctx;
+  const int n = (int)s->n;
+  const int n_pad = PadN(n);
+  const int tblocks = n_pad / kAllPairsThreads;
+  const int chunks = ChunkCount(ctx, tblocks, n_pad);
+  int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
+  const int nch = (n_pad + chunk - 1) / chunk;
+  void* part = nullptr;
+  int rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part);
+  if (rc) return rc;
+  int slot;
+  TimerBegin(ctx, "allpairs_f32", &slot);
+  hipLaunchKernelGGL(AllPairsF32Kernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, ctx->stream, s->src, n_pad,
+                     chunk, (double*)part);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(AllPairsReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, n_pad, nch,
+                     (const double*)part, s->q, s->flags, s->fx, s->fy, s->fz);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+int AllPairsForcesF64(am_store* s) {
+  am_ctx* ctx = s->ctx;
+  const int n = (int)s->n;
+  void* src = nullptr;
+  int rc = Scratch(ctx, "src_f64", (size_t)n * sizeof(double4), &src);
+  if (rc) return rc;
+  hipLaunchKernelGGL(PackF64SourceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z, s->q,
+                     s->flags, (double4*)src);
+  AM_HIP(hipGetLastError());
+  int slot;
+  TimerBegin(ctx, "allpairs_f64", &slot);
+  hipLaunchKernelGGL(AllPairsF64Kernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, (const double4*)src,
+                     s->fx, s->fy, s->fz);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+// Per-particle q_i phi_i (SI: J when multiplied by k_C/2 and summed) into out[n] (device).
+int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_scale) {
+  am_ctx* ctx = s->ctx;
+  const int n = (int)s->n;
+  if (precision == AM_FP64) {
+    void* src = nullptr;
+    int rc = Scratch(ctx, "src_f64", (size_t)n * sizeof(double4), &src);
+    if (rc) return rc;
+    hipLaunchKernelGGL(PackF64SourceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
+                       s->q, s->flags, (double4*)src);
+    AM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(AllPairsPotentialF64Kernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n,
+                       (const double4*)src, out);
+    AM_HIP(hipGetLastError());
+    *si_scale = 1.0;
+    return AM_OK;
+  }
+  const int n_pad = PadN(n);
+  const int tblocks = n_pad / kAllPairsThreads;
+  const int chunks = ChunkCount(ctx, tblocks, n_pad);
+  int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
+  const int nch = (n_pad + chunk - 1) / chunk;
+  void* part = nullptr;
+  int rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(AllPairsPotentialF32Kernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, ctx->stream, s->src,
+                     n_pad, chunk, (double*)part);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(AllPairsPotentialReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, n_pad, nch,
+                     (const double*)part, s->src, out);
+  AM_HIP(hipGetLastError());
+  *si_scale = kElementaryCharge * kElementaryCharge * kUnitScale;  // e^2/unit -> C^2/m
+  return AM_OK;
+}
+
+}  // namespace am
+
+using namespace am;
+
+extern "C" int am_compute_allpairs_coulomb_forces(am_store* s, int precision) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_compute_allpairs_coulomb_forces: NULL store");
+  if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
+  if (s->n == 0) return AM_OK;
+  if (s->n > (int64_t)1 << 28) return Fail(AM_ERR_CAPACITY, "all-pairs: n too large");
+  AM_HIP(hipSetDevice(s->ctx->device));
+  return precision == AM_FP32 ? AllPairsForcesF32(s) : AllPairsForcesF64(s);
+}

@@ -1,0 +1,391 @@
+// Uniform cell lists and the kernels built on them:
+//   * ComputeStandardCoulombForces (src/interaction/coulomb_interaction.cc:23-80): pairs with
+//     r^2 < r_c^2 (the radius search of the reference's neighbour query), FP64.
+//   * the shifted-cutoff standard energy (src/analysis/energy_analysis.cc:16-43), r^2 <= r_c^2.
+//   * AnnihilationInteraction (src/interaction/annihilation.cc:21-61), canonical order.
+//
+// A cell list sorts particles by the 63-bit key (cx, cy, cz) of their cell of edge h (slightly
+// larger than the interaction radius, so every pair within the radius lies in the 27-cell
+// stencil) with the stable radix sort; a query binary-searches each stencil cell's range.
+// Cell coordinates are clamped to +-2^20 cells (clamping is 1-Lipschitz, so pairs within the
+// radius stay within the stencil; only far-flung particles share boundary cells).
+#include <algorithm>
+#include <cmath>
+
+#include "am_internal.hpp"
+
+namespace am {
+
+constexpr uint64_t kDeadKey = ~0ull;  // annihilated particles sort last and match no cell
+
+__device__ inline int64_t CellCoord(double v, double inv_h) {
+  const double c = floor(v * inv_h);
+  if (!(c == c)) return 1 << 20;  // NaN
+  const double lo = -(double)(1 << 20), hi = (double)(1 << 20) - 1.0;
+  return (int64_t)fmin(fmax(c, lo), hi) + (1 << 20);
+}
+
+__device__ inline uint64_t CellKey(int64_t cx, int64_t cy, int64_t cz) {
+  return ((uint64_t)cx << 42) | ((uint64_t)cy << 21) | (uint64_t)cz;
+}
+
+__global__ void CellKeyKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, const uint8_t* __restrict__ flags, double inv_h,
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  idx[i] = (uint32_t)i;
+  if (flags[i] & kFlagAnnihilated) {
+    keys[i] = kDeadKey;
+    return;
+  }
+  keys[i] = CellKey(CellCoord(x[i], inv_h), CellCoord(y[i], inv_h), CellCoord(z[i], inv_h));
+}
+
+__device__ inline int64_t LowerBound(const uint64_t* __restrict__ keys, int64_t n, uint64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Calls f(j) for every particle j in the 27 cells around particle i's cell (including i).
+template <typename F>
+__device__ inline void ForEachStencil(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx, int64_t n,
+                                      double px, double py, double pz, double inv_h, F&& f) {
+  const int64_t cx = CellCoord(px, inv_h), cy = CellCoord(py, inv_h), cz = CellCoord(pz, inv_h);
+  const int64_t lim = (int64_t)1 << 21;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int64_t z = cz + dz;
+    if (z < 0 || z >= lim) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t y = cy + dy;
+      if (y < 0 || y >= lim) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int64_t x = cx + dx;
+        if (x < 0 || x >= lim) continue;
+        const uint64_t key = CellKey(x, y, z);
+        for (int64_t k = LowerBound(keys, n, key); k < n && keys[k] == key; ++k) f((int64_t)idx[k]);
+      }
+    }
+  }
+}
+
+struct CellList {
+  uint64_t* keys = nullptr;
+  uint32_t* idx = nullptr;
+  double inv_h = 0.0;
+};
+
+static int BuildCellList(am_store* s, double radius, const char* tag, CellList* cl) {
+  am_ctx* ctx = s->ctx;
+  const int64_t n = s->n;
+  std::string kname = std::string(tag) + "_keys", iname = std::string(tag) + "_idx";
+  void *k, *v;
+  int rc;
+  if ((rc = Scratch(ctx, kname.c_str(), (size_t)n * 8, &k))) return rc;
+  if ((rc = Scratch(ctx, iname.c_str(), (size_t)n * 4, &v))) return rc;
+  cl->keys = (uint64_t*)k;
+  cl->idx = (uint32_t*)v;
+  cl->inv_h = 1.0 / (radius * (1.0 + 1e-9));
+  hipLaunchKernelGGL(CellKeyKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z, s->flags,
+                     cl->inv_h, cl->keys, cl->idx);
+  AM_HIP(hipGetLastError());
+  return RadixSortPairs(ctx, cl->keys, cl->idx, n, 64);
+}
+
+// coulomb_interaction.cc:60-73 (pair term r_ij * (k q_i q_j / d2 / distance)).
+__global__ __launch_bounds__(256) void StandardForceKernel(int64_t n, const double* __restrict__ x,
+                                                           const double* __restrict__ y, const double* __restrict__ z,
+                                                           const double* __restrict__ q,
+                                                           const uint8_t* __restrict__ flags,
+                                                           const uint64_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ idx, double inv_h, double rc2,
+                                                           double* __restrict__ fx, double* __restrict__ fy,
+                                                           double* __restrict__ fz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const double qi = q[i];
+  const double px = x[i], py = y[i], pz = z[i];
+  const bool live = !(flags[i] & kFlagAnnihilated) && qi != 0.0 && isfinite(px) && isfinite(py) && isfinite(pz);
+  if (live) {
+    ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+      if (j == i) return;
+      const double qj = q[j];
+      if (qj == 0.0) return;
+      const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+      const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
+      if (!(d2 < rc2)) return;
+      if (d2 < kMinSquaredDistance) return;
+      const double dist = sqrt(d2);
+      const double mag = kCoulomb * qi * qj / d2;
+      const double c = mag / dist;
+      ax = ax + rx * c;
+      ay = ay + ry * c;
+      az = az + rz * c;
+    });
+  }
+  if (!(flags[i] & kFlagAnnihilated)) {
+    fx[i] = ax;
+    fy[i] = ay;
+    fz[i] = az;
+  }
+}
+
+// energy_analysis.cc:22-41 over ordered pairs (the caller halves the sum): e_i = sum over
+// neighbours j != i with d2 <= rc^2 and d2 >= 1e-30 of k q_i q_j (1/r - 1/rc).
+__global__ __launch_bounds__(256) void StandardEnergyKernel(int64_t n, const double* __restrict__ x,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ z,
+                                                            const double* __restrict__ q,
+                                                            const uint8_t* __restrict__ flags,
+                                                            const uint64_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ idx, double inv_h, double rc,
+                                                            double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e = 0.0;
+  const double qi = q[i];
+  if (!(flags[i] & kFlagAnnihilated) && qi != 0.0) {
+    const double px = x[i], py = y[i], pz = z[i];
+    const double rc2 = rc * rc;
+    ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+      if (j == i) return;
+      const double qj = q[j];
+      if (qj == 0.0) return;
+      const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+      const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
+      if (d2 > rc2) return;
+      if (d2 < 1e-30) return;
+      e = e + kCoulomb * qi * qj * (1.0 / sqrt(d2) - 1.0 / rc);
+    });
+  }
+  out[i] = e;
+}
+
+__global__ void ZeroAliveForceKernel(int64_t n, const uint8_t* __restrict__ flags, double* __restrict__ fx,
+                                     double* __restrict__ fy, double* __restrict__ fz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (flags[i] & kFlagAnnihilated)) return;
+  fx[i] = 0.0;
+  fy[i] = 0.0;
+  fz[i] = 0.0;
+}
+
+int StandardEnergyTerms(am_store* s, double rc, double* out) {
+  CellList cl;
+  int rc_ = BuildCellList(s, rc, "cl_energy", &cl);
+  if (rc_) return rc_;
+  hipLaunchKernelGGL(StandardEnergyKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, s->ctx->stream, s->n, s->x, s->y,
+                     s->z, s->q, s->flags, cl.keys, cl.idx, cl.inv_h, rc, out);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+// ---- annihilation ------------------------------------------------------------------------------
+// Candidates of i: alive j != i, IsAntiparticlePair(species_i, species_j), d2 < r_a^2.
+template <typename F>
+__device__ inline void ForEachCandidate(int64_t i, int64_t n, const double* __restrict__ x,
+                                        const double* __restrict__ y, const double* __restrict__ z,
+                                        const uint8_t* __restrict__ species, const uint8_t* __restrict__ flags,
+                                        const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
+                                        double inv_h, double ra2, F&& f) {
+  const double px = x[i], py = y[i], pz = z[i];
+  const int si = species[i];
+  ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+    if (j == i || (flags[j] & kFlagAnnihilated)) return;
+    if (!IsPair(si, species[j])) return;
+    const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+    if (!(Dot3(rx, ry, rz, rx, ry, rz) < ra2)) return;
+    f(j);
+  });
+}
+
+__global__ void ContactCountKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ z, const uint8_t* __restrict__ species,
+                                   const uint8_t* __restrict__ flags, const uint64_t* __restrict__ keys,
+                                   const uint32_t* __restrict__ idx, double inv_h, double ra2,
+                                   uint32_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = 0;
+  if (!(flags[i] & kFlagAnnihilated))
+    ForEachCandidate(i, n, x, y, z, species, flags, keys, idx, inv_h, ra2, [&](int64_t) { ++c; });
+  count[i] = c;
+}
+
+// Writes i's candidates into [off[i], off[i+1]) sorted by j ascending (insertion sort of the
+// short segment), as (i, j) pairs.
+__global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ z, const uint8_t* __restrict__ species,
+                                   const uint8_t* __restrict__ flags, const uint64_t* __restrict__ keys,
+                                   const uint32_t* __restrict__ idx, double inv_h, double ra2,
+                                   const uint32_t* __restrict__ off, uint2* __restrict__ pairs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = off[i], e = off[i + 1];
+  if (b == e) return;
+  uint32_t w = b;
+  ForEachCandidate(i, n, x, y, z, species, flags, keys, idx, inv_h, ra2, [&](int64_t j) {
+    uint32_t k = w++;
+    while (k > b && pairs[k - 1].y > (uint32_t)j) {
+      pairs[k] = pairs[k - 1];
+      --k;
+    }
+    pairs[k] = make_uint2((uint32_t)i, (uint32_t)j);
+  });
+}
+
+constexpr int kResolveLds = 8192;  // candidate entries resolved from LDS
+
+// Greedy resolution in scan order.  entries: the (i, j) list sorted by (i, j); candidates are
+// symmetric, so the distinct i values are exactly the involved particles.  All threads first
+// compute for every entry whether its draw passes (U < p) and the local slot of j; thread 0
+// then walks the list once.  marked[] is indexed by local slot (position of the particle's
+// first entry's run in the list).
+__global__ __launch_bounds__(256) void ContactResolveKernel(const uint2* __restrict__ pairs,
+                                                            const uint32_t* __restrict__ total_ptr, double p,
+                                                            uint64_t seed, uint64_t step,
+                                                            uint8_t* __restrict__ flags, float4* __restrict__ src,
+                                                            unsigned int* __restrict__ n_marked,
+                                                            uint32_t* __restrict__ gslot,
+                                                            uint8_t* __restrict__ gpass,
+                                                            uint8_t* __restrict__ gmark) {
+  __shared__ uint32_t lslot[kResolveLds];
+  __shared__ uint8_t lpass[kResolveLds];
+  __shared__ uint8_t lmark[kResolveLds];
+  const uint32_t m = *total_ptr;
+  const bool use_lds = m <= (uint32_t)kResolveLds;
+  uint32_t* slot = use_lds ? lslot : gslot;
+  uint8_t* pass = use_lds ? lpass : gpass;
+  uint8_t* mark = use_lds ? lmark : gmark;
+  for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+    const uint2 pr = pairs[e];
+    // slot of particle pr.y = index of the first entry whose i == pr.y
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pairs[mid].x < pr.y) lo = mid + 1;
+      else hi = mid;
+    }
+    slot[e] = lo;
+    pass[e] = AnnihilationUniform(seed, step, pr.x, pr.y) < p ? 1 : 0;
+    mark[e] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int count = 0;
+    uint32_t e = 0;
+    while (e < m) {
+      const uint32_t i = pairs[e].x;
+      const uint32_t first = e;
+      uint32_t end = e;
+      while (end < m && pairs[end].x == i) ++end;
+      if (!mark[first]) {
+        for (uint32_t k = first; k < end; ++k) {
+          const uint32_t sj = slot[k];
+          if (mark[sj]) continue;
+          if (pass[k]) {
+            mark[first] = 1;
+            mark[sj] = 1;
+            count += 2;
+            break;
+          }
+        }
+      }
+      e = end;
+    }
+    *n_marked = count;
+  }
+  __syncthreads();
+  // Apply: the first entry of each run carries the particle's mark.
+  for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+    if (!mark[e]) continue;
+    const uint32_t i = pairs[e].x;
+    flags[i] |= kFlagAnnihilated;
+    src[2 * i].w = 0.f;
+  }
+}
+
+}  // namespace am
+
+using namespace am;
+
+extern "C" {
+
+int am_compute_standard_coulomb_forces(am_store* s, double cutoff_radius, int precision) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_compute_standard_coulomb_forces: NULL store");
+  if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
+  if (s->n == 0) return AM_OK;
+  AM_HIP(hipSetDevice(s->ctx->device));
+  am_ctx* ctx = s->ctx;
+  if (cutoff_radius <= 0.0) {  // coulomb_interaction.cc:33-37
+    hipLaunchKernelGGL(ZeroAliveForceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, s->flags,
+                       s->fx, s->fy, s->fz);
+    AM_HIP(hipGetLastError());
+    return AM_OK;
+  }
+  CellList cl;
+  int rc = BuildCellList(s, cutoff_radius, "cl_force", &cl);
+  if (rc) return rc;
+  int slot;
+  TimerBegin(ctx, "standard", &slot);
+  hipLaunchKernelGGL(StandardForceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, s->x, s->y, s->z,
+                     s->q, s->flags, cl.keys, cl.idx, cl.inv_h, cutoff_radius * cutoff_radius, s->fx, s->fy, s->fz);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+int am_annihilation(am_store* s, double contact_radius, double probability, uint64_t seed, uint64_t step,
+                    int64_t* n_annihilated) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_annihilation: NULL store");
+  if (n_annihilated) *n_annihilated = 0;
+  if (s->n == 0 || contact_radius <= 0.0) return AM_OK;  // annihilation.cc:38
+  AM_HIP(hipSetDevice(s->ctx->device));
+  am_ctx* ctx = s->ctx;
+  const int64_t n = s->n;
+  CellList cl;
+  int rc = BuildCellList(s, contact_radius, "cl_contact", &cl);
+  if (rc) return rc;
+  void *cnt, *off, *marked;
+  if ((rc = Scratch(ctx, "contact_count", (size_t)n * 4, &cnt))) return rc;
+  if ((rc = Scratch(ctx, "contact_off", (size_t)(n + 1) * 4, &off))) return rc;
+  if ((rc = Scratch(ctx, "contact_marked", 8, &marked))) return rc;
+  const double ra2 = contact_radius * contact_radius;
+  int slot;
+  TimerBegin(ctx, "annihilation", &slot);
+  hipLaunchKernelGGL(ContactCountKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
+                     s->species, s->flags, cl.keys, cl.idx, cl.inv_h, ra2, (uint32_t*)cnt);
+  AM_HIP(hipGetLastError());
+  if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)off, n))) return rc;
+  uint32_t total = 0;
+  AM_HIP(hipMemcpyAsync(&total, (uint32_t*)off + n, 4, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipStreamSynchronize(ctx->stream));
+  if (total > 0) {
+    void *pairs, *gslot, *gpass, *gmark;
+    if ((rc = Scratch(ctx, "contact_pairs", (size_t)total * 8, &pairs))) return rc;
+    if ((rc = Scratch(ctx, "contact_gslot", (size_t)total * 4, &gslot))) return rc;
+    if ((rc = Scratch(ctx, "contact_gpass", (size_t)total, &gpass))) return rc;
+    if ((rc = Scratch(ctx, "contact_gmark", (size_t)total, &gmark))) return rc;
+    hipLaunchKernelGGL(ContactWriteKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
+                       s->species, s->flags, cl.keys, cl.idx, cl.inv_h, ra2, (const uint32_t*)off, (uint2*)pairs);
+    AM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ContactResolveKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
+                       (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked,
+                       (uint32_t*)gslot, (uint8_t*)gpass, (uint8_t*)gmark);
+    AM_HIP(hipGetLastError());
+    unsigned int hm = 0;
+    AM_HIP(hipMemcpyAsync(&hm, marked, 4, hipMemcpyDeviceToHost, ctx->stream));
+    AM_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_annihilated) *n_annihilated = hm;
+  }
+  TimerEnd(ctx, slot);
+  return AM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+/*
+ * antimatter-kernel for MI355X — C ABI of the per-step hot path.
+ *
+ * Every entry point returns an int status (AM_OK = 0, negative on error; no exceptions cross
+ * the ABI) and leaves a message for am_last_error().  Arguments are plain pointers and sizes.
+ * Citations name the reference interface each entry point replaces (paths relative to the
+ * reference repository root).
+ *
+ * Objects:
+ *   am_ctx   — one GPU device and one HIP stream.  Calls on a context (and on the stores made
+ *              from it) must be serialised by the caller, as the reference's solvers are called
+ *              from the driver's single thread (src/antimatter-kernel.cc:153-197).
+ *   am_store — structure-of-arrays particle store resident in HBM; the GPU counterpart of the
+ *              ResourceManager's ChargedAntiparticle agents (src/agents/charged_antiparticle.h:14-79).
+ *              Particle i keeps index i for its whole life; annihilated particles stay in the
+ *              store, flagged, exactly as IsAnnihilated() agents are skipped by every solver.
+ */
+#ifndef ANTIMATTER_AMD_AM_API_H_
+#define ANTIMATTER_AMD_AM_API_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AM_OK 0
+#define AM_ERR_INVALID -1  /* invalid argument (mirrors std::invalid_argument in the reference) */
+#define AM_ERR_HIP -2      /* HIP runtime error */
+#define AM_ERR_CAPACITY -3 /* size exceeds a capacity */
+#define AM_ERR_STATE -4    /* call out of order / missing data */
+
+/* Arithmetic used by a solver's pair kernel. */
+#define AM_FP64 0 /* FP64 end to end (parity path) */
+#define AM_FP32 1 /* FP32 pair math on hi/lo-split coordinates, FP64 accumulation per tile */
+
+/* ParticleSpecies (src/agents/particle_species.h:9-17). */
+#define AM_ANTIPROTON 0
+#define AM_POSITRON 1
+#define AM_ANTIHYDROGEN 2
+#define AM_ELECTRON 3
+#define AM_PROTON 4
+#define AM_ANTIHELIUM 5
+
+typedef struct am_ctx am_ctx;
+typedef struct am_store am_store;
+
+/* PenningTrapField (src/fields/penning_trap.h:27-69): B = b0 z_hat,
+ * E = (U0 / 2 d^2)(x, y, -2 z), phi = (U0 / 4 d^2)(2 z^2 - rho^2). */
+typedef struct {
+  double b0;
+  double trap_voltage;
+  double d; /* characteristic dimension, d = sqrt(z0^2/2 + rho0^2/4) */
+} am_trap_field;
+
+/* EnergyBreakdown (src/analysis/energy_analysis.h:16-23). */
+typedef struct {
+  double kinetic;
+  double trap_potential;
+  double energy_potential;
+  int64_t num_active_particles;
+} am_energy_breakdown;
+
+/* EnergyHistory (src/analysis/energy_analysis.h:26-31). */
+typedef struct {
+  int32_t has_previous;
+  double prev_trap_potential;
+  double prev_coulomb_potential;
+} am_energy_history;
+
+/* ComputationMethod (src/params/interaction_params.h:11-16), plus all-pairs. */
+#define AM_METHOD_STANDARD 0
+#define AM_METHOD_BARNES_HUT 1
+#define AM_METHOD_FAST_MULTIPOLE 2
+#define AM_METHOD_ALLPAIRS 3 /* exact all-pairs: the theta = 0 limit of BH/FMM */
+
+/* EnergyComputationSettings (src/params/interaction_params.h:18-25). */
+typedef struct {
+  int32_t method;
+  double cutoff_radius;
+  double barnes_hut_theta;
+  double fast_multipole_theta;
+  double fast_multipole_order;
+} am_energy_settings;
+
+/* ---- errors / version ---------------------------------------------------------------------- */
+const char* am_last_error(void);
+int am_version(void);
+
+/* ---- field --------------------------------------------------------------------------------- */
+/* PenningTrapField::MakeFromGeometry (src/fields/penning_trap.cc:15-20). */
+int am_trap_make_from_geometry(double b0, double trap_voltage, double z0, double rho0, am_trap_field* out);
+/* CyclotronCooling::CoolingTime (src/forces/cooling.cc:16-22). */
+double am_cooling_time(double charge, double mass, double b_mag);
+/* GetDefaultSpeciesProperties (src/agents/particle_species.h:27-46). */
+int am_species_properties(int species, double* mass, double* charge);
+
+/* ---- initial conditions (host) --------------------------------------------------------------- */
+/* PenningTrapField::SetParticlePositions (src/fields/penning_trap.cc:118-161): counts[s] particles
+ * of species s in enum order; out: 3 * sum(counts) doubles.  Uses libstdc++ <random>. */
+int am_set_particle_positions(const uint64_t counts[6], const double radii[3], int seed, double* out);
+/* PenningTrapField::SetParticleVelocities (src/fields/penning_trap.cc:51-115). */
+int am_set_particle_velocities(const uint64_t counts[6], const double temperatures[6], int seed, double* out);
+/* PhysicsParams dt rule (src/params/physics_params.cc:115-129): listed[s] != 0 for species named
+ * in particle_populations. */
+double am_derive_dt(const int32_t listed[6], double b0);
+
+/* ---- context ------------------------------------------------------------------------------- */
+int am_ctx_create(int device, am_ctx** out);
+int am_ctx_destroy(am_ctx* ctx);
+int am_ctx_synchronize(am_ctx* ctx);
+/* Kernel timing: when enabled, every solver/push launch is bracketed by HIP events on the
+ * context's stream; am_ctx_kernel_time returns the summed milliseconds and launch count of
+ * the named kernel family ("allpairs_f32", "allpairs_f64", "push", "bh_walk", ...). */
+int am_ctx_enable_timing(am_ctx* ctx, int enable);
+int am_ctx_kernel_time(am_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+int am_ctx_reset_timing(am_ctx* ctx);
+
+/* ---- particle store -------------------------------------------------------------------------- */
+int am_store_create(am_ctx* ctx, int64_t capacity, am_store** out);
+int am_store_destroy(am_store* store);
+/* Upload n particles (ChargedAntiparticle ctor, src/agents/charged_antiparticle.cc:5-15).
+ * pos/vel: 3n doubles (x0,y0,z0,x1,...).  species: n bytes.  charge/mass: n doubles or NULL
+ * (NULL = species defaults).  annihilated: n bytes or NULL.  trap: n uint32 or NULL (trap id
+ * for ensembles, NULL = all 0).  Resets the Coulomb force to zero and the Lorentz
+ * back-stagger flags (lorentz.h:34) to "not yet staggered". */
+int am_store_upload(am_store* store, int64_t n, const double* pos, const double* vel, const uint8_t* species,
+                    const double* charge, const double* mass, const uint8_t* annihilated, const uint32_t* trap);
+/* Download any subset (NULL skips): pos/vel/force 3n doubles, annihilated/staggered n bytes. */
+int am_store_download(am_store* store, double* pos, double* vel, double* coulomb_force, uint8_t* annihilated,
+                      uint8_t* staggered);
+/* Overwrite the Coulomb force buffer (SetCoulombForce for all agents); 3n doubles. */
+int am_store_set_coulomb_force(am_store* store, const double* force);
+int am_store_count(am_store* store, int64_t* n, int64_t* n_active);
+
+/* ---- Coulomb solvers: write the store's Coulomb force for every alive particle ---------------- */
+/* ComputeStandardCoulombForces (src/interaction/coulomb_interaction.h:10, .cc:23-80): pairs with
+ * r^2 < r_c^2, cell list of cell size r_c.  cutoff_radius <= 0 sets all forces to zero. */
+int am_compute_standard_coulomb_forces(am_store* store, double cutoff_radius, int precision);
+/* Exact all-pairs force without cutoff: the theta = 0 limit of ComputeBarnesHutCoulombForces
+ * (src/interaction/barnes-hut.h:22-24) with the AddPointCharge pair term (barnes-hut.cc:19-27). */
+int am_compute_allpairs_coulomb_forces(am_store* store, int precision);
+/* ComputeBarnesHutCoulombForces (src/interaction/barnes-hut.h:76, .cc:271-299): dual-monopole
+ * octree, leaf capacity 8, MAC (2h)^2 < theta^2 |x - c|^2 against the geometric centre. */
+int am_compute_barnes_hut_coulomb_forces(am_store* store, double theta, int precision);
+/* ComputeFastMultipoleCoulombForces (src/interaction/fast-multipole.h:73, .cc:684-713). */
+int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double theta);
+
+/* ---- behaviours ---------------------------------------------------------------------------- */
+/* LorentzForce::Run for every alive agent (src/forces/lorentz.cc:37-71): F = qE + F_coul,
+ * first-call back-stagger v <- Boris(v, F, -dt/2), v <- Boris(v, F, dt), x += v dt, F_coul <- 0. */
+int am_lorentz_force(am_store* store, const am_trap_field* field, double dt);
+/* CyclotronCooling::Run for every alive agent (src/forces/cooling.cc:24-65).  The Langevin
+ * noise (bath_temp_K > 0) draws counter-based normals keyed on (seed, step, particle index). */
+int am_cyclotron_cooling(am_store* store, const am_trap_field* field, double tau_scale, double bath_temp_K,
+                         double dt, uint64_t seed, uint64_t step);
+/* LorentzForce then CyclotronCooling fused in one kernel (the per-agent behaviour order of
+ * src/antimatter-kernel.cc:93-94); tau_scale <= 0 disables cooling (cooling.cc:35). */
+int am_push(am_store* store, const am_trap_field* field, double dt, double tau_scale, double bath_temp_K,
+            uint64_t seed, uint64_t step);
+/* AnnihilationInteraction::Run for every agent (src/interaction/annihilation.cc:21-61) in the
+ * canonical order: on current positions, scan i ascending; candidates j ascending (alive,
+ * opposite matter per IsAnnihilationPair, r^2 < r_a^2); the first candidate whose draw
+ * U(seed, step, i, j) < p marks both.  n_annihilated (optional) receives the number marked. */
+int am_annihilation(am_store* store, double contact_radius, double probability, uint64_t seed, uint64_t step,
+                    int64_t* n_annihilated);
+/* IsAntiparticlePair (src/interaction/annihilation.h:9, .cc:12-19). */
+int am_is_antiparticle_pair(int species_a, int species_b);
+
+/* ---- energy ---------------------------------------------------------------------------------- */
+/* ComputeTotalEnergy (src/analysis/energy_analysis.h:35, .cc:84-145): KE = sum m v^2 / 2,
+ * PE_trap = sum q phi(x), PE_coul by settings->method (standard: shifted cutoff potential,
+ * r <= r_c; BH / FMM potential; all-pairs), FP64 reductions.  history (optional) time-centres
+ * the potentials with the previous call. */
+int am_compute_total_energy(am_store* store, const am_trap_field* field, const am_energy_settings* settings,
+                            am_energy_history* history, am_energy_breakdown* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANTIMATTER_AMD_AM_API_H_ */
