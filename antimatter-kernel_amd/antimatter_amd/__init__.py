@@ -90,6 +90,9 @@ SIGNATURES = {
     "am_compute_standard_coulomb_forces": (_int, [_vp, _d, _int]),
     "am_compute_allpairs_coulomb_forces": (_int, [_vp, _int]),
     "am_compute_barnes_hut_coulomb_forces": (_int, [_vp, _d, _int]),
+    "am_barnes_hut_forces_counted": (_int, [_vp, _d, _int, _vp]),
+    "am_bh_tree_stats": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
     "am_lorentz_force": (_int, [_vp, _vp, _d]),
     "am_cyclotron_cooling": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
@@ -263,6 +266,27 @@ class Store:
 
     def compute_barnes_hut_coulomb_forces(self, theta: float, precision: int = FP32):
         _check(lib().am_compute_barnes_hut_coulomb_forces(self.h, theta, precision))
+
+    def barnes_hut_forces_counted(self, theta: float, precision: int = FP32):
+        n = self.count()[0]
+        c = np.zeros(n, dtype=np.int64)
+        _check(lib().am_barnes_hut_forces_counted(self.h, theta, precision, _p(c)))
+        return c
+
+    def bh_tree(self):
+        """Tree of the most recent Barnes-Hut build (reference DFS preorder)."""
+        m, nn, md, ov = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().am_bh_tree_stats(self.h, ctypes.byref(m), ctypes.byref(nn), ctypes.byref(md),
+                                      ctypes.byref(ov)))
+        order = np.zeros(m.value, np.uint32)
+        b = np.zeros(nn.value, np.int32)
+        e = np.zeros(nn.value, np.int32)
+        leaf = np.zeros(nn.value, np.uint8)
+        qp = np.zeros(nn.value)
+        qn = np.zeros(nn.value)
+        _check(lib().am_bh_tree_download(self.h, _p(order), _p(b), _p(e), _p(leaf), _p(qp), _p(qn)))
+        return dict(order=order, begin=b, end=e, leaf=leaf, qp=qp, qn=qn, max_depth=md.value,
+                    deep_overflow=ov.value)
 
     def compute_fast_multipole_coulomb_forces(self, order: int, theta: float):
         _check(lib().am_compute_fast_multipole_coulomb_forces(self.h, int(order), theta))

@@ -143,6 +143,19 @@ int am_compute_allpairs_coulomb_forces(am_store* store, int precision);
 /* ComputeBarnesHutCoulombForces (src/interaction/barnes-hut.h:76, .cc:271-299): dual-monopole
  * octree, leaf capacity 8, MAC (2h)^2 < theta^2 |x - c|^2 against the geometric centre. */
 int am_compute_barnes_hut_coulomb_forces(am_store* store, double theta, int precision);
+/* ComputeBarnesHutCoulombForces plus the per-particle interaction count of BarnesHutTree::Accumulate
+ * (barnes-hut.cc:183-221): leaf pairs + accepted monopoles, n int64 into counts (host). */
+int am_barnes_hut_forces_counted(am_store* store, double theta, int precision, int64_t* counts);
+/* BarnesHutTree::NumNodes (barnes-hut.h:46) and build statistics of the most recent Barnes-Hut build:
+ * alive particles, nodes, deepest level, and depth-21 leaves holding more than 8 particles (the
+ * reference, whose depth cap is 32, would split those further). */
+int am_bh_tree_stats(am_store* store, int64_t* n_alive, int64_t* n_nodes, int32_t* max_depth,
+                     int64_t* deep_overflow);
+/* BarnesHutTree::order_ and nodes_ (barnes-hut.h:49-73) of the most recent build, in the reference's
+ * DFS preorder: order[n_alive] particle indices; per node begin/end/is_leaf and the two monopole
+ * charges (any pointer may be NULL). */
+int am_bh_tree_download(am_store* store, uint32_t* order, int32_t* begin, int32_t* end, uint8_t* leaf, double* qp,
+                        double* qn);
 /* ComputeFastMultipoleCoulombForces (src/interaction/fast-multipole.h:73, .cc:684-713). */
 int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double theta);
 

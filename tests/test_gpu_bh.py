@@ -1,0 +1,101 @@
+"""Barnes-Hut on the GPU against the oracle restatement of barnes-hut.cc."""
+import numpy as np
+import pytest
+
+import antimatter_amd as am
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def cloud(n, seed=21, frac_neutral=0.05, frac_dead=0.04):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (n, 3)) * np.array([1e-3, 1e-3, 4e-3])
+    sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    sp[rng.random(n) < frac_neutral] = 2
+    ann = (rng.random(n) < frac_dead).astype(np.uint8)
+    return pos, sp, ann
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture
+def store(ctx):
+    s = am.Store(ctx, 20000)
+    yield s
+    s.close()
+
+
+def test_tree_order_and_leaves_match_reference(store):
+    pos, sp, ann = cloud(6000)
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    store.compute_barnes_hut_coulomb_forces(0.5, am.FP64)
+    t = store.bh_tree()
+    alive = np.where(ann == 0)[0]
+    m, q = oracle.mass_charge(sp)
+    ref = oracle.bh_tree(pos[alive], q[alive], 8)
+    # gathered index -> particle index
+    np.testing.assert_array_equal(t["order"], alive[ref["order"]])
+    np.testing.assert_array_equal(t["begin"], ref["begin"])
+    np.testing.assert_array_equal(t["end"], ref["end"])
+    np.testing.assert_array_equal(t["leaf"], ref["leaf"])
+    np.testing.assert_array_equal(t["qp"], ref["qp"])
+    np.testing.assert_array_equal(t["qn"], ref["qn"])
+    assert t["deep_overflow"] == 0
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 0.9])
+def test_fp64_forces_and_counts_bit_identical(store, theta):
+    pos, sp, ann = cloud(4000, seed=22)
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    cnt = store.barnes_hut_forces_counted(theta, am.FP64)
+    f = store.forces()
+    m, q = oracle.mass_charge(sp)
+    ref, rcnt = oracle.bh_forces(pos, q, ann, theta, 8, want_counts=True)
+    live = ann == 0
+    np.testing.assert_array_equal(cnt[live], rcnt[live])
+    np.testing.assert_array_equal(f[live], ref[live])
+
+
+def test_fp32_theta0_is_allpairs_and_theta_error(store):
+    pos, sp, ann = cloud(8000, seed=23, frac_neutral=0.0, frac_dead=0.0)
+    store.upload(pos, np.zeros_like(pos), sp)
+    m, q = oracle.mass_charge(sp)
+    ap = oracle.allpairs_forces(pos, q)
+    store.compute_barnes_hut_coulomb_forces(0.0, am.FP32)
+    assert rel_l2(store.forces(), ap) < 1e-5
+    store.compute_barnes_hut_coulomb_forces(0.7, am.FP32)
+    f32 = store.forces()
+    ref = oracle.bh_forces(pos, q, None, 0.7)
+    assert rel_l2(f32, ref) < 1e-5  # FP32 arithmetic error on top of the method error
+    err = rel_l2(ref, ap)
+    print(f"BH theta=0.7 method error vs all-pairs: {err:.3e}")
+    assert 1e-3 < err < 5e-2
+
+
+def test_potential_energy_matches_oracle(store):
+    pos, sp, ann = cloud(3000, seed=24)
+    vel = np.random.default_rng(1).normal(0, 1e3, pos.shape)
+    store.upload(pos, vel, sp, annihilated=ann)
+    trap = am.make_trap(1.0, 10.0, 0.01, 0.01)
+    m, q = oracle.mass_charge(sp)
+    for theta in (0.0, 0.6):
+        e = store.compute_total_energy(trap, "barnes-hut", barnes_hut_theta=theta)
+        ref = oracle.bh_potential_energy(pos, q, ann, theta)
+        assert e.energy_potential == pytest.approx(ref, rel=1e-12)
+
+
+def test_degenerate_inputs(store):
+    # one particle, all particles at one point, all annihilated
+    store.upload(np.zeros((1, 3)), np.zeros((1, 3)), np.array([1], np.uint8))
+    store.compute_barnes_hut_coulomb_forces(0.5, am.FP32)
+    assert np.all(store.forces() == 0)
+    pos = np.full((20, 3), 1e-4)
+    sp = np.array([1, 3] * 10, np.uint8)
+    store.upload(pos, np.zeros_like(pos), sp)
+    store.compute_barnes_hut_coulomb_forces(0.5, am.FP64)
+    assert np.all(store.forces() == 0)  # coincident pairs are skipped (r^2 < 1e-30)
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=np.ones(20, np.uint8))
+    store.compute_barnes_hut_coulomb_forces(0.5, am.FP32)
