@@ -94,6 +94,7 @@ SIGNATURES = {
     "am_bh_tree_stats": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
+    "am_fmm_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "am_lorentz_force": (_int, [_vp, _vp, _d]),
     "am_cyclotron_cooling": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
     "am_push": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
@@ -290,6 +291,13 @@ class Store:
 
     def compute_fast_multipole_coulomb_forces(self, order: int, theta: float):
         _check(lib().am_compute_fast_multipole_coulomb_forces(self.h, int(order), theta))
+
+    def fmm_stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        d = ctypes.c_int32()
+        _check(lib().am_fmm_stats(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(d)))
+        return dict(sources=v[0].value, cells=v[1].value, m2l_pairs=v[2].value, p2p_cell_pairs=v[3].value,
+                    max_depth=d.value)
 
     # ---- behaviours ----
     def lorentz_force(self, field: TrapField, dt: float):

@@ -160,6 +160,8 @@ int64_t CountActive(am_store* s, int* status);
 // Radix sort (stable, LSD) of 64-bit keys with 32-bit values.  keys/vals are sorted in place
 // (ping-pong through scratch).  key_bits: number of low bits that may be non-zero.
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
+// Deterministic FP64 sum of n device doubles into *host_out (synchronises the stream).
+int ReduceSum(am_ctx* ctx, const double* in, int64_t n, double* host_out);
 // Exclusive prefix sum of n uint32 counts into out (n+1 entries; out[n] = total).
 int ExclusiveScan(am_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n);
 

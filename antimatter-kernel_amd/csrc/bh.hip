@@ -27,71 +27,13 @@
 #include <cmath>
 #include <vector>
 
-#include "am_internal.hpp"
+#include "octree.hpp"
 
 namespace am {
 
-constexpr int kMaxLevel = 21;
-
-struct RootBox {
-  double cx, cy, cz, half;
-};
-
-// ---- gather / bbox / keys ---------------------------------------------------------------------
-__global__ void AliveFlagKernel(int64_t n, const uint8_t* __restrict__ flags, uint32_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (flags[i] & kFlagAnnihilated) ? 0u : 1u;
-}
-
-__global__ void CompactKernel(int64_t n, const uint8_t* __restrict__ flags, const uint32_t* __restrict__ off,
-                              uint32_t* __restrict__ gathered) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && !(flags[i] & kFlagAnnihilated)) gathered[off[i]] = (uint32_t)i;
-}
-
-constexpr int kBoxThreads = 256;
-constexpr int kBoxBlocks = 256;
-
-__global__ __launch_bounds__(kBoxThreads) void BBoxPartialKernel(int m, const uint32_t* __restrict__ idx,
-                                                                 const double* __restrict__ x,
-                                                                 const double* __restrict__ y,
-                                                                 const double* __restrict__ z,
-                                                                 double* __restrict__ part) {
-  __shared__ double sh[6][kBoxThreads];
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int k = blockIdx.x * kBoxThreads + threadIdx.x; k < m; k += gridDim.x * kBoxThreads) {
-    const uint32_t i = idx[k];
-    const double p[3] = {x[i], y[i], z[i]};
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], p[d]);
-      hi[d] = fmax(hi[d], p[d]);
-    }
-  }
-  for (int d = 0; d < 3; ++d) {
-    sh[d][threadIdx.x] = lo[d];
-    sh[3 + d][threadIdx.x] = hi[d];
-  }
-  __syncthreads();
-  for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int d = 0; d < 3; ++d) {
-        sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + w]);
-        sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + w]);
-      }
-    __syncthreads();
-  }
-  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-// Root cube of BarnesHutTree::Build (barnes-hut.cc:47-62).
-__global__ void BHRootKernel(int blocks, const double* __restrict__ part, RootBox* __restrict__ root,
-                             double* __restrict__ box_out) {
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < blocks; ++b)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], part[b * 6 + d]);
-      hi[d] = fmax(hi[d], part[b * 6 + 3 + d]);
-    }
+// Root cube of BarnesHutTree::Build (barnes-hut.cc:47-62) from the gathered bounding box.
+__global__ void BHRootKernel(const double* __restrict__ box, RootBox* __restrict__ root) {
+  const double lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
   RootBox r;
   r.cx = 0.5 * (lo[0] + hi[0]);
   r.cy = 0.5 * (lo[1] + hi[1]);
@@ -101,125 +43,6 @@ __global__ void BHRootKernel(int blocks, const double* __restrict__ part, RootBo
   if (half < 1e-30) half = 1e-30;
   r.half = half;
   *root = r;
-  for (int d = 0; d < 3; ++d) {
-    box_out[d] = lo[d];
-    box_out[3 + d] = hi[d];
-  }
-}
-
-// Morton key by repeating the reference descent: octant test against the current centre, then
-// move the centre by +-half/2 in FP64 (barnes-hut.cc:30-33,124-136).
-__global__ void MortonKernel(int m, const uint32_t* __restrict__ idx, const double* __restrict__ x,
-                             const double* __restrict__ y, const double* __restrict__ z,
-                             const RootBox* __restrict__ root, uint64_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const uint32_t i = idx[k];
-  const double px = x[i], py = y[i], pz = z[i];
-  double cx = root->cx, cy = root->cy, cz = root->cz, h = root->half;
-  uint64_t key = 0;
-  for (int l = 0; l < kMaxLevel; ++l) {
-    const int o = (px >= cx ? 1 : 0) | (py >= cy ? 2 : 0) | (pz >= cz ? 4 : 0);
-    key = (key << 3) | (uint64_t)o;
-    const double ch = 0.5 * h;
-    cx = cx + ((o & 1) ? ch : -ch);
-    cy = cy + ((o & 2) ? ch : -ch);
-    cz = cz + ((o & 4) ? ch : -ch);
-    h = ch;
-  }
-  keys[k] = key;
-  vals[k] = i;
-}
-
-// ---- level-synchronous build ------------------------------------------------------------------
-struct NodeArrays {  // BFS order
-  int* b;
-  int* e;
-  int* depth;
-  int* first_child;
-  int* nchild;
-  uint8_t* leaf;
-  double* cx;
-  double* cy;
-  double* cz;
-  double* half;
-};
-
-__device__ inline int Digit(uint64_t key, int level) { return (int)((key >> (3 * (kMaxLevel - 1 - level))) & 7u); }
-
-__global__ void SplitCountKernel(int n0, int n1, NodeArrays nd, const uint64_t* __restrict__ keys, int leaf_cap,
-                                 int* __restrict__ bounds, uint32_t* __restrict__ cnt) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int node = n0 + t;
-  if (node >= n1) return;
-  const int b = nd.b[node], e = nd.e[node], d = nd.depth[node];
-  int* bd = bounds + 9 * t;
-  if (e - b <= leaf_cap || d >= kMaxLevel) {
-    nd.leaf[node] = 1;
-    nd.nchild[node] = 0;
-    nd.first_child[node] = -1;
-    cnt[t] = 0;
-    return;
-  }
-  nd.leaf[node] = 0;
-  bd[0] = b;
-  for (int o = 1; o < 8; ++o) {  // first position whose digit at level d is >= o
-    int lo = bd[o - 1], hi = e;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (Digit(keys[mid], d) < o) lo = mid + 1;
-      else hi = mid;
-    }
-    bd[o] = lo;
-  }
-  bd[8] = e;
-  uint32_t c = 0;
-  for (int o = 0; o < 8; ++o) c += bd[o + 1] > bd[o] ? 1u : 0u;
-  cnt[t] = c;
-}
-
-__global__ void WriteChildrenKernel(int n0, int n1, int next_base, NodeArrays nd, const int* __restrict__ bounds,
-                                    const uint32_t* __restrict__ off) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int node = n0 + t;
-  if (node >= n1 || nd.leaf[node]) return;
-  const int* bd = bounds + 9 * t;
-  int c = next_base + (int)off[t];
-  nd.first_child[node] = c;
-  int k = 0;
-  const double h = nd.half[node];
-  const double ch = 0.5 * h;
-  const double px = nd.cx[node], py = nd.cy[node], pz = nd.cz[node];
-  for (int o = 0; o < 8; ++o) {
-    if (bd[o + 1] == bd[o]) continue;
-    const int ci = c + k++;
-    nd.b[ci] = bd[o];
-    nd.e[ci] = bd[o + 1];
-    nd.depth[ci] = nd.depth[node] + 1;
-    nd.cx[ci] = px + ((o & 1) ? ch : -ch);
-    nd.cy[ci] = py + ((o & 2) ? ch : -ch);
-    nd.cz[ci] = pz + ((o & 4) ? ch : -ch);
-    nd.half[ci] = ch;
-  }
-  nd.nchild[node] = k;
-}
-
-// Leaves keep particle-index order (the reference's stable partition stops at the leaf).
-__global__ void LeafSortKernel(int nn, NodeArrays nd, uint32_t* __restrict__ vals, uint32_t* __restrict__ over) {
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= nn || !nd.leaf[node]) return;
-  const int b = nd.b[node], e = nd.e[node];
-  for (int k = b + 1; k < e; ++k) {
-    const uint32_t v = vals[k];
-    int j = k - 1;
-    while (j >= b && vals[j] > v) {
-      vals[j + 1] = vals[j];
-      --j;
-    }
-    vals[j + 1] = v;
-  }
-  if (nd.depth[node] >= kMaxLevel && e - b > 8) atomicAdd(over, 1u);
 }
 
 struct Moments {  // BFS order
@@ -300,13 +123,6 @@ __global__ void MomentKernel(int n0, int n1, NodeArrays nd, Moments mo, const ui
     mo.cny[node] = nd.cy[node];
     mo.cnz[node] = nd.cz[node];
   }
-}
-
-__global__ void PreorderKeyKernel(int nn, NodeArrays nd, uint64_t* __restrict__ keys, uint32_t* __restrict__ ids) {
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= nn) return;
-  keys[node] = ((uint64_t)nd.b[node] << 5) | (uint64_t)nd.depth[node];
-  ids[node] = (uint32_t)node;
 }
 
 // Walk-ready node record in DFS preorder.
@@ -519,6 +335,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 
 // ---- orchestration ------------------------------------------------------------------------------
 struct BHTree {
+  Octree tree;
   int m = 0;            // alive particles
   int nn = 0;           // nodes
   uint32_t* order = nullptr;  // sorted position -> particle index
@@ -528,175 +345,58 @@ struct BHTree {
   uint32_t deep_overflow = 0;
 };
 
-static int Alloc(am_ctx* ctx, const char* name, size_t bytes, void** p) { return Scratch(ctx, name, bytes, p); }
-
 int BuildBHTree(am_store* s, int leaf_cap, BHTree* t) {
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = s->n;
   int rc;
-  void *alive, *off, *gathered, *part, *root, *box, *keys, *vals;
-  if ((rc = Alloc(ctx, "bh_alive", (size_t)n * 4, &alive))) return rc;
-  if ((rc = Alloc(ctx, "bh_off", (size_t)(n + 1) * 4, &off))) return rc;
-  if ((rc = Alloc(ctx, "bh_gathered", (size_t)n * 4, &gathered))) return rc;
-  if ((rc = Alloc(ctx, "bh_part", (size_t)kBoxBlocks * 6 * 8, &part))) return rc;
-  if ((rc = Alloc(ctx, "bh_root", sizeof(RootBox) + 6 * 8, &root))) return rc;
-  box = (char*)root + sizeof(RootBox);
-  hipLaunchKernelGGL(AliveFlagKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->flags, (uint32_t*)alive);
-  AM_HIP(hipGetLastError());
-  if ((rc = ExclusiveScan(ctx, (const uint32_t*)alive, (uint32_t*)off, n))) return rc;
-  uint32_t m = 0;
-  AM_HIP(hipMemcpyAsync(&m, (uint32_t*)off + n, 4, hipMemcpyDeviceToHost, st));
-  AM_HIP(hipStreamSynchronize(st));
-  t->m = (int)m;
+  void *gathered, *root;
+  if ((rc = Scratch(ctx, "bh_gathered", (size_t)n * 4, &gathered))) return rc;
+  if ((rc = Scratch(ctx, "bh_root", sizeof(RootBox) + 6 * 8, &root))) return rc;
+  double* box = (double*)((char*)root + sizeof(RootBox));
+  int m = 0;
+  if ((rc = GatherParticles(ctx, n, s->flags, s->q, 0, (uint32_t*)gathered, &m, "bh"))) return rc;
+  t->m = m;
   t->nn = 0;
   if (m == 0) return AM_OK;
-  hipLaunchKernelGGL(CompactKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->flags, (const uint32_t*)off,
-                     (uint32_t*)gathered);
-  const int bb = std::min(kBoxBlocks, Blocks(m, kBoxThreads));
-  hipLaunchKernelGGL(BBoxPartialKernel, dim3(bb), dim3(kBoxThreads), 0, st, (int)m, (const uint32_t*)gathered, s->x,
-                     s->y, s->z, (double*)part);
-  hipLaunchKernelGGL(BHRootKernel, dim3(1), dim3(1), 0, st, bb, (const double*)part, (RootBox*)root, (double*)box);
-  if ((rc = Alloc(ctx, "bh_keys", (size_t)m * 8, &keys))) return rc;
-  if ((rc = Alloc(ctx, "bh_vals", (size_t)m * 4, &vals))) return rc;
-  hipLaunchKernelGGL(MortonKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, (int)m, (const uint32_t*)gathered, s->x,
-                     s->y, s->z, (const RootBox*)root, (uint64_t*)keys, (uint32_t*)vals);
+  if ((rc = GatheredBBox(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, box, "bh"))) return rc;
+  hipLaunchKernelGGL(BHRootKernel, dim3(1), dim3(1), 0, st, (const double*)box, (RootBox*)root);
   AM_HIP(hipGetLastError());
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)keys, (uint32_t*)vals, m, 3 * kMaxLevel))) return rc;
-
-  // Node arrays with a generous capacity; grown (with copy) when a level would overflow.
-  size_t cap = (size_t)3 * m + 1024;
-  std::vector<void*> arrays;
-  NodeArrays nd;
-  auto alloc_nodes = [&](size_t c) -> int {
-    void* p;
-    int r;
-    const char* names[] = {"bh_nb", "bh_ne", "bh_nd", "bh_nfc", "bh_nnc", "bh_nleaf", "bh_ncx", "bh_ncy", "bh_ncz",
-                           "bh_nhalf"};
-    const size_t sz[] = {4, 4, 4, 4, 4, 1, 8, 8, 8, 8};
-    void** dst[] = {(void**)&nd.b, (void**)&nd.e, (void**)&nd.depth, (void**)&nd.first_child, (void**)&nd.nchild,
-                    (void**)&nd.leaf, (void**)&nd.cx, (void**)&nd.cy, (void**)&nd.cz, (void**)&nd.half};
-    for (int a = 0; a < 10; ++a) {
-      if ((r = Alloc(ctx, names[a], c * sz[a], &p))) return r;
-      *dst[a] = p;
-    }
-    return AM_OK;
-  };
-  if ((rc = alloc_nodes(cap))) return rc;
-  // root node (BFS index 0)
-  {
-    int zero = 0, mm = (int)m;
-    AM_HIP(hipMemcpyAsync(nd.b, &zero, 4, hipMemcpyHostToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.e, &mm, 4, hipMemcpyHostToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.depth, &zero, 4, hipMemcpyHostToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.cx, &((RootBox*)root)->cx, 8, hipMemcpyDeviceToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.cy, &((RootBox*)root)->cy, 8, hipMemcpyDeviceToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.cz, &((RootBox*)root)->cz, 8, hipMemcpyDeviceToDevice, st));
-    AM_HIP(hipMemcpyAsync(nd.half, &((RootBox*)root)->half, 8, hipMemcpyDeviceToDevice, st));
-    AM_HIP(hipStreamSynchronize(st));  // host ints above go out of scope
-  }
-  std::vector<int> level_start = {0, 1};
-  int total = 1;
-  for (int d = 0; d <= kMaxLevel; ++d) {
-    const int n0 = level_start[d], n1 = level_start[d + 1];
-    const int cnt_nodes = n1 - n0;
-    if (cnt_nodes == 0) break;
-    void *bounds, *cnt, *coff;
-    if ((rc = Alloc(ctx, "bh_bounds", (size_t)cnt_nodes * 9 * 4, &bounds))) return rc;
-    if ((rc = Alloc(ctx, "bh_cnt", (size_t)cnt_nodes * 4, &cnt))) return rc;
-    if ((rc = Alloc(ctx, "bh_coff", (size_t)(cnt_nodes + 1) * 4, &coff))) return rc;
-    hipLaunchKernelGGL(SplitCountKernel, dim3(Blocks(cnt_nodes, 256)), dim3(256), 0, st, n0, n1, nd,
-                       (const uint64_t*)keys, leaf_cap, (int*)bounds, (uint32_t*)cnt);
-    AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)coff, cnt_nodes))) return rc;
-    uint32_t kids = 0;
-    AM_HIP(hipMemcpyAsync(&kids, (uint32_t*)coff + cnt_nodes, 4, hipMemcpyDeviceToHost, st));
-    AM_HIP(hipStreamSynchronize(st));
-    if (kids == 0) break;
-    if ((size_t)total + kids > cap) {  // grow with copy
-      const size_t ncap = std::max(cap * 2, (size_t)total + kids + 1024);
-      NodeArrays old = nd;
-      std::vector<std::pair<void*, size_t>> saved;
-      const size_t sz[] = {4, 4, 4, 4, 4, 1, 8, 8, 8, 8};
-      void* olds[] = {old.b, old.e, old.depth, old.first_child, old.nchild, old.leaf, old.cx, old.cy, old.cz, old.half};
-      for (int a = 0; a < 10; ++a) {
-        void* tmp;
-        AM_HIP(hipMallocAsync(&tmp, total * sz[a], st));
-        AM_HIP(hipMemcpyAsync(tmp, olds[a], total * sz[a], hipMemcpyDeviceToDevice, st));
-        saved.push_back({tmp, total * sz[a]});
-      }
-      if ((rc = alloc_nodes(ncap))) return rc;
-      void* news[] = {nd.b, nd.e, nd.depth, nd.first_child, nd.nchild, nd.leaf, nd.cx, nd.cy, nd.cz, nd.half};
-      for (int a = 0; a < 10; ++a) {
-        AM_HIP(hipMemcpyAsync(news[a], saved[a].first, saved[a].second, hipMemcpyDeviceToDevice, st));
-        AM_HIP(hipFreeAsync(saved[a].first, st));
-      }
-      cap = ncap;
-    }
-    hipLaunchKernelGGL(WriteChildrenKernel, dim3(Blocks(cnt_nodes, 256)), dim3(256), 0, st, n0, n1, total, nd,
-                       (const int*)bounds, (const uint32_t*)coff);
-    AM_HIP(hipGetLastError());
-    total += (int)kids;
-    level_start.push_back(total);
-  }
-  // Mark the deepest level (never split) as leaves.
-  {
-    const int n0 = level_start[level_start.size() - 2], n1 = level_start.back();
-    if (n1 > n0) {
-      void *bounds, *cnt;
-      if ((rc = Alloc(ctx, "bh_bounds", (size_t)(n1 - n0) * 9 * 4, &bounds))) return rc;
-      if ((rc = Alloc(ctx, "bh_cnt", (size_t)(n1 - n0) * 4, &cnt))) return rc;
-      hipLaunchKernelGGL(SplitCountKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, nd,
-                         (const uint64_t*)keys, (int)m + 1, (int*)bounds, (uint32_t*)cnt);
-      AM_HIP(hipGetLastError());
-    }
-  }
-  const int nn = total;
-  const int nlevels = (int)level_start.size() - 1;
-  void* over;
-  if ((rc = Alloc(ctx, "bh_over", 4, &over))) return rc;
-  AM_HIP(hipMemsetAsync(over, 0, 4, st));
-  hipLaunchKernelGGL(LeafSortKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, nd, (uint32_t*)vals,
-                     (uint32_t*)over);
-  AM_HIP(hipGetLastError());
+  Octree& o = t->tree;
+  if ((rc = BuildOctree(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, (const RootBox*)root, leaf_cap, "bh",
+                        &o)))
+    return rc;
+  const int nn = o.nn;
+  const int nlevels = (int)o.level_start.size() - 1;
   Moments mo;
   {
     const char* names[] = {"bh_qp", "bh_qn", "bh_cpx", "bh_cpy", "bh_cpz", "bh_cnx", "bh_cny", "bh_cnz"};
     double** dst[] = {&mo.qp, &mo.qn, &mo.cpx, &mo.cpy, &mo.cpz, &mo.cnx, &mo.cny, &mo.cnz};
     for (int a = 0; a < 8; ++a) {
       void* p;
-      if ((rc = Alloc(ctx, names[a], (size_t)nn * 8, &p))) return rc;
+      if ((rc = Scratch(ctx, names[a], (size_t)nn * 8, &p))) return rc;
       *dst[a] = (double*)p;
     }
   }
   for (int d = nlevels - 1; d >= 0; --d) {
-    const int n0 = level_start[d], n1 = level_start[d + 1];
+    const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
     if (n1 <= n0) continue;
-    hipLaunchKernelGGL(MomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, nd, mo,
-                       (const uint32_t*)vals, s->x, s->y, s->z, s->q);
+    hipLaunchKernelGGL(MomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd, mo,
+                       (const uint32_t*)o.order, s->x, s->y, s->z, s->q);
     AM_HIP(hipGetLastError());
   }
-  void *pkeys, *pids, *wn, *wn32;
-  if ((rc = Alloc(ctx, "bh_pkeys", (size_t)nn * 8, &pkeys))) return rc;
-  if ((rc = Alloc(ctx, "bh_pids", (size_t)nn * 4, &pids))) return rc;
-  if ((rc = Alloc(ctx, "bh_walknodes", (size_t)nn * sizeof(WalkNode), &wn))) return rc;
-  if ((rc = Alloc(ctx, "bh_walknodes32", (size_t)nn * sizeof(WalkNodeF32), &wn32))) return rc;
-  hipLaunchKernelGGL(PreorderKeyKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, nd, (uint64_t*)pkeys,
-                     (uint32_t*)pids);
+  void *wn, *wn32;
+  if ((rc = Scratch(ctx, "bh_walknodes", (size_t)nn * sizeof(WalkNode), &wn))) return rc;
+  if ((rc = Scratch(ctx, "bh_walknodes32", (size_t)nn * sizeof(WalkNodeF32), &wn32))) return rc;
+  hipLaunchKernelGGL(GatherPreorderKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, o.nd, mo,
+                     (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeF32*)wn32);
   AM_HIP(hipGetLastError());
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)pkeys, (uint32_t*)pids, nn, 40))) return rc;
-  hipLaunchKernelGGL(GatherPreorderKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, nd, mo,
-                     (const uint64_t*)pkeys, (const uint32_t*)pids, (WalkNode*)wn, (WalkNodeF32*)wn32);
-  AM_HIP(hipGetLastError());
-  uint32_t ov = 0;
-  AM_HIP(hipMemcpyAsync(&ov, over, 4, hipMemcpyDeviceToHost, st));
-  AM_HIP(hipStreamSynchronize(st));
   t->nn = nn;
-  t->order = (uint32_t*)vals;
+  t->order = o.order;
   t->nodes = (WalkNode*)wn;
   t->nodes32 = (WalkNodeF32*)wn32;
   t->max_depth = nlevels - 1;
-  t->deep_overflow = ov;
+  t->deep_overflow = o.deep_overflow;
   return AM_OK;
 }
 

@@ -1,12 +1,802 @@
-// Fast multipole solver (placeholder until the device FMM lands).
-#include "am_internal.hpp"
+// Cartesian-Taylor fast multipole method (src/interaction/fast-multipole.cc:17-713) on the device.
+//
+// Same method, same tree and same operators as the reference; the parallel structure is new:
+//   * rescale to the unit cube (fast-multipole.cc:355-390) and the count-capped octree (leaf 32)
+//     from Morton keys (octree.hpp), cells numbered in the reference's DFS preorder;
+//   * radii bottom-up, P2M (one wave per leaf, one lane per multi-index), M2M bottom-up level by
+//     level (one wave per parent), both in the reference's per-coefficient summation order;
+//   * the serial dual-tree traversal (:484-526) becomes a level-synchronous expansion of (T, S)
+//     pairs: each pair is accepted for M2L ((r_T + r_S)^2 < theta^2 d^2), kept for P2P (two
+//     leaves) or split exactly as the reference splits it, so the M2L and P2P sets are the
+//     reference's; the lists are then sorted by target cell;
+//   * M2L: one wave per target cell; per source cell the derivative tensor D_alpha(t - c) of
+//     1/r is built in LDS degree by degree with the reference recurrence (:207-232), then lane
+//     gamma contracts L_gamma += (1/gamma!) sum_alpha (-1)^|alpha| M_alpha D_{alpha+gamma};
+//   * L2L top-down level by level; L2P fused with P2P (one wave per leaf: 32 lanes per half of
+//     the source list), unscaled to newtons (:663-674).
+// All FP64.  Summation order differs from the reference only where the traversal order does
+// (M2L and P2P list order), so results agree with it to rounding.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "octree.hpp"
 
 namespace am {
-int FastMultipolePotentialEnergy(am_store* s, int order, double theta, double* pe) {
-  return Fail(AM_ERR_STATE, "FMM potential not available in this build");
+
+constexpr int kFmmMaxOrder = 8;
+constexpr int kFmmMaxNm = (kFmmMaxOrder + 1) * (kFmmMaxOrder + 2) * (kFmmMaxOrder + 3) / 6;  // 165
+constexpr int kFmmMaxNd = (2 * kFmmMaxOrder + 1) * (2 * kFmmMaxOrder + 2) * (2 * kFmmMaxOrder + 3) / 6;  // 969
+
+// Device view of the expansion tables (FmmTables, fast-multipole.cc:41-178).
+struct FmmTab {
+  int P, nm, nd;
+  const int* multi;      // [nd][3]
+  const double* fact;    // [nd]
+  const double* inv_fact;
+  const double* sign;
+  const int* rec;        // [nd][10]: axis, delta[3], minus1[3], minus2[3]
+  const int* deg_start;  // [2P+2]: first index of each total degree
+  const int* m2l;        // [nm][nm]
+  const int* m2m_off;    // [nm+1] CSR by target coefficient a
+  const int* m2m_bc;     // [..][2]: (b, c) in the reference list order
+  const int* l2l_off;
+  const int* l2l_bc;
+  const int* grad;       // [nm][3]
+};
+
+struct HostTab {
+  int P, nm, nd;
+  std::vector<int> multi, rec, deg_start, m2l, m2m_off, m2m_bc, l2l_off, l2l_bc, grad;
+  std::vector<double> fact, inv_fact, sign;
+};
+
+// Tables built on the host with the reference's definitions (fast-multipole.cc:78-177).
+static HostTab MakeTables(int p) {
+  HostTab t;
+  t.P = p;
+  const int size = 2 * p + 1;
+  std::vector<int> lookup((size_t)size * size * size, -1);
+  std::vector<std::array<int, 3>> multi;
+  t.deg_start.clear();
+  for (int deg = 0; deg <= 2 * p; ++deg) {
+    t.deg_start.push_back((int)multi.size());
+    for (int a = deg; a >= 0; --a)
+      for (int b = deg - a; b >= 0; --b) {
+        lookup[((size_t)a * size + b) * size + (deg - a - b)] = (int)multi.size();
+        multi.push_back({a, b, deg - a - b});
+      }
+  }
+  t.deg_start.push_back((int)multi.size());
+  auto idx = [&](int a, int b, int c) {
+    if (a < 0 || b < 0 || c < 0 || a >= size || b >= size || c >= size) return -1;
+    return lookup[((size_t)a * size + b) * size + c];
+  };
+  t.nd = (int)multi.size();
+  t.nm = (p + 1) * (p + 2) * (p + 3) / 6;
+  for (auto& m : multi) t.multi.insert(t.multi.end(), m.begin(), m.end());
+  std::vector<double> f(2 * p + 1, 1.0);
+  for (int i = 1; i <= 2 * p; ++i) f[i] = f[i - 1] * i;
+  for (int k = 0; k < t.nd; ++k) {
+    const auto& m = multi[k];
+    const double fk = f[m[0]] * f[m[1]] * f[m[2]];
+    t.fact.push_back(fk);
+    t.inv_fact.push_back(1.0 / fk);
+    t.sign.push_back(((m[0] + m[1] + m[2]) % 2 == 0) ? 1.0 : -1.0);
+  }
+  t.rec.assign((size_t)t.nd * 10, -1);
+  for (int k = 1; k < t.nd; ++k) {
+    const auto& m = multi[k];
+    int* r = &t.rec[(size_t)k * 10];
+    r[0] = m[0] > 0 ? 0 : (m[1] > 0 ? 1 : 2);
+    for (int j = 0; j < 3; ++j) {
+      r[1 + j] = m[j];
+      std::array<int, 3> q = m;
+      if (m[j] >= 1) { q[j] -= 1; r[4 + j] = idx(q[0], q[1], q[2]); }
+      if (m[j] >= 2) { q[j] -= 1; r[7 + j] = idx(q[0], q[1], q[2]); }
+    }
+  }
+  t.m2l.resize((size_t)t.nm * t.nm);
+  for (int g = 0; g < t.nm; ++g)
+    for (int a = 0; a < t.nm; ++a)
+      t.m2l[(size_t)g * t.nm + a] = idx(multi[a][0] + multi[g][0], multi[a][1] + multi[g][1], multi[a][2] + multi[g][2]);
+  std::vector<std::vector<std::array<int, 2>>> m2m(t.nm), l2l(t.nm);
+  for (int a = 0; a < t.nm; ++a)
+    for (int b = 0; b < t.nm; ++b) {
+      if (multi[b][0] <= multi[a][0] && multi[b][1] <= multi[a][1] && multi[b][2] <= multi[a][2])
+        m2m[a].push_back({b, idx(multi[a][0] - multi[b][0], multi[a][1] - multi[b][1], multi[a][2] - multi[b][2])});
+      if (multi[a][0] <= multi[b][0] && multi[a][1] <= multi[b][1] && multi[a][2] <= multi[b][2])
+        l2l[a].push_back({b, idx(multi[b][0] - multi[a][0], multi[b][1] - multi[a][1], multi[b][2] - multi[a][2])});
+    }
+  t.m2m_off.push_back(0);
+  t.l2l_off.push_back(0);
+  for (int a = 0; a < t.nm; ++a) {
+    for (auto& pr : m2m[a]) { t.m2m_bc.push_back(pr[0]); t.m2m_bc.push_back(pr[1]); }
+    for (auto& pr : l2l[a]) { t.l2l_bc.push_back(pr[0]); t.l2l_bc.push_back(pr[1]); }
+    t.m2m_off.push_back((int)t.m2m_bc.size() / 2);
+    t.l2l_off.push_back((int)t.l2l_bc.size() / 2);
+  }
+  t.grad.resize((size_t)t.nm * 3);
+  for (int g = 0; g < t.nm; ++g)
+    for (int k = 0; k < 3; ++k) {
+      std::array<int, 3> q = multi[g];
+      t.grad[(size_t)g * 3 + k] = q[k] >= 1 ? (q[k] -= 1, idx(q[0], q[1], q[2])) : -1;
+    }
+  return t;
 }
+
+// Upload (cached per context and order) and return the device view.
+static int DeviceTables(am_ctx* ctx, int p, FmmTab* out) {
+  const HostTab t = MakeTables(p);
+  std::vector<char> blob;
+  auto put = [&](const void* src, size_t bytes) {
+    size_t off = (blob.size() + 15) / 16 * 16;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  const size_t o_multi = put(t.multi.data(), t.multi.size() * 4);
+  const size_t o_fact = put(t.fact.data(), t.fact.size() * 8);
+  const size_t o_ifact = put(t.inv_fact.data(), t.inv_fact.size() * 8);
+  const size_t o_sign = put(t.sign.data(), t.sign.size() * 8);
+  const size_t o_rec = put(t.rec.data(), t.rec.size() * 4);
+  const size_t o_deg = put(t.deg_start.data(), t.deg_start.size() * 4);
+  const size_t o_m2l = put(t.m2l.data(), t.m2l.size() * 4);
+  const size_t o_m2mo = put(t.m2m_off.data(), t.m2m_off.size() * 4);
+  const size_t o_m2mb = put(t.m2m_bc.data(), t.m2m_bc.size() * 4);
+  const size_t o_l2lo = put(t.l2l_off.data(), t.l2l_off.size() * 4);
+  const size_t o_l2lb = put(t.l2l_bc.data(), t.l2l_bc.size() * 4);
+  const size_t o_grad = put(t.grad.data(), t.grad.size() * 4);
+  const std::string name = "fmm_tables_" + std::to_string(p);
+  const bool fresh = ctx->scratch.find(name) == ctx->scratch.end();
+  void* dev;
+  int rc = Scratch(ctx, name.c_str(), blob.size(), &dev);
+  if (rc) return rc;
+  if (fresh) {
+    AM_HIP(hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
+    AM_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  char* d = (char*)dev;
+  *out = FmmTab{t.P, t.nm, t.nd, (const int*)(d + o_multi), (const double*)(d + o_fact), (const double*)(d + o_ifact),
+                (const double*)(d + o_sign), (const int*)(d + o_rec), (const int*)(d + o_deg),
+                (const int*)(d + o_m2l), (const int*)(d + o_m2mo), (const int*)(d + o_m2mb),
+                (const int*)(d + o_l2lo), (const int*)(d + o_l2lb), (const int*)(d + o_grad)};
+  return AM_OK;
+}
+
+struct FmmFrame {
+  double mid[3];
+  double length;
+  double force_scale;  // k_C / L^2
+  double min_sq;       // 1e-30 / L^2
+};
+
+// Unit-cube rescaling (fast-multipole.cc:355-390).
+__global__ void FrameKernel(const double* __restrict__ box, FmmFrame* __restrict__ fr, RootBox* __restrict__ root) {
+  const double lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
+  const double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  const double L = fmax(ext, 1e-15);
+  FmmFrame f;
+  for (int d = 0; d < 3; ++d) f.mid[d] = 0.5 * (lo[d] + hi[d]);
+  f.length = L;
+  f.force_scale = kCoulomb / (L * L);
+  f.min_sq = kMinSquaredDistance / (L * L);
+  *fr = f;
+  RootBox r;
+  r.cx = 0.0;
+  r.cy = 0.0;
+  r.cz = 0.0;
+  r.half = 0.5 * (1.0 + 1e-9);
+  *root = r;
+}
+
+__global__ void RescaleKernel(int m, const uint32_t* __restrict__ idx, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z,
+                              const FmmFrame* __restrict__ fr, double* __restrict__ rx, double* __restrict__ ry,
+                              double* __restrict__ rz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = idx[k];
+  const double L = fr->length;
+  rx[i] = (x[i] - fr->mid[0]) / L;
+  ry[i] = (y[i] - fr->mid[1]) / L;
+  rz[i] = (z[i] - fr->mid[2]) / L;
+}
+
+// Tree-ordered particle copies (fast-multipole.cc:401-410).
+__global__ void SortedCopyKernel(int m, const uint32_t* __restrict__ order, const double* __restrict__ rx,
+                                 const double* __restrict__ ry, const double* __restrict__ rz,
+                                 const double* __restrict__ q, double4* __restrict__ sp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = order[k];
+  sp[k] = make_double4(rx[i], ry[i], rz[i], q[i]);
+}
+
+struct Cell {  // preorder cell record
+  double cx, cy, cz, radius;
+  int b, e, leaf, nchild;
+  int child[8];
+};
+
+// Radii bottom-up (fast-multipole.cc:412-438), in BFS order.
+__global__ void RadiusKernel(int n0, int n1, NodeArrays nd, const double4* __restrict__ sp, double* __restrict__ rad) {
+  const int node = n0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= n1) return;
+  const double cx = nd.cx[node], cy = nd.cy[node], cz = nd.cz[node];
+  if (nd.leaf[node]) {
+    double r2 = 0.0;
+    for (int k = nd.b[node]; k < nd.e[node]; ++k) {
+      const double dx = sp[k].x - cx, dy = sp[k].y - cy, dz = sp[k].z - cz;
+      r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    rad[node] = sqrt(r2);
+  } else {
+    double r = 0.0;
+    const int c0 = nd.first_child[node];
+    for (int c = c0; c < c0 + nd.nchild[node]; ++c) {
+      const double dx = nd.cx[c] - cx, dy = nd.cy[c] - cy, dz = nd.cz[c] - cz;
+      r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz) + rad[c]);
+    }
+    rad[node] = r;
+  }
+}
+
+__global__ void GatherCellsKernel(int nn, NodeArrays nd, const uint32_t* __restrict__ pre,
+                                  const uint32_t* __restrict__ bfs2pre, const double* __restrict__ rad,
+                                  Cell* __restrict__ cells) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nn) return;
+  const int node = (int)pre[p];
+  Cell c;
+  c.cx = nd.cx[node];
+  c.cy = nd.cy[node];
+  c.cz = nd.cz[node];
+  c.radius = rad[node];
+  c.b = nd.b[node];
+  c.e = nd.e[node];
+  c.leaf = nd.leaf[node];
+  c.nchild = nd.nchild[node];
+  for (int k = 0; k < 8; ++k) c.child[k] = k < c.nchild ? (int)bfs2pre[nd.first_child[node] + k] : -1;
+  cells[p] = c;
+}
+
+__device__ inline double Monomial(const FmmTab& t, int k, double x, double y, double z) {
+  const int a = t.multi[3 * k], b = t.multi[3 * k + 1], c = t.multi[3 * k + 2];
+  double px = 1.0, py = 1.0, pz = 1.0;
+  for (int i = 0; i < a; ++i) px = px * x;
+  for (int i = 0; i < b; ++i) py = py * y;
+  for (int i = 0; i < c; ++i) pz = pz * z;
+  return px * py * pz;
+}
+
+// P2M (fast-multipole.cc:443-461): one wave per cell, lane = coefficient.
+__global__ __launch_bounds__(64) void P2MKernel(int nn, const Cell* __restrict__ cells, const double4* __restrict__ sp,
+                                                FmmTab t, double* __restrict__ M) {
+  const int p = blockIdx.x;
+  const Cell c = cells[p];
+  const int a = threadIdx.x;
+  if (!c.leaf) {
+    for (int k = a; k < t.nm; k += 64) M[(size_t)p * t.nm + k] = 0.0;
+    return;
+  }
+  for (int k = a; k < t.nm; k += 64) {
+    double acc = 0.0;
+    for (int j = c.b; j < c.e; ++j) {
+      const double4 s = sp[j];
+      if (s.w == 0.0) continue;
+      acc = acc + s.w * Monomial(t, k, s.x - c.cx, s.y - c.cy, s.z - c.cz) * t.inv_fact[k];
+    }
+    M[(size_t)p * t.nm + k] = acc;
+  }
+}
+
+// M2M (fast-multipole.cc:463-482) for the parents of one BFS level.
+__global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uint32_t* __restrict__ bfs2pre,
+                                                FmmTab t, double* __restrict__ M) {
+  __shared__ double mono[kFmmMaxNm];
+  const int node = n0 + blockIdx.x;
+  if (nd.leaf[node]) return;
+  const int p = (int)bfs2pre[node];
+  const int c0 = nd.first_child[node];
+  double acc[3] = {0.0, 0.0, 0.0};  // up to 3 coefficients per lane (nm <= 165 < 192)
+  for (int k = 0; k < 3; ++k) {
+    const int a = threadIdx.x + 64 * k;
+    if (a < t.nm) acc[k] = M[(size_t)p * t.nm + a];
+  }
+  for (int c = c0; c < c0 + nd.nchild[node]; ++c) {
+    const double dx = nd.cx[c] - nd.cx[node], dy = nd.cy[c] - nd.cy[node], dz = nd.cz[c] - nd.cz[node];
+    __syncthreads();
+    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = Monomial(t, k, dx, dy, dz);
+    __syncthreads();
+    const double* mc = M + (size_t)bfs2pre[c] * t.nm;
+    for (int k = 0; k < 3; ++k) {
+      const int a = threadIdx.x + 64 * k;
+      if (a >= t.nm) continue;
+      double v = acc[k];
+      for (int q = t.m2m_off[a]; q < t.m2m_off[a + 1]; ++q) {
+        const int b = t.m2m_bc[2 * q], cc = t.m2m_bc[2 * q + 1];
+        v = v + mc[b] * mono[cc] * t.inv_fact[cc];
+      }
+      acc[k] = v;
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const int a = threadIdx.x + 64 * k;
+    if (a < t.nm) M[(size_t)p * t.nm + a] = acc[k];
+  }
+}
+
+// One expansion step of the dual-tree traversal (fast-multipole.cc:494-525).  kind[i]: 0 split,
+// 1 M2L, 2 P2P; cnt[i] = number of child pairs produced.
+__global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
+                                    double theta2, uint32_t* __restrict__ n_m2l, uint32_t* __restrict__ n_p2p,
+                                    uint32_t* __restrict__ n_new) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint2 pr = pairs[i];
+  const Cell& T = cells[pr.x];
+  const Cell& S = cells[pr.y];
+  const double dx = T.cx - S.cx, dy = T.cy - S.cy, dz = T.cz - S.cz;
+  const double dist2 = dx * dx + dy * dy + dz * dz;
+  const double rs = T.radius + S.radius;
+  uint32_t a = 0, b = 0, c = 0;
+  if (rs * rs < theta2 * dist2) a = 1;
+  else if (T.leaf && S.leaf) b = 1;
+  else {
+    const bool split_t = S.leaf || (!T.leaf && T.radius >= S.radius);
+    c = split_t ? (uint32_t)T.nchild : (uint32_t)S.nchild;
+  }
+  n_m2l[i] = a;
+  n_p2p[i] = b;
+  n_new[i] = c;
+}
+
+__global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
+                                    double theta2, const uint32_t* __restrict__ o_m2l,
+                                    const uint32_t* __restrict__ o_p2p, const uint32_t* __restrict__ o_new,
+                                    uint64_t* __restrict__ m2l_out, uint64_t* __restrict__ p2p_out,
+                                    uint2* __restrict__ next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint2 pr = pairs[i];
+  const Cell& T = cells[pr.x];
+  const Cell& S = cells[pr.y];
+  const double dx = T.cx - S.cx, dy = T.cy - S.cy, dz = T.cz - S.cz;
+  const double dist2 = dx * dx + dy * dy + dz * dz;
+  const double rs = T.radius + S.radius;
+  const uint64_t key = ((uint64_t)pr.x << 32) | pr.y;
+  if (rs * rs < theta2 * dist2) {
+    m2l_out[o_m2l[i]] = key;
+  } else if (T.leaf && S.leaf) {
+    p2p_out[o_p2p[i]] = key;
+  } else {
+    const bool split_t = S.leaf || (!T.leaf && T.radius >= S.radius);
+    uint2* out = next + o_new[i];
+    if (split_t)
+      for (int k = 0; k < T.nchild; ++k) out[k] = make_uint2((uint32_t)T.child[k], pr.y);
+    else
+      for (int k = 0; k < S.nchild; ++k) out[k] = make_uint2(pr.x, (uint32_t)S.child[k]);
+  }
+}
+
+// CSR offsets of target cells in a (T << 32 | S)-sorted pair list.
+__global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int64_t np, int64_t* __restrict__ off) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nn) return;
+  const uint64_t k = (uint64_t)t << 32;
+  int64_t lo = 0, hi = np;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  off[t] = lo;
+}
+
+// M2L (fast-multipole.cc:528-555): one wave per target cell.
+__global__ __launch_bounds__(64) void M2LKernel(int nn, const Cell* __restrict__ cells,
+                                                const uint64_t* __restrict__ list, const int64_t* __restrict__ off,
+                                                FmmTab t, const double* __restrict__ M, double* __restrict__ L) {
+  __shared__ double D[kFmmMaxNd];
+  __shared__ double Ms[kFmmMaxNm];
+  const int p = blockIdx.x;
+  const int64_t b = off[p], e = off[p + 1];
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (b == e) {
+    for (int k = 0; k < 3; ++k) {
+      const int g = threadIdx.x + 64 * k;
+      if (g < t.nm) L[(size_t)p * t.nm + g] = 0.0;
+    }
+    return;
+  }
+  const Cell T = cells[p];
+  for (int64_t q = b; q < e; ++q) {
+    const int s = (int)(uint32_t)list[q];
+    const Cell& S = cells[s];
+    const double xs[3] = {T.cx - S.cx, T.cy - S.cy, T.cz - S.cz};
+    __syncthreads();
+    for (int k = threadIdx.x; k < t.nm; k += 64) Ms[k] = M[(size_t)s * t.nm + k];
+    const double inv_r2 = 1.0 / (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
+    if (threadIdx.x == 0) D[0] = sqrt(inv_r2);
+    for (int g = 1; g <= 2 * t.P; ++g) {
+      __syncthreads();
+      for (int k = t.deg_start[g] + threadIdx.x; k < t.deg_start[g + 1]; k += 64) {
+        const int* r = t.rec + 10 * k;
+        const int i = r[0];
+        double sum = (2 * r[1 + i] - 1) * xs[i] * D[r[4 + i]];
+        if (r[7 + i] >= 0) sum += (r[1 + i] - 1) * (r[1 + i] - 1) * D[r[7 + i]];
+        for (int j = 0; j < 3; ++j) {
+          if (j == i || r[1 + j] == 0) continue;
+          sum += 2.0 * r[1 + j] * xs[j] * D[r[4 + j]];
+          if (r[7 + j] >= 0) sum += r[1 + j] * (r[1 + j] - 1) * D[r[7 + j]];
+        }
+        D[k] = -sum * inv_r2;
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+      const int g = threadIdx.x + 64 * k;
+      if (g >= t.nm) continue;
+      const int* ix = t.m2l + (size_t)g * t.nm;
+      double a = 0.0;
+      for (int al = 0; al < t.nm; ++al) a += t.sign[al] * Ms[al] * D[ix[al]];
+      acc[k] += a * t.inv_fact[g];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const int g = threadIdx.x + 64 * k;
+    if (g < t.nm) L[(size_t)p * t.nm + g] = acc[k];
+  }
+}
+
+// L2L (fast-multipole.cc:557-578) for the parents of one BFS level.
+__global__ __launch_bounds__(64) void L2LKernel(int n0, NodeArrays nd, const uint32_t* __restrict__ bfs2pre, FmmTab t,
+                                                double* __restrict__ L) {
+  __shared__ double mono[kFmmMaxNm];
+  __shared__ double lp[kFmmMaxNm];
+  const int node = n0 + blockIdx.x;
+  if (nd.leaf[node]) return;
+  const int p = (int)bfs2pre[node];
+  for (int k = threadIdx.x; k < t.nm; k += 64) lp[k] = L[(size_t)p * t.nm + k];
+  const int c0 = nd.first_child[node];
+  for (int c = c0; c < c0 + nd.nchild[node]; ++c) {
+    const double dx = nd.cx[c] - nd.cx[node], dy = nd.cy[c] - nd.cy[node], dz = nd.cz[c] - nd.cz[node];
+    __syncthreads();
+    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = Monomial(t, k, dx, dy, dz);
+    __syncthreads();
+    double* lc = L + (size_t)bfs2pre[c] * t.nm;
+    for (int a = threadIdx.x; a < t.nm; a += 64) {
+      double v = lc[a];
+      for (int q = t.l2l_off[a]; q < t.l2l_off[a + 1]; ++q) {
+        const int bb = t.l2l_bc[2 * q], cc = t.l2l_bc[2 * q + 1];
+        v += lp[bb] * t.fact[bb] * t.inv_fact[a] * t.inv_fact[cc] * mono[cc];
+      }
+      lc[a] = v;
+    }
+  }
+}
+
+// L2P + P2P (fast-multipole.cc:580-661) and unscaling (:663-674): one wave per leaf; lane l owns
+// particle b + (l & 31) of the current 32-particle chunk and the sources of parity l >> 5.
+__global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restrict__ cells,
+                                                   const uint64_t* __restrict__ list,
+                                                   const int64_t* __restrict__ off, FmmTab t,
+                                                   const double* __restrict__ L, const double4* __restrict__ sp,
+                                                   const uint32_t* __restrict__ order,
+                                                   const FmmFrame* __restrict__ fr, int want_forces,
+                                                   double* __restrict__ fx, double* __restrict__ fy,
+                                                   double* __restrict__ fz, double* __restrict__ phi_out) {
+  __shared__ double lt[kFmmMaxNm];
+  const int p = blockIdx.x;
+  const Cell T = cells[p];
+  if (!T.leaf) return;
+  for (int k = threadIdx.x; k < t.nm; k += 64) lt[k] = L[(size_t)p * t.nm + k];
+  __syncthreads();
+  const int lane = threadIdx.x & 31, half = threadIdx.x >> 5;
+  const double min_sq = fr->min_sq;
+  const int64_t lb = off[p], le = off[p + 1];
+  for (int base = T.b; base < T.e; base += 32) {
+    const int i = base + lane;
+    const bool valid = i < T.e;
+    const double4 si = sp[valid ? i : T.b];
+    double pot = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+    if (valid && half == 0) {  // L2P
+      const double x = si.x - T.cx, y = si.y - T.cy, z = si.z - T.cz;
+      if (!want_forces)
+        for (int g = 0; g < t.nm; ++g) pot += lt[g] * Monomial(t, g, x, y, z);
+      if (want_forces) {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int g = 1; g < t.nm; ++g) {
+          const int* gi = t.grad + 3 * g;
+          const int* mg = t.multi + 3 * g;
+          if (gi[0] >= 0) gx += mg[0] * lt[g] * Monomial(t, gi[0], x, y, z);
+          if (gi[1] >= 0) gy += mg[1] * lt[g] * Monomial(t, gi[1], x, y, z);
+          if (gi[2] >= 0) gz += mg[2] * lt[g] * Monomial(t, gi[2], x, y, z);
+        }
+        ex = -gx;
+        ey = -gy;
+        ez = -gz;
+      }
+    }
+    if (valid) {
+      for (int64_t q = lb; q < le; ++q) {
+        const Cell& S = cells[(int)(uint32_t)list[q]];
+        for (int j = S.b + half; j < S.e; j += 2) {
+          if (j == i) continue;
+          const double4 sj = sp[j];
+          if (sj.w == 0.0) continue;
+          const double rx = si.x - sj.x, ry = si.y - sj.y, rz = si.z - sj.z;
+          const double r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 < min_sq) continue;
+          const double inv_r = 1.0 / sqrt(r2);
+          if (want_forces) {
+            const double f = sj.w * inv_r / r2;
+            ex += rx * f;
+            ey += ry * f;
+            ez += rz * f;
+          } else {
+            pot += sj.w * inv_r;
+          }
+        }
+      }
+    }
+    ex += __shfl_down(ex, 32);
+    ey += __shfl_down(ey, 32);
+    ez += __shfl_down(ez, 32);
+    pot += __shfl_down(pot, 32);
+    if (valid && half == 0) {
+      const uint32_t o = order[i];
+      if (want_forces) {
+        const double s = fr->force_scale * si.w;
+        fx[o] = s * ex;
+        fy[o] = s * ey;
+        fz[o] = s * ez;
+      } else {
+        phi_out[i] = si.w * pot;  // q_i phi_i in tree order
+      }
+    }
+  }
+}
+
+__global__ void ZeroNeutralForceKernel(int64_t n, const uint8_t* __restrict__ flags, const double* __restrict__ q,
+                                       double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (flags[i] & kFlagAnnihilated) || q[i] != 0.0) return;
+  fx[i] = 0.0;
+  fy[i] = 0.0;
+  fz[i] = 0.0;
+}
+
+struct FmmRunStats {
+  int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0;
+  int max_depth = 0;
+};
+static FmmRunStats g_fmm_stats;
+
+// want_forces: forces into the store (sources = alive, q != 0); otherwise the potential energy
+// (sources = all alive particles, fast-multipole.cc:335-340 via energy_analysis.cc:71-81).
+int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, double* pe) {
+  am_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s->n;
+  int rc;
+  order = std::min(std::max(order, 1), kFmmMaxOrder);
+  FmmTab tab;
+  if ((rc = DeviceTables(ctx, order, &tab))) return rc;
+  if (want_forces) {
+    hipLaunchKernelGGL(ZeroNeutralForceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->flags, s->q, s->fx,
+                       s->fy, s->fz);
+    AM_HIP(hipGetLastError());
+  }
+  void *gathered, *frame, *rx, *ry, *rz;
+  if ((rc = Scratch(ctx, "fmm_gathered", (size_t)n * 4, &gathered))) return rc;
+  if ((rc = Scratch(ctx, "fmm_frame", sizeof(FmmFrame) + sizeof(RootBox) + 6 * 8, &frame))) return rc;
+  RootBox* root = (RootBox*)((char*)frame + sizeof(FmmFrame));
+  double* box = (double*)((char*)root + sizeof(RootBox));
+  int m = 0;
+  if ((rc = GatherParticles(ctx, n, s->flags, s->q, want_forces ? 1 : 0, (uint32_t*)gathered, &m, "fmm"))) return rc;
+  if (pe) *pe = 0.0;
+  g_fmm_stats = FmmRunStats();
+  if (m == 0) return AM_OK;
+  if ((rc = GatheredBBox(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, box, "fmm"))) return rc;
+  hipLaunchKernelGGL(FrameKernel, dim3(1), dim3(1), 0, st, (const double*)box, (FmmFrame*)frame, root);
+  if ((rc = Scratch(ctx, "fmm_rx", (size_t)n * 8, &rx))) return rc;
+  if ((rc = Scratch(ctx, "fmm_ry", (size_t)n * 8, &ry))) return rc;
+  if ((rc = Scratch(ctx, "fmm_rz", (size_t)n * 8, &rz))) return rc;
+  hipLaunchKernelGGL(RescaleKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)gathered, s->x, s->y,
+                     s->z, (const FmmFrame*)frame, (double*)rx, (double*)ry, (double*)rz);
+  AM_HIP(hipGetLastError());
+  Octree o;
+  if ((rc = BuildOctree(ctx, (const uint32_t*)gathered, m, (const double*)rx, (const double*)ry, (const double*)rz,
+                        root, 32, "fmm", &o)))
+    return rc;
+  const int nn = o.nn;
+  const int nlevels = (int)o.level_start.size() - 1;
+  void *sp, *rad, *cells, *M, *L;
+  if ((rc = Scratch(ctx, "fmm_sp", (size_t)m * sizeof(double4), &sp))) return rc;
+  if ((rc = Scratch(ctx, "fmm_rad", (size_t)nn * 8, &rad))) return rc;
+  if ((rc = Scratch(ctx, "fmm_cells", (size_t)nn * sizeof(Cell), &cells))) return rc;
+  if ((rc = Scratch(ctx, "fmm_M", (size_t)nn * tab.nm * 8, &M))) return rc;
+  if ((rc = Scratch(ctx, "fmm_L", (size_t)nn * tab.nm * 8, &L))) return rc;
+  hipLaunchKernelGGL(SortedCopyKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
+                     (const double*)rx, (const double*)ry, (const double*)rz, s->q, (double4*)sp);
+  for (int d = nlevels - 1; d >= 0; --d) {
+    const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
+    hipLaunchKernelGGL(RadiusKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd,
+                       (const double4*)sp, (double*)rad);
+  }
+  hipLaunchKernelGGL(GatherCellsKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, o.nd, (const uint32_t*)o.pre,
+                     (const uint32_t*)o.bfs2pre, (const double*)rad, (Cell*)cells);
+  AM_HIP(hipGetLastError());
+  int slot;
+  TimerBegin(ctx, "fmm_upward", &slot);
+  hipLaunchKernelGGL(P2MKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const double4*)sp, tab, (double*)M);
+  for (int d = nlevels - 2; d >= 0; --d) {
+    const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
+    hipLaunchKernelGGL(M2MKernel, dim3(n1 - n0), dim3(64), 0, st, n0, o.nd, (const uint32_t*)o.bfs2pre, tab,
+                       (double*)M);
+  }
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+
+  // Dual-tree traversal, level-synchronous.
+  const double theta2 = theta * theta;
+  int64_t n_m2l = 0, n_p2p = 0;
+  size_t cap_m2l = (size_t)64 * nn + 1024, cap_p2p = (size_t)32 * nn + 1024;
+  void *m2l, *p2p;
+  if ((rc = Scratch(ctx, "fmm_m2l", cap_m2l * 8, &m2l))) return rc;
+  if ((rc = Scratch(ctx, "fmm_p2p", cap_p2p * 8, &p2p))) return rc;
+  void *fa, *fb;
+  size_t cap_fa = (size_t)8 * nn + 64, cap_fb = cap_fa;
+  if ((rc = Scratch(ctx, "fmm_front_a", cap_fa * 8, &fa))) return rc;
+  if ((rc = Scratch(ctx, "fmm_front_b", cap_fb * 8, &fb))) return rc;
+  {
+    const uint2 rootpair = make_uint2(0u, 0u);
+    AM_HIP(hipMemcpyAsync(fa, &rootpair, 8, hipMemcpyHostToDevice, st));
+    AM_HIP(hipStreamSynchronize(st));
+  }
+  int64_t np = 1;
+  TimerBegin(ctx, "fmm_traverse", &slot);
+  while (np > 0) {
+    void *c1, *c2, *c3, *o1, *o2, *o3;
+    if ((rc = Scratch(ctx, "fmm_c1", (size_t)np * 4, &c1))) return rc;
+    if ((rc = Scratch(ctx, "fmm_c2", (size_t)np * 4, &c2))) return rc;
+    if ((rc = Scratch(ctx, "fmm_c3", (size_t)np * 4, &c3))) return rc;
+    if ((rc = Scratch(ctx, "fmm_o1", (size_t)(np + 1) * 4, &o1))) return rc;
+    if ((rc = Scratch(ctx, "fmm_o2", (size_t)(np + 1) * 4, &o2))) return rc;
+    if ((rc = Scratch(ctx, "fmm_o3", (size_t)(np + 1) * 4, &o3))) return rc;
+    hipLaunchKernelGGL(TraverseCountKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
+                       (const Cell*)cells, theta2, (uint32_t*)c1, (uint32_t*)c2, (uint32_t*)c3);
+    AM_HIP(hipGetLastError());
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c1, (uint32_t*)o1, np))) return rc;
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c2, (uint32_t*)o2, np))) return rc;
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c3, (uint32_t*)o3, np))) return rc;
+    uint32_t tot[3];
+    AM_HIP(hipMemcpyAsync(&tot[0], (uint32_t*)o1 + np, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipMemcpyAsync(&tot[1], (uint32_t*)o2 + np, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipMemcpyAsync(&tot[2], (uint32_t*)o3 + np, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    // grow outputs (preserving contents)
+    auto grow = [&](const char* name, void** buf, size_t& cap, size_t need, size_t used, size_t elem) -> int {
+      if (need <= cap) return AM_OK;
+      const size_t ncap = std::max(need, cap * 2);
+      void* tmp = nullptr;
+      if (used) {
+        AM_HIP(hipMalloc(&tmp, used * elem));
+        AM_HIP(hipMemcpyAsync(tmp, *buf, used * elem, hipMemcpyDeviceToDevice, st));
+        AM_HIP(hipStreamSynchronize(st));
+      }
+      int r = Scratch(ctx, name, ncap * elem, buf);
+      if (r) return r;
+      if (used) {
+        AM_HIP(hipMemcpyAsync(*buf, tmp, used * elem, hipMemcpyDeviceToDevice, st));
+        AM_HIP(hipStreamSynchronize(st));
+        AM_HIP(hipFree(tmp));
+      }
+      cap = ncap;
+      return AM_OK;
+    };
+    if ((rc = grow("fmm_m2l", &m2l, cap_m2l, (size_t)n_m2l + tot[0], (size_t)n_m2l, 8))) return rc;
+    if ((rc = grow("fmm_p2p", &p2p, cap_p2p, (size_t)n_p2p + tot[1], (size_t)n_p2p, 8))) return rc;
+    if ((rc = grow("fmm_front_b", &fb, cap_fb, tot[2], 0, 8))) return rc;
+    hipLaunchKernelGGL(TraverseWriteKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
+                       (const Cell*)cells, theta2, (const uint32_t*)o1, (const uint32_t*)o2, (const uint32_t*)o3,
+                       (uint64_t*)m2l + n_m2l, (uint64_t*)p2p + n_p2p, (uint2*)fb);
+    AM_HIP(hipGetLastError());
+    n_m2l += tot[0];
+    n_p2p += tot[1];
+    np = tot[2];
+    std::swap(fa, fb);
+    std::swap(cap_fa, cap_fb);
+    // keep the scratch names pointing at the right buffers
+    std::swap(ctx->scratch["fmm_front_a"], ctx->scratch["fmm_front_b"]);
+  }
+  void *v1, *v2, *off_m2l, *off_p2p;
+  if ((rc = Scratch(ctx, "fmm_v1", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &v1))) return rc;
+  if ((rc = Scratch(ctx, "fmm_v2", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &v2))) return rc;
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, 64))) return rc;
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, 64))) return rc;
+  if ((rc = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return rc;
+  if ((rc = Scratch(ctx, "fmm_off_p2p", (size_t)(nn + 1) * 8, &off_p2p))) return rc;
+  hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)m2l, n_m2l,
+                     (int64_t*)off_m2l);
+  hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)p2p, n_p2p,
+                     (int64_t*)off_p2p);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+
+  TimerBegin(ctx, "fmm_m2l", &slot);
+  hipLaunchKernelGGL(M2LKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)m2l,
+                     (const int64_t*)off_m2l, tab, (const double*)M, (double*)L);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  TimerBegin(ctx, "fmm_downward", &slot);
+  for (int d = 0; d < nlevels - 1; ++d) {
+    const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
+    hipLaunchKernelGGL(L2LKernel, dim3(n1 - n0), dim3(64), 0, st, n0, o.nd, (const uint32_t*)o.bfs2pre, tab,
+                       (double*)L);
+  }
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  void* phi = nullptr;
+  if (!want_forces) {
+    if ((rc = Scratch(ctx, "fmm_phi", (size_t)m * 8, &phi))) return rc;
+  }
+  TimerBegin(ctx, "fmm_l2p_p2p", &slot);
+  hipLaunchKernelGGL(L2PP2PKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)p2p,
+                     (const int64_t*)off_p2p, tab, (const double*)L, (const double4*)sp, (const uint32_t*)o.order,
+                     (const FmmFrame*)frame, want_forces ? 1 : 0, s->fx, s->fy, s->fz, (double*)phi);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  g_fmm_stats.cells = nn;
+  g_fmm_stats.m2l_pairs = n_m2l;
+  g_fmm_stats.p2p_cell_pairs = n_p2p;
+  g_fmm_stats.m = m;
+  g_fmm_stats.max_depth = nlevels - 1;
+  if (!want_forces) {
+    double sum = 0.0;
+    if ((rc = ReduceSum(ctx, (const double*)phi, m, &sum))) return rc;
+    FmmFrame fh;
+    AM_HIP(hipMemcpyAsync(&fh, frame, sizeof(fh), hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    *pe = 0.5 * kCoulomb / fh.length * sum;
+  }
+  return AM_OK;
+}
+
+int FastMultipolePotentialEnergy(am_store* s, int order, double theta, double* pe) {
+  return FastMultipoleRun(s, order, theta, false, pe);
+}
+
 }  // namespace am
 
-extern "C" int am_compute_fast_multipole_coulomb_forces(am_store* s, int order, double theta) {
-  return am::Fail(AM_ERR_STATE, "FMM not available in this build");
+using namespace am;
+
+extern "C" {
+
+int am_compute_fast_multipole_coulomb_forces(am_store* s, int order, double theta) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_compute_fast_multipole_coulomb_forces: NULL store");
+  if (!(theta >= 0.0)) return Fail(AM_ERR_INVALID, "fast_multipole_theta must be >= 0");
+  if (s->n == 0) return AM_OK;
+  if (s->n > (int64_t)1 << 30) return Fail(AM_ERR_CAPACITY, "FMM: n too large");
+  AM_HIP(hipSetDevice(s->ctx->device));
+  return FastMultipoleRun(s, order, theta, true, nullptr);
 }
+
+// FMM statistics of the most recent run: sources, cells, M2L cell pairs, P2P cell pairs, depth.
+int am_fmm_stats(am_store* s, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs, int64_t* p2p_cell_pairs,
+                 int32_t* max_depth) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_fmm_stats: NULL store");
+  if (n_sources) *n_sources = g_fmm_stats.m;
+  if (n_cells) *n_cells = g_fmm_stats.cells;
+  if (m2l_pairs) *m2l_pairs = g_fmm_stats.m2l_pairs;
+  if (p2p_cell_pairs) *p2p_cell_pairs = g_fmm_stats.p2p_cell_pairs;
+  if (max_depth) *max_depth = g_fmm_stats.max_depth;
+  return AM_OK;
+}
+
+}  // extern "C"

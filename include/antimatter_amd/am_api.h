@@ -158,6 +158,10 @@ int am_bh_tree_download(am_store* store, uint32_t* order, int32_t* begin, int32_
                         double* qn);
 /* ComputeFastMultipoleCoulombForces (src/interaction/fast-multipole.h:73, .cc:684-713). */
 int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double theta);
+/* Statistics of the most recent FMM call (the sizes FastMultipoleSolver::Compute builds,
+ * fast-multipole.cc:397-526): sources, cells, M2L cell pairs, P2P cell pairs, deepest level. */
+int am_fmm_stats(am_store* store, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs,
+                 int64_t* p2p_cell_pairs, int32_t* max_depth);
 
 /* ---- behaviours ---------------------------------------------------------------------------- */
 /* LorentzForce::Run for every alive agent (src/forces/lorentz.cc:37-71): F = qE + F_coul,

@@ -1,0 +1,76 @@
+"""FMM on the GPU against the oracle restatement of fast-multipole.cc."""
+import numpy as np
+import pytest
+
+import antimatter_amd as am
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def cloud(n, seed=31, frac_neutral=0.05, frac_dead=0.04):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (n, 3)) * np.array([1e-3, 1e-3, 4e-3])
+    sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    sp[rng.random(n) < frac_neutral] = 2
+    ann = (rng.random(n) < frac_dead).astype(np.uint8)
+    return pos, sp, ann
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture
+def store(ctx):
+    s = am.Store(ctx, 20000)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("order,theta", [(4, 0.5), (2, 0.7), (6, 0.3), (1, 0.5), (8, 0.5)])
+def test_fmm_forces_match_oracle(store, order, theta):
+    pos, sp, ann = cloud(5000)
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    store.compute_fast_multipole_coulomb_forces(order, theta)
+    f = store.forces()
+    m, q = oracle.mass_charge(sp)
+    ref, st = oracle.fmm_forces(pos, q, ann, order, theta, want_stats=True)
+    live = ann == 0
+    gs = store.fmm_stats()
+    assert gs["cells"] == st["cells"]
+    assert gs["m2l_pairs"] == st["m2l_pairs"]
+    np.testing.assert_allclose(f[live], ref[live], rtol=1e-9, atol=1e-10 * np.abs(ref).max())
+    assert np.all(f[(sp == 2) & live] == 0)
+
+
+def test_fmm_theta_zero_is_allpairs(store):
+    pos, sp, ann = cloud(3000, seed=32, frac_neutral=0.0, frac_dead=0.0)
+    store.upload(pos, np.zeros_like(pos), sp)
+    store.compute_fast_multipole_coulomb_forces(4, 0.0)
+    m, q = oracle.mass_charge(sp)
+    assert rel_l2(store.forces(), oracle.allpairs_forces(pos, q)) < 1e-12
+    assert store.fmm_stats()["m2l_pairs"] == 0
+
+
+def test_fmm_potential_energy(store):
+    pos, sp, ann = cloud(4000, seed=33)
+    vel = np.zeros_like(pos)
+    store.upload(pos, vel, sp, annihilated=ann)
+    trap = am.make_trap(1.0, 10.0, 0.01, 0.01)
+    m, q = oracle.mass_charge(sp)
+    e = store.compute_total_energy(trap, "fast-multipole", fast_multipole_order=4, fast_multipole_theta=0.5)
+    ref = oracle.fmm_potential_energy(pos, q, ann, 4, 0.5)
+    assert e.energy_potential == pytest.approx(ref, rel=1e-10)
+
+
+def test_fmm_small_and_degenerate(store):
+    for n in (1, 2, 33):
+        pos, sp, ann = cloud(n, seed=34 + n, frac_neutral=0.0, frac_dead=0.0)
+        store.upload(pos, np.zeros_like(pos), sp)
+        store.compute_fast_multipole_coulomb_forces(4, 0.5)
+        m, q = oracle.mass_charge(sp)
+        np.testing.assert_allclose(store.forces(), oracle.fmm_forces(pos, q, None, 4, 0.5), rtol=1e-10, atol=1e-30)
+    store.upload(np.zeros((4, 3)), np.zeros((4, 3)), np.array([2, 2, 2, 2], np.uint8))  # all neutral
+    store.compute_fast_multipole_coulomb_forces(4, 0.5)
+    assert np.all(store.forces() == 0)
