@@ -95,11 +95,14 @@ SIGNATURES = {
     "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
     "am_fmm_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "am_compute_fast_multipole_coulomb_forces_ex": (_int, [_vp, _int, _d, _int]),
     "am_lorentz_force": (_int, [_vp, _vp, _d]),
     "am_cyclotron_cooling": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
     "am_push": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
     "am_annihilation": (_int, [_vp, _d, _d, _u64, _u64, _vp]),
     "am_compute_total_energy": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "am_debug_exclusive_scan": (_int, [_vp, _vp, _i64, _vp]),
+    "am_debug_radix_sort": (_int, [_vp, _vp, _vp, _i64, _int]),
 }
 
 _lib = None
@@ -180,6 +183,18 @@ class Context:
 
     def synchronize(self):
         _check(lib().am_ctx_synchronize(self.h))
+
+    def exclusive_scan(self, counts):
+        a = np.ascontiguousarray(counts, dtype=np.uint32)
+        out = np.zeros(len(a) + 1, dtype=np.uint32)
+        _check(lib().am_debug_exclusive_scan(self.h, _p(a), len(a), _p(out)))
+        return out
+
+    def radix_sort(self, keys, vals, key_bits=64):
+        k = np.array(keys, dtype=np.uint64)
+        v = np.array(vals, dtype=np.uint32)
+        _check(lib().am_debug_radix_sort(self.h, _p(k), _p(v), len(k), int(key_bits)))
+        return k, v
 
     def enable_timing(self, on: bool = True):
         _check(lib().am_ctx_enable_timing(self.h, int(on)))
@@ -289,8 +304,11 @@ class Store:
         return dict(order=order, begin=b, end=e, leaf=leaf, qp=qp, qn=qn, max_depth=md.value,
                     deep_overflow=ov.value)
 
-    def compute_fast_multipole_coulomb_forces(self, order: int, theta: float):
-        _check(lib().am_compute_fast_multipole_coulomb_forces(self.h, int(order), theta))
+    def compute_fast_multipole_coulomb_forces(self, order: int, theta: float, precision: int = FP64):
+        if precision == FP64:
+            _check(lib().am_compute_fast_multipole_coulomb_forces(self.h, int(order), theta))
+        else:
+            _check(lib().am_compute_fast_multipole_coulomb_forces_ex(self.h, int(order), theta, precision))
 
     def fmm_stats(self):
         v = [ctypes.c_int64() for _ in range(4)]

@@ -216,6 +216,16 @@ __global__ void SortedCopyKernel(int m, const uint32_t* __restrict__ order, cons
   sp[k] = make_double4(rx[i], ry[i], rz[i], q[i]);
 }
 
+// FP32 hi/lo source mirror in tree order (for the FP32 near field).
+__global__ void SortedSrcKernel(int m, const uint32_t* __restrict__ order, const float4* __restrict__ src,
+                                float4* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = order[k];
+  out[2 * k] = src[2 * i];
+  out[2 * k + 1] = src[2 * i + 1];
+}
+
 struct Cell {  // preorder cell record
   double cx, cy, cz, radius;
   int b, e, leaf, nchild;
@@ -397,59 +407,101 @@ __global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int
   off[t] = lo;
 }
 
-// M2L (fast-multipole.cc:528-555): one wave per target cell.
-__global__ __launch_bounds__(64) void M2LKernel(int nn, const Cell* __restrict__ cells,
-                                                const uint64_t* __restrict__ list, const int64_t* __restrict__ off,
-                                                FmmTab t, const double* __restrict__ M, double* __restrict__ L) {
-  __shared__ double D[kFmmMaxNd];
-  __shared__ double Ms[kFmmMaxNm];
+// M2L (fast-multipole.cc:528-555): one workgroup of kM2LWaves waves per target cell; wave w takes
+// sources w, w + kM2LWaves, ... of the cell's list.  The index tables live in LDS (orders up to
+// kM2LLdsOrder; larger orders read them from global memory).  Per source the wave builds the
+// derivative tensor D of 1/r degree by degree in its LDS slice (the recurrence of :207-232), then
+// lane gamma contracts sum_alpha (-1)^|alpha| M_alpha D_{alpha+gamma}.  The waves' partial sums
+// are combined in wave order at the end, so the result is deterministic.
+constexpr int kM2LWaves = 4;
+constexpr int kM2LLdsOrder = 5;  // keeps the workgroup under 64 KB of LDS
+
+__global__ __launch_bounds__(64 * kM2LWaves) void M2LKernel(int nn, const Cell* __restrict__ cells,
+                                                             const uint64_t* __restrict__ list,
+                                                             const int64_t* __restrict__ off, FmmTab t,
+                                                             const double* __restrict__ M, double* __restrict__ L) {
+  // Dynamic LDS sized for the order: D[w][nd], Ms[w][nm], part[w][nm], sign[nm] (doubles), then
+  // m2l[nm*nm] and rec[nd*10] (ints, when the order allows).
+  extern __shared__ double lds[];
+  const int nm = t.nm, nd = t.nd;
+  double* D0 = lds;
+  double* Ms0 = D0 + kM2LWaves * nd;
+  double* part0 = Ms0 + kM2LWaves * nm;
+  double* s_sign = part0 + kM2LWaves * nm;
+  int* s_m2l = (int*)(s_sign + nm);
+  int* s_rec = s_m2l + nm * nm;
   const int p = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t b = off[p], e = off[p + 1];
-  double acc[3] = {0.0, 0.0, 0.0};
   if (b == e) {
-    for (int k = 0; k < 3; ++k) {
-      const int g = threadIdx.x + 64 * k;
-      if (g < t.nm) L[(size_t)p * t.nm + g] = 0.0;
-    }
+    for (int g = threadIdx.x; g < nm; g += 64 * kM2LWaves) L[(size_t)p * nm + g] = 0.0;
     return;
   }
+  const bool lds_tab = t.P <= kM2LLdsOrder;
+  if (lds_tab) {
+    for (int k = threadIdx.x; k < nm * nm; k += 64 * kM2LWaves) s_m2l[k] = t.m2l[k];
+    for (int k = threadIdx.x; k < t.nd * 10; k += 64 * kM2LWaves) s_rec[k] = t.rec[k];
+  }
+  for (int k = threadIdx.x; k < nm; k += 64 * kM2LWaves) s_sign[k] = t.sign[k];
+  const int* m2l = lds_tab ? s_m2l : t.m2l;
+  const int* rec = lds_tab ? s_rec : t.rec;
+  double* Dw = D0 + wave * nd;
+  double* Mw = Ms0 + wave * nm;
   const Cell T = cells[p];
-  for (int64_t q = b; q < e; ++q) {
-    const int s = (int)(uint32_t)list[q];
-    const Cell& S = cells[s];
-    const double xs[3] = {T.cx - S.cx, T.cy - S.cy, T.cz - S.cz};
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int64_t count = e - b;
+  const int64_t rounds = (count + kM2LWaves - 1) / kM2LWaves;
+  for (int64_t r = 0; r < rounds; ++r) {
+    const int64_t q = b + r * kM2LWaves + wave;
+    const bool active = q < e;
     __syncthreads();
-    for (int k = threadIdx.x; k < t.nm; k += 64) Ms[k] = M[(size_t)s * t.nm + k];
-    const double inv_r2 = 1.0 / (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
-    if (threadIdx.x == 0) D[0] = sqrt(inv_r2);
+    double xs[3] = {0.0, 0.0, 0.0}, inv_r2 = 0.0;
+    if (active) {
+      const int s = (int)(uint32_t)list[q];
+      const Cell& S = cells[s];
+      xs[0] = T.cx - S.cx;
+      xs[1] = T.cy - S.cy;
+      xs[2] = T.cz - S.cz;
+      for (int k = lane; k < nm; k += 64) Mw[k] = s_sign[k] * M[(size_t)s * nm + k];
+      inv_r2 = 1.0 / (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
+      if (lane == 0) Dw[0] = sqrt(inv_r2);
+    }
     for (int g = 1; g <= 2 * t.P; ++g) {
       __syncthreads();
-      for (int k = t.deg_start[g] + threadIdx.x; k < t.deg_start[g + 1]; k += 64) {
-        const int* r = t.rec + 10 * k;
-        const int i = r[0];
-        double sum = (2 * r[1 + i] - 1) * xs[i] * D[r[4 + i]];
-        if (r[7 + i] >= 0) sum += (r[1 + i] - 1) * (r[1 + i] - 1) * D[r[7 + i]];
+      if (!active) continue;
+      for (int k = t.deg_start[g] + lane; k < t.deg_start[g + 1]; k += 64) {
+        const int* rr = rec + 10 * k;
+        const int i = rr[0];
+        double sum = (2 * rr[1 + i] - 1) * xs[i] * Dw[rr[4 + i]];
+        if (rr[7 + i] >= 0) sum += (rr[1 + i] - 1) * (rr[1 + i] - 1) * Dw[rr[7 + i]];
         for (int j = 0; j < 3; ++j) {
-          if (j == i || r[1 + j] == 0) continue;
-          sum += 2.0 * r[1 + j] * xs[j] * D[r[4 + j]];
-          if (r[7 + j] >= 0) sum += r[1 + j] * (r[1 + j] - 1) * D[r[7 + j]];
+          if (j == i || rr[1 + j] == 0) continue;
+          sum += 2.0 * rr[1 + j] * xs[j] * Dw[rr[4 + j]];
+          if (rr[7 + j] >= 0) sum += rr[1 + j] * (rr[1 + j] - 1) * Dw[rr[7 + j]];
         }
-        D[k] = -sum * inv_r2;
+        Dw[k] = -sum * inv_r2;
       }
     }
     __syncthreads();
+    if (!active) continue;
     for (int k = 0; k < 3; ++k) {
-      const int g = threadIdx.x + 64 * k;
-      if (g >= t.nm) continue;
-      const int* ix = t.m2l + (size_t)g * t.nm;
+      const int g = lane + 64 * k;
+      if (g >= nm) continue;
+      const int* ix = m2l + g * nm;
       double a = 0.0;
-      for (int al = 0; al < t.nm; ++al) a += t.sign[al] * Ms[al] * D[ix[al]];
+      for (int al = 0; al < nm; ++al) a += Mw[al] * Dw[ix[al]];
       acc[k] += a * t.inv_fact[g];
     }
   }
   for (int k = 0; k < 3; ++k) {
-    const int g = threadIdx.x + 64 * k;
-    if (g < t.nm) L[(size_t)p * t.nm + g] = acc[k];
+    const int g = lane + 64 * k;
+    if (g < nm) part0[wave * nm + g] = acc[k];
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < nm; g += 64 * kM2LWaves) {
+    double v = 0.0;
+    for (int w = 0; w < kM2LWaves; ++w) v += part0[w * nm + g];
+    L[(size_t)p * nm + g] = v;
   }
 }
 
@@ -488,6 +540,7 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
                                                    const double* __restrict__ L, const double4* __restrict__ sp,
                                                    const uint32_t* __restrict__ order,
                                                    const FmmFrame* __restrict__ fr, int want_forces,
+                                                   const float4* __restrict__ sp32,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz, double* __restrict__ phi_out) {
   __shared__ double lt[kFmmMaxNm];
@@ -522,7 +575,31 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
         ez = -gz;
       }
     }
-    if (valid) {
+    double fx32 = 0.0, fy32 = 0.0, fz32 = 0.0;  // FP32 near field, e/unit^2
+    if (valid && sp32 != nullptr && want_forces) {
+      const float4 th = sp32[2 * i], tl = sp32[2 * i + 1];
+      for (int64_t q = lb; q < le; ++q) {
+        const Cell& S = cells[(int)(uint32_t)list[q]];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        for (int j = S.b + half; j < S.e; j += 2) {
+          if (j == i) continue;
+          const float4 sh = sp32[2 * j];
+          const float4 sl = sp32[2 * j + 1];
+          const float dx = (th.x - sh.x) + (tl.x - sl.x);
+          const float dy = (th.y - sh.y) + (tl.y - sl.y);
+          const float dz = (th.z - sh.z) + (tl.z - sl.z);
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+          const float ri = __builtin_amdgcn_rsqf(r2);
+          const float c = (sh.w * ri) * (ri * ri);
+          ax = fmaf(c, dx, ax);
+          ay = fmaf(c, dy, ay);
+          az = fmaf(c, dz, az);
+        }
+        fx32 += ax;
+        fy32 += ay;
+        fz32 += az;
+      }
+    } else if (valid) {
       for (int64_t q = lb; q < le; ++q) {
         const Cell& S = cells[(int)(uint32_t)list[q]];
         for (int j = S.b + half; j < S.e; j += 2) {
@@ -548,13 +625,17 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
     ey += __shfl_down(ey, 32);
     ez += __shfl_down(ez, 32);
     pot += __shfl_down(pot, 32);
+    fx32 += __shfl_down(fx32, 32);
+    fy32 += __shfl_down(fy32, 32);
+    fz32 += __shfl_down(fz32, 32);
     if (valid && half == 0) {
       const uint32_t o = order[i];
       if (want_forces) {
         const double s = fr->force_scale * si.w;
-        fx[o] = s * ex;
-        fy[o] = s * ey;
-        fz[o] = s * ez;
+        const double s32 = kCoulomb * si.w * kElementaryCharge * kInvUnit2;
+        fx[o] = s * ex + s32 * fx32;
+        fy[o] = s * ey + s32 * fy32;
+        fz[o] = s * ez + s32 * fz32;
       } else {
         phi_out[i] = si.w * pot;  // q_i phi_i in tree order
       }
@@ -579,7 +660,7 @@ static FmmRunStats g_fmm_stats;
 
 // want_forces: forces into the store (sources = alive, q != 0); otherwise the potential energy
 // (sources = all alive particles, fast-multipole.cc:335-340 via energy_analysis.cc:71-81).
-int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, double* pe) {
+int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, double* pe, int precision = AM_FP64) {
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = s->n;
@@ -730,7 +811,10 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   AM_HIP(hipGetLastError());
 
   TimerBegin(ctx, "fmm_m2l", &slot);
-  hipLaunchKernelGGL(M2LKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)m2l,
+  const size_t m2l_lds = sizeof(double) * ((size_t)kM2LWaves * (tab.nd + 2 * tab.nm) + tab.nm) +
+                         (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
+  hipLaunchKernelGGL(M2LKernel, dim3(nn), dim3(64 * kM2LWaves), m2l_lds, st, nn, (const Cell*)cells,
+                     (const uint64_t*)m2l,
                      (const int64_t*)off_m2l, tab, (const double*)M, (double*)L);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
@@ -742,6 +826,13 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   }
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
+  void* sp32 = nullptr;
+  if (want_forces && precision == AM_FP32) {
+    if ((rc = Scratch(ctx, "fmm_sp32", (size_t)m * 2 * sizeof(float4), &sp32))) return rc;
+    hipLaunchKernelGGL(SortedSrcKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order, s->src,
+                       (float4*)sp32);
+    AM_HIP(hipGetLastError());
+  }
   void* phi = nullptr;
   if (!want_forces) {
     if ((rc = Scratch(ctx, "fmm_phi", (size_t)m * 8, &phi))) return rc;
@@ -749,7 +840,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   TimerBegin(ctx, "fmm_l2p_p2p", &slot);
   hipLaunchKernelGGL(L2PP2PKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)p2p,
                      (const int64_t*)off_p2p, tab, (const double*)L, (const double4*)sp, (const uint32_t*)o.order,
-                     (const FmmFrame*)frame, want_forces ? 1 : 0, s->fx, s->fy, s->fz, (double*)phi);
+                     (const FmmFrame*)frame, want_forces ? 1 : 0, (const float4*)sp32, s->fx, s->fy, s->fz,
+                     (double*)phi);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   g_fmm_stats.cells = nn;
@@ -785,6 +877,18 @@ int am_compute_fast_multipole_coulomb_forces(am_store* s, int order, double thet
   if (s->n > (int64_t)1 << 30) return Fail(AM_ERR_CAPACITY, "FMM: n too large");
   AM_HIP(hipSetDevice(s->ctx->device));
   return FastMultipoleRun(s, order, theta, true, nullptr);
+}
+
+// FMM with a choice of near-field arithmetic: AM_FP64 (the reference arithmetic) or AM_FP32 (P2P on
+// hi/lo coordinates, FP64 per-cell accumulation; expansions stay FP64).
+int am_compute_fast_multipole_coulomb_forces_ex(am_store* s, int order, double theta, int precision) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_compute_fast_multipole_coulomb_forces_ex: NULL store");
+  if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
+  if (!(theta >= 0.0)) return Fail(AM_ERR_INVALID, "fast_multipole_theta must be >= 0");
+  if (s->n == 0) return AM_OK;
+  if (s->n > (int64_t)1 << 30) return Fail(AM_ERR_CAPACITY, "FMM: n too large");
+  AM_HIP(hipSetDevice(s->ctx->device));
+  return FastMultipoleRun(s, order, theta, true, nullptr, precision);
 }
 
 // FMM statistics of the most recent run: sources, cells, M2L cell pairs, P2P cell pairs, depth.

@@ -7,7 +7,7 @@
 //      wave the lanes with equal digits are found with 8 ballots (one per digit bit), so a key's
 //      rank = base[digit] + keys of that digit in earlier waves of the round + earlier lanes of
 //      its own wave.  Input order is preserved within a digit: the sort is stable.
-// Scan: one workgroup of 1024 threads; each thread owns a contiguous segment.
+// Scan: see ExclusiveScan below.
 #include <algorithm>
 
 #include "am_internal.hpp"
@@ -81,21 +81,58 @@ __global__ __launch_bounds__(kSortThreads) void RadixScatterKernel(
   }
 }
 
-constexpr int kScanThreads = 1024;
+// Exclusive scan: tiles of 4096 counts.  (1) tile sums with coalesced loads; (2) one workgroup
+// scans the tile sums; (3) each tile is staged in LDS (coalesced), every thread scans 16
+// consecutive counts, the 256 thread totals are scanned, and the tile is stored coalesced.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTileN = kScanThreads * kScanItems;
+constexpr int kScanSumsThreads = 1024;
 
-__global__ __launch_bounds__(kScanThreads) void ExclusiveScanKernel(const uint32_t* __restrict__ in,
-                                                                     uint32_t* __restrict__ out, int64_t n) {
-  __shared__ uint32_t sums[kScanThreads];
+__device__ inline int PadIdx(int i) { return i + (i >> 4); }  // one pad word per 16: no bank conflicts
+
+__device__ inline uint32_t BlockExclusiveScan256(uint32_t v, uint32_t* sh, uint32_t* total) {
   const int t = threadIdx.x;
-  const int64_t seg = (n + kScanThreads - 1) / kScanThreads;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint32_t a = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[t];
+  *total = sh[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void ScanTileSumKernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                  uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sh[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTileN;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  BlockExclusiveScan256(s, sh, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanSumsThreads) void ScanSumsKernel(const uint32_t* __restrict__ in,
+                                                                   uint32_t* __restrict__ out, int64_t n) {
+  __shared__ uint32_t sums[kScanSumsThreads];
+  const int t = threadIdx.x;
+  const int64_t seg = (n + kScanSumsThreads - 1) / kScanSumsThreads;
   const int64_t b = std::min<int64_t>(n, (int64_t)t * seg);
   const int64_t e = std::min<int64_t>(n, b + seg);
   uint32_t s = 0;
   for (int64_t k = b; k < e; ++k) s += in[k];
   sums[t] = s;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the segment sums.
-  for (int off = 1; off < kScanThreads; off <<= 1) {
+  for (int off = 1; off < kScanSumsThreads; off <<= 1) {
     const uint32_t v = t >= off ? sums[t - off] : 0u;
     __syncthreads();
     sums[t] += v;
@@ -107,11 +144,61 @@ __global__ __launch_bounds__(kScanThreads) void ExclusiveScanKernel(const uint32
     out[k] = run;
     run += v;
   }
-  if (t == kScanThreads - 1) out[n] = sums[kScanThreads - 1];
+  if (t == kScanSumsThreads - 1) out[n] = sums[kScanSumsThreads - 1];
+}
+
+__global__ __launch_bounds__(kScanThreads) void ScanTileApplyKernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                    const uint32_t* __restrict__ tile_off,
+                                                                    int num_tiles, uint32_t* __restrict__ out) {
+  __shared__ uint32_t tile[kScanTileN + kScanTileN / 16];
+  __shared__ uint32_t sh[kScanThreads];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kScanTileN;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int li = k * kScanThreads + t;
+    const int64_t i = base + li;
+    tile[PadIdx(li)] = i < n ? in[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) s += tile[PadIdx(t * kScanItems + k)];
+  uint32_t tot;
+  uint32_t run = BlockExclusiveScan256(s, sh, &tot) + tile_off[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const int li = PadIdx(t * kScanItems + k);
+    const uint32_t v = tile[li];
+    tile[li] = run;
+    run += v;
+  }
+  __syncthreads();
+  for (int k = 0; k < kScanItems; ++k) {
+    const int li = k * kScanThreads + t;
+    const int64_t i = base + li;
+    if (i < n) out[i] = tile[PadIdx(li)];
+  }
+  if (blockIdx.x == 0 && t == 0) out[n] = tile_off[num_tiles];
 }
 
 int ExclusiveScan(am_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
-  hipLaunchKernelGGL(ExclusiveScanKernel, dim3(1), dim3(kScanThreads), 0, ctx->stream, in, out, n);
+  if (n <= 0) {
+    hipLaunchKernelGGL(ScanSumsKernel, dim3(1), dim3(kScanSumsThreads), 0, ctx->stream, in, out, (int64_t)0);
+    AM_HIP(hipGetLastError());
+    return AM_OK;
+  }
+  const int64_t tiles = (n + kScanTileN - 1) / kScanTileN;
+  if (tiles > 0x7fffffff) return Fail(AM_ERR_CAPACITY, "scan: n too large");
+  void* buf;
+  int rc = Scratch(ctx, "scan_tiles", (size_t)(2 * tiles + 1) * 4, &buf);
+  if (rc) return rc;
+  uint32_t* sums = (uint32_t*)buf;
+  uint32_t* offs = sums + tiles;
+  hipLaunchKernelGGL(ScanTileSumKernel, dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, in, n, sums);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ScanSumsKernel, dim3(1), dim3(kScanSumsThreads), 0, ctx->stream, (const uint32_t*)sums, offs,
+                     tiles);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(ScanTileApplyKernel, dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, in, n,
+                     (const uint32_t*)offs, (int)tiles, out);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -151,3 +238,39 @@ int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int k
 }
 
 }  // namespace am
+
+using namespace am;
+
+extern "C" {
+
+// Diagnostics: the device scan and radix sort on host buffers (for tests).
+int am_debug_exclusive_scan(am_ctx* ctx, const uint32_t* in, int64_t n, uint32_t* out) {
+  if (!ctx || n < 0 || (n > 0 && (!in || !out))) return Fail(AM_ERR_INVALID, "am_debug_exclusive_scan: bad args");
+  void *din, *dout;
+  int rc;
+  if ((rc = Scratch(ctx, "dbg_in", (size_t)(n + 1) * 4, &din))) return rc;
+  if ((rc = Scratch(ctx, "dbg_out", (size_t)(n + 1) * 4, &dout))) return rc;
+  if (n) AM_HIP(hipMemcpyAsync(din, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = ExclusiveScan(ctx, (const uint32_t*)din, (uint32_t*)dout, n))) return rc;
+  AM_HIP(hipMemcpyAsync(out, dout, (size_t)(n + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipStreamSynchronize(ctx->stream));
+  return AM_OK;
+}
+
+int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
+  if (!ctx || n < 0 || (n > 0 && (!keys || !vals))) return Fail(AM_ERR_INVALID, "am_debug_radix_sort: bad args");
+  if (n == 0) return AM_OK;
+  void *dk, *dv;
+  int rc;
+  if ((rc = Scratch(ctx, "dbg_keys", (size_t)n * 8, &dk))) return rc;
+  if ((rc = Scratch(ctx, "dbg_vals", (size_t)n * 4, &dv))) return rc;
+  AM_HIP(hipMemcpyAsync(dk, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  AM_HIP(hipMemcpyAsync(dv, vals, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)dk, (uint32_t*)dv, n, key_bits))) return rc;
+  AM_HIP(hipMemcpyAsync(keys, dk, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipMemcpyAsync(vals, dv, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipStreamSynchronize(ctx->stream));
+  return AM_OK;
+}
+
+}  // extern "C"

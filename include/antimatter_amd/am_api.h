@@ -158,6 +158,10 @@ int am_bh_tree_download(am_store* store, uint32_t* order, int32_t* begin, int32_
                         double* qn);
 /* ComputeFastMultipoleCoulombForces (src/interaction/fast-multipole.h:73, .cc:684-713). */
 int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double theta);
+/* ComputeFastMultipoleCoulombForces with a choice of near-field (P2P) arithmetic: AM_FP64 is the
+ * reference arithmetic (= am_compute_fast_multipole_coulomb_forces); AM_FP32 evaluates P2P on hi/lo
+ * coordinates with FP64 per-cell accumulation (expansions stay FP64). */
+int am_compute_fast_multipole_coulomb_forces_ex(am_store* store, int order, double theta, int precision);
 /* Statistics of the most recent FMM call (the sizes FastMultipoleSolver::Compute builds,
  * fast-multipole.cc:397-526): sources, cells, M2L cell pairs, P2P cell pairs, deepest level. */
 int am_fmm_stats(am_store* store, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs,
@@ -191,6 +195,12 @@ int am_is_antiparticle_pair(int species_a, int species_b);
  * the potentials with the previous call. */
 int am_compute_total_energy(am_store* store, const am_trap_field* field, const am_energy_settings* settings,
                             am_energy_history* history, am_energy_breakdown* out);
+
+/* ---- diagnostics (no reference counterpart: checks of the device building blocks) ------------ */
+/* Exclusive prefix sum of n uint32 on the device; out receives n + 1 values (out[n] = total). */
+int am_debug_exclusive_scan(am_ctx* ctx, const uint32_t* in, int64_t n, uint32_t* out);
+/* Stable LSD radix sort of (key, value) pairs on the device, in place on the host arrays. */
+int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
 
 #ifdef __cplusplus
 }

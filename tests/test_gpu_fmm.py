@@ -74,3 +74,15 @@ def test_fmm_small_and_degenerate(store):
     store.upload(np.zeros((4, 3)), np.zeros((4, 3)), np.array([2, 2, 2, 2], np.uint8))  # all neutral
     store.compute_fast_multipole_coulomb_forces(4, 0.5)
     assert np.all(store.forces() == 0)
+
+
+def test_fmm_fp32_near_field(store):
+    pos, sp, ann = cloud(6000, seed=35, frac_neutral=0.0, frac_dead=0.0)
+    store.upload(pos, np.zeros_like(pos), sp)
+    m, q = oracle.mass_charge(sp)
+    store.compute_fast_multipole_coulomb_forces(4, 0.5, am.FP32)
+    f32 = store.forces()
+    ref = oracle.fmm_forces(pos, q, None, 4, 0.5)
+    assert rel_l2(f32, ref) < 1e-6  # FP32 near-field arithmetic on top of the method
+    store.compute_fast_multipole_coulomb_forces(4, 0.0, am.FP32)
+    assert rel_l2(store.forces(), oracle.allpairs_forces(pos, q)) < 1e-5
