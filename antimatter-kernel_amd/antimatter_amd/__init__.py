@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -101,6 +102,12 @@ SIGNATURES = {
     "am_push": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
     "am_annihilation": (_int, [_vp, _d, _d, _u64, _u64, _vp]),
     "am_compute_total_energy": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "am_ensemble_create": (_int, [_vp, _i32, _vp, _i64, _vp]),
+    "am_ensemble_destroy": (_int, [_vp]),
+    "am_ensemble_set_chunk": (_int, [_vp, _i32]),
+    "am_ensemble_autotune": (_int, [_vp, _vp, _i32, _vp, _vp]),
+    "am_ensemble_run": (_int, [_vp, _i32, _u64, _i32]),
+    "am_ensemble_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "am_debug_exclusive_scan": (_int, [_vp, _vp, _i64, _vp]),
     "am_debug_radix_sort": (_int, [_vp, _vp, _vp, _i64, _int]),
 }
@@ -169,9 +176,12 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().am_ctx_create(device, ctypes.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # stores made from this context (closed first)
 
     def close(self):
         if self.h:
+            for c in list(self._children):
+                c.close()
             lib().am_ctx_destroy(self.h)
             self.h = None
 
@@ -219,10 +229,15 @@ class Store:
         _check(lib().am_store_create(ctx.h, int(capacity), ctypes.byref(h)))
         self.h = h
         self.capacity = int(capacity)
+        self._children = weakref.WeakSet()  # ensembles over this store (closed first)
+        ctx._children.add(self)
 
     def close(self):
         if self.h:
-            lib().am_store_destroy(self.h)
+            for c in list(self._children):
+                c.close()
+            if self.ctx.h:
+                lib().am_store_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -343,3 +358,4 @@ class Store:
 
 
 from .driver import simulate, derive_dt, initial_conditions  # noqa: E402,F401
+from .ensemble import Ensemble, TrapDesc, make_config5, trap_seed  # noqa: E402,F401

@@ -25,21 +25,6 @@ namespace am {
 constexpr int kAllPairsThreads = 256;
 constexpr int kTile = 128;  // sources per FP32 partial sum
 
-// Pair interaction in FP32 on hi/lo coordinates.  Returns s = q_j / (r^2 + eps^2)^(3/2) and
-// the separation d = x_i - x_j (units).
-__device__ __forceinline__ void PairF32(const float4 th, const float4 tl, const float4 sh, const float4 sl,
-                                        float& px, float& py, float& pz) {
-  const float dx = (th.x - sh.x) + (tl.x - sl.x);
-  const float dy = (th.y - sh.y) + (tl.y - sl.y);
-  const float dz = (th.z - sh.z) + (tl.z - sl.z);
-  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
-  const float ri = __builtin_amdgcn_rsqf(r2);
-  const float s = (sh.w * ri) * (ri * ri);
-  px = fmaf(s, dx, px);
-  py = fmaf(s, dy, py);
-  pz = fmaf(s, dz, pz);
-}
-
 // part layout: [chunk][component][n_pad] doubles (acc in e/unit^2 before scaling).
 __global__ __launch_bounds__(kAllPairsThreads) void AllPairsF32Kernel(const float4* __restrict__ src, int n_pad,
                                                                        int chunk, double* __restrict__ part) {

@@ -153,6 +153,22 @@ __device__ inline double Dot3(double ax, double ay, double az, double bx, double
   return __dadd_rn(__dadd_rn(__dmul_rn(ax, bx), __dmul_rn(ay, by)), __dmul_rn(az, bz));
 }
 
+// FP32 pair interaction on hi/lo coordinates (units of 2^-20 m): accumulates
+// q_j (x_i - x_j) / (r^2 + eps^2)^(3/2) into (px, py, pz).  dx = (xh_i - xh_j) + (xl_i - xl_j) keeps
+// ~2^-24 relative accuracy on dx even for pairs much closer than the coordinates' magnitude.
+__device__ __forceinline__ void PairF32(const float4 th, const float4 tl, const float4 sh, const float4 sl,
+                                        float& px, float& py, float& pz) {
+  const float dx = (th.x - sh.x) + (tl.x - sl.x);
+  const float dy = (th.y - sh.y) + (tl.y - sl.y);
+  const float dz = (th.z - sh.z) + (tl.z - sl.z);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float s = (sh.w * ri) * (ri * ri);
+  px = fmaf(s, dx, px);
+  py = fmaf(s, dy, py);
+  pz = fmaf(s, dz, pz);
+}
+
 // Launch helpers ----------------------------------------------------------------------------------
 int RefreshSource(am_store* s);  // rebuild store->src from the FP64 state
 int64_t CountActive(am_store* s, int* status);
@@ -162,6 +178,8 @@ int64_t CountActive(am_store* s, int* status);
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
 // Deterministic FP64 sum of n device doubles into *host_out (synchronises the stream).
 int ReduceSum(am_ctx* ctx, const double* in, int64_t n, double* host_out);
+// Pre-sizes the scratch of RadixSortPairs / ExclusiveScan for n elements (no allocation later).
+int ReserveSortScratch(am_ctx* ctx, int64_t n);
 // Exclusive prefix sum of n uint32 counts into out (n+1 entries; out[n] = total).
 int ExclusiveScan(am_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n);
 

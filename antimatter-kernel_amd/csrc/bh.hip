@@ -191,19 +191,6 @@ __device__ inline int WaveMin(int v) {
   return v;
 }
 
-__device__ __forceinline__ void PairF32Acc(float4 th, float4 tl, float4 sh, float4 sl, float& ax, float& ay,
-                                           float& az) {
-  const float dx = (th.x - sh.x) + (tl.x - sl.x);
-  const float dy = (th.y - sh.y) + (tl.y - sl.y);
-  const float dz = (th.z - sh.z) + (tl.z - sl.z);
-  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
-  const float ri = __builtin_amdgcn_rsqf(r2);
-  const float s = (sh.w * ri) * (ri * ri);
-  ax = fmaf(s, dx, ax);
-  ay = fmaf(s, dy, ay);
-  az = fmaf(s, dz, az);
-}
-
 // AddPointCharge (barnes-hut.cc:19-27) without contraction.
 __device__ __forceinline__ void PairF64Acc(double tx, double ty, double tz, double tq, double sx, double sy,
                                            double sz, double sq, double& ax, double& ay, double& az) {
@@ -255,7 +242,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
             if (j == i) continue;
             const float4 sh = src[2 * j];
             if (sh.w == 0.f) continue;
-            PairF32Acc(th, tl, sh, src[2 * j + 1], fx, fy, fz);
+            PairF32(th, tl, sh, src[2 * j + 1], fx, fy, fz);
             ++cnt;
           }
           ax += fx;
@@ -286,8 +273,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
           if (MODE == 0) {
             const WalkNodeF32 f = nodes32[c];
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            if (nd.qp > 0.0) { PairF32Acc(th, tl, f.p_hi, f.p_lo, fx, fy, fz); ++cnt; }
-            if (nd.qn > 0.0) { PairF32Acc(th, tl, f.n_hi, f.n_lo, fx, fy, fz); ++cnt; }
+            if (nd.qp > 0.0) { PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz); ++cnt; }
+            if (nd.qn > 0.0) { PairF32(th, tl, f.n_hi, f.n_lo, fx, fy, fz); ++cnt; }
             ax += fx;
             ay += fy;
             az += fz;

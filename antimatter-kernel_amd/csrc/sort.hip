@@ -203,6 +203,23 @@ int ExclusiveScan(am_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
   return AM_OK;
 }
 
+// Sizes every scratch buffer a scan of n counts and a sort of n keys will ask for, so that a
+// later call allocates nothing (required before stream capture).
+int ReserveSortScratch(am_ctx* ctx, int64_t n) {
+  const int64_t num_tiles = (n + kSortTile - 1) / kSortTile;
+  const int64_t hist = 256 * num_tiles;
+  const int64_t scan_n = std::max<int64_t>(n, hist);
+  const int64_t scan_tiles = (scan_n + kScanTileN - 1) / kScanTileN;
+  void* p;
+  int rc;
+  if ((rc = Scratch(ctx, "sort_keys", (size_t)n * 8, &p))) return rc;
+  if ((rc = Scratch(ctx, "sort_vals", (size_t)n * 4, &p))) return rc;
+  if ((rc = Scratch(ctx, "sort_hist", (size_t)hist * 4, &p))) return rc;
+  if ((rc = Scratch(ctx, "sort_offs", (size_t)(hist + 1) * 4, &p))) return rc;
+  if ((rc = Scratch(ctx, "scan_tiles", (size_t)(2 * scan_tiles + 1) * 4, &p))) return rc;
+  return AM_OK;
+}
+
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
   if (n <= 1) return AM_OK;
   if (n > 0x7fffffffLL) return Fail(AM_ERR_CAPACITY, "radix sort: n too large");

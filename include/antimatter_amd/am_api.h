@@ -196,6 +196,35 @@ int am_is_antiparticle_pair(int species_a, int species_b);
 int am_compute_total_energy(am_store* store, const am_trap_field* field, const am_energy_settings* settings,
                             am_energy_history* history, am_energy_breakdown* out);
 
+/* ---- batched ensemble of independent traps (no single reference counterpart: each trap is one
+ * reference Simulate loop, src/antimatter-kernel.cc:153-197, with its own field and settings) ---- */
+typedef struct am_ensemble am_ensemble;
+typedef struct {
+  int64_t begin, end;          /* particle range [begin, end) of the trap in the store */
+  double b0, trap_voltage, d;  /* PenningTrapField */
+  double dt;                   /* Lorentz step (PhysicsParams dt rule per trap) */
+  double tau_scale, bath_temp; /* CyclotronCooling (tau_scale <= 0: off) */
+  int32_t annihilation;        /* AnnihilationInteraction attached */
+  double contact_radius, probability;
+} am_trap_desc;
+/* Creates an ensemble over ascending, disjoint trap ranges of a store (writes the store's trap ids).
+ * contact_capacity: candidate-pair buffer of the contact step (<= 0: 16 per particle).  Per step:
+ * all-pairs forces inside each trap (AM_FP32 arithmetic), LorentzForce + CyclotronCooling,
+ * AnnihilationInteraction in canonical order.  Draws of trap t are keyed by
+ * seed + t * 0x9E3779B97F4A7C15 and trap-local particle indices. */
+int am_ensemble_create(am_store* store, int32_t n_traps, const am_trap_desc* traps, int64_t contact_capacity,
+                       am_ensemble** out);
+int am_ensemble_destroy(am_ensemble* ens);
+/* Source-chunk length of the force work items (>= 128). */
+int am_ensemble_set_chunk(am_ensemble* ens, int32_t chunk);
+/* Autotuner: times each candidate chunk length (NULL: 1024..32768) on the device, keeps the fastest. */
+int am_ensemble_autotune(am_ensemble* ens, const int32_t* candidates, int32_t n_candidates, int32_t* chosen,
+                         double* times_ms);
+/* Advances every trap `steps` steps; use_graph replays one hipGraph-captured step. */
+int am_ensemble_run(am_ensemble* ens, int32_t steps, uint64_t seed, int32_t use_graph);
+int am_ensemble_stats(am_ensemble* ens, int64_t* steps_done, int64_t* annihilated, int32_t* chunk, int64_t* n_items,
+                      double* pair_interactions_per_step);
+
 /* ---- diagnostics (no reference counterpart: checks of the device building blocks) ------------ */
 /* Exclusive prefix sum of n uint32 on the device; out receives n + 1 values (out[n] = total). */
 int am_debug_exclusive_scan(am_ctx* ctx, const uint32_t* in, int64_t n, uint32_t* out);
