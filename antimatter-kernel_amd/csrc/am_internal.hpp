@@ -89,6 +89,11 @@ struct am_store {
   // with x_units = x * 2^20 = x_hi + x_lo.  Refreshed by every kernel that moves particles.
   // Allocated for PadN(capacity) particles; entries [n, PadN(n)) are kept zero.
   float4* src = nullptr;
+  // Spatial order cache of the sorted all-pairs kernel: perm[k] = particle at sorted position k,
+  // rebuilt every few calls (the order only affects speed; the near/far split is exact per call).
+  uint32_t* perm = nullptr;
+  int64_t perm_n = -1;
+  int perm_age = 0;
 };
 
 namespace am {

@@ -292,7 +292,7 @@ int am_store_destroy(am_store* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   void* ptrs[] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->q, s->m,
-                  s->species, s->flags, s->trap, s->src};
+                  s->species, s->flags, s->trap, s->src, s->perm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -307,6 +307,7 @@ int am_store_upload(am_store* s, int64_t n, const double* pos, const double* vel
   hipStream_t st = s->ctx->stream;
   AM_HIP(hipSetDevice(s->ctx->device));
   s->n = n;
+  s->perm_n = -1;  // spatial order must be rebuilt
   if (n == 0) return AM_OK;
   std::vector<double> hq(n), hm(n);
   std::vector<uint8_t> hf(n);

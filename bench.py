@@ -160,7 +160,8 @@ def main():
                         "B0=3 T, U0=50 V, z0=rho0=0.01 m",
             "n_particles": n,
             "dt_s": dt,
-            "pair_arithmetic": "fp32 hi/lo coordinates, fp64 per-128-source tile accumulation",
+            "pair_arithmetic": "fp32: hi/lo coordinates for near tiles, tile-local coordinates for far tiles; "
+                               "fp64 per-128-source tile accumulation",
             "update_arithmetic": "fp64",
         },
         "force_rel_err_l2": err,

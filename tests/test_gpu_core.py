@@ -95,6 +95,26 @@ def test_allpairs_fp32_full_config2_size(ctx):
     s.close()
 
 
+def test_allpairs_fp32_sorted_path_with_dead_and_neutral(ctx):
+    """The spatially sorted kernel (N >= 4096): annihilated and neutral particles, repeated calls
+    (cached order), then a re-upload of a different cloud."""
+    rng = np.random.default_rng(40)
+    for n in (5000, 12345):
+        pos = rng.uniform(-1, 1, (n, 3)) * np.array([1e-3, 2e-3, 4e-3])
+        sp = rng.choice([0, 1, 2, 3, 4], n).astype(np.uint8)
+        ann = (rng.random(n) < 0.1).astype(np.uint8)
+        s = upload(ctx, pos, np.zeros_like(pos), sp, ann, cap=20000)
+        m, q = oracle.mass_charge(sp)
+        ref = oracle.allpairs_forces(pos, q, ann)
+        for _ in range(3):
+            s.compute_allpairs_coulomb_forces(am.FP32)
+            f = s.forces()
+            live = ann == 0
+            assert rel_l2(f[live], ref[live]) < 1e-5
+            assert np.all(f[(sp == 2) & live] == 0)
+        s.close()
+
+
 def test_allpairs_edge_cases(ctx):
     s = am.Store(ctx, 16)
     s.upload(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.uint8))
