@@ -108,6 +108,19 @@ SIGNATURES = {
     "am_ensemble_autotune": (_int, [_vp, _vp, _i32, _vp, _vp]),
     "am_ensemble_run": (_int, [_vp, _i32, _u64, _i32]),
     "am_ensemble_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "am_mlp_create": (_int, [_vp, _u64, _vp]),
+    "am_mlp_destroy": (_int, [_vp]),
+    "am_mlp_num_params": (_int, []),
+    "am_mlp_set_params": (_int, [_vp, _vp]),
+    "am_mlp_get_params": (_int, [_vp, _vp]),
+    "am_mlp_set_normalization": (_int, [_vp, _vp, _vp, _vp]),
+    "am_mlp_set_baseline": (_int, [_vp, _int]),
+    "am_mlp_quadrupole_baseline": (_int, [_vp, _i64, _vp]),
+    "am_mlp_gradients": (_int, [_vp, _vp, _vp, _i32, _vp, _vp]),
+    "am_mlp_train": (_int, [_vp, _vp, _vp, _i64, _i32, _i32, ctypes.c_float, _u64, _vp]),
+    "am_mlp_infer": (_int, [_vp, _vp, _i32, _vp]),
+    "am_bh_learned_training_data": (_int, [_vp, _d, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "am_compute_barnes_hut_learned_forces": (_int, [_vp, _d, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp]),
     "am_debug_exclusive_scan": (_int, [_vp, _vp, _i64, _vp]),
     "am_debug_radix_sort": (_int, [_vp, _vp, _vp, _i64, _int]),
 }

@@ -27,7 +27,7 @@
 #include <cmath>
 #include <vector>
 
-#include "octree.hpp"
+#include "bh.hpp"
 
 namespace am {
 
@@ -126,22 +126,82 @@ __global__ void MomentKernel(int n0, int n1, NodeArrays nd, Moments mo, const ui
 }
 
 // Walk-ready node record in DFS preorder.
-struct WalkNode {
-  double cx, cy, cz, half;     // geometric cube
-  double pqx, pqy, pqz, qp;    // positive charge centre and charge
-  double nqx, nqy, nqz, qn;    // negative charge centre and |charge|
-  int b, e, skip, leaf;        // particle range in the sorted order, subtree skip pointer
+// Second moments of each sign's charges about its charge centre, S = sum |q| (x - c)(x - c)^T
+// (xx, yy, zz, xy, xz, yz), bottom-up with the parallel-axis shift; used by the learned
+// far-field correction only.
+struct SecondMoments {  // BFS order, 6 per node per sign
+  double* sp;
+  double* sn;
 };
-struct WalkNodeF32 {  // FP32 hi/lo mirror of the monopoles in units (charge in e)
-  float4 p_hi, p_lo, n_hi, n_lo;
-};
+
+__global__ void SecondMomentKernel(int n0, int n1, NodeArrays nd, Moments mo, SecondMoments sm,
+                                   const uint32_t* __restrict__ vals, const double* __restrict__ x,
+                                   const double* __restrict__ y, const double* __restrict__ z,
+                                   const double* __restrict__ q) {
+  const int node = n0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= n1) return;
+  double ap[6] = {0, 0, 0, 0, 0, 0}, an[6] = {0, 0, 0, 0, 0, 0};
+  const double cp[3] = {mo.cpx[node], mo.cpy[node], mo.cpz[node]};
+  const double cn[3] = {mo.cnx[node], mo.cny[node], mo.cnz[node]};
+  auto add = [](double* a, double w, double dx, double dy, double dz) {
+    a[0] += w * dx * dx;
+    a[1] += w * dy * dy;
+    a[2] += w * dz * dz;
+    a[3] += w * dx * dy;
+    a[4] += w * dx * dz;
+    a[5] += w * dy * dz;
+  };
+  if (nd.leaf[node]) {
+    for (int k = nd.b[node]; k < nd.e[node]; ++k) {
+      const uint32_t j = vals[k];
+      const double qj = q[j];
+      if (qj > 0.0) add(ap, qj, x[j] - cp[0], y[j] - cp[1], z[j] - cp[2]);
+      else if (qj < 0.0) add(an, -qj, x[j] - cn[0], y[j] - cn[1], z[j] - cn[2]);
+    }
+  } else {
+    const int c0 = nd.first_child[node], c1 = c0 + nd.nchild[node];
+    for (int c = c0; c < c1; ++c) {
+      if (mo.qp[c] > 0.0) {
+        for (int k = 0; k < 6; ++k) ap[k] += sm.sp[6 * c + k];
+        add(ap, mo.qp[c], mo.cpx[c] - cp[0], mo.cpy[c] - cp[1], mo.cpz[c] - cp[2]);
+      }
+      if (mo.qn[c] > 0.0) {
+        for (int k = 0; k < 6; ++k) an[k] += sm.sn[6 * c + k];
+        add(an, mo.qn[c], mo.cnx[c] - cn[0], mo.cny[c] - cn[1], mo.cnz[c] - cn[2]);
+      }
+    }
+  }
+  for (int k = 0; k < 6; ++k) {
+    sm.sp[6 * node + k] = ap[k];
+    sm.sn[6 * node + k] = an[k];
+  }
+}
 
 __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const uint64_t* __restrict__ skeys,
                                      const uint32_t* __restrict__ pre, WalkNode* __restrict__ out,
-                                     WalkNodeF32* __restrict__ out32) {
+                                     WalkNodeF32* __restrict__ out32, SecondMoments sm,
+                                     FeatNode* __restrict__ feat) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nn) return;
   const int node = (int)pre[p];
+  if (feat) {
+    FeatNode f;
+    const double h = nd.half[node];
+    const double cg[3] = {nd.cx[node], nd.cy[node], nd.cz[node]};
+    const double cs[2][3] = {{mo.cpx[node], mo.cpy[node], mo.cpz[node]}, {mo.cnx[node], mo.cny[node], mo.cnz[node]}};
+    const double Q[2] = {mo.qp[node], mo.qn[node]};
+    const double* S[2] = {sm.sp + 6 * node, sm.sn + 6 * node};
+    float* dst[2] = {f.fp, f.fn};
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      for (int d = 0; d < 3; ++d) dst[sgn][d] = Q[sgn] > 0.0 ? (float)((cs[sgn][d] - cg[d]) / h) : 0.f;
+      const double norm = Q[sgn] > 0.0 ? 1.0 / (Q[sgn] * h * h) : 0.0;
+      for (int k = 0; k < 6; ++k) dst[sgn][3 + k] = (float)(S[sgn][k] * norm);
+      dst[sgn][9] = (float)(Q[sgn] / kElementaryCharge);
+    }
+    f.h = (float)(h * kUnitScale);
+    f.pad = 0.f;
+    feat[p] = f;
+  }
   WalkNode w;
   w.cx = nd.cx[node];
   w.cy = nd.cy[node];
@@ -214,10 +274,14 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
                                                      const double* __restrict__ z, const double* __restrict__ q,
                                                      const float4* __restrict__ src, double theta2,
                                                      double* __restrict__ out_x, double* __restrict__ out_y,
-                                                     double* __restrict__ out_z, int64_t* __restrict__ counts) {
+                                                     double* __restrict__ out_z, int64_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec,
+                                                     int rec_cap, uint32_t* __restrict__ rec_cnt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < m;
-  const uint32_t i = order[valid ? k : m - 1];
+  const int kc = valid ? k : m - 1;
+  const uint32_t i = order[tsel ? tsel[kc] : kc];
+  uint32_t nrec = 0;
   const double px = x[i], py = y[i], pz = z[i], qi = q[i];
   const float4 th = src[2 * i], tl = src[2 * i + 1];
   // ComputeForce returns zero for a neutral target (barnes-hut.cc:168); the energy sum skips
@@ -278,6 +342,10 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
             ax += fx;
             ay += fy;
             az += fz;
+            if (rec) {
+              if (nrec < (uint32_t)rec_cap) rec[(size_t)k * rec_cap + nrec] = (uint32_t)c;
+              ++nrec;
+            }
           } else if (MODE == 1) {
             if (nd.qp > 0.0) { PairF64Acc(px, py, pz, qi, nd.pqx, nd.pqy, nd.pqz, nd.qp, ax, ay, az); ++cnt; }
             if (nd.qn > 0.0) { PairF64Acc(px, py, pz, qi, nd.nqx, nd.nqy, nd.nqz, -nd.qn, ax, ay, az); ++cnt; }
@@ -305,6 +373,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
     c = __ballot(opened) ? c + 1 : WaveMin(next);
   }
   if (!valid) return;
+  if (rec_cnt) rec_cnt[k] = nrec;
   if (MODE == 0) {
     const double scale = qi != 0.0 ? kCoulomb * qi * kElementaryCharge * kInvUnit2 : 0.0;
     out_x[i] = scale * ax;
@@ -321,18 +390,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
-struct BHTree {
-  Octree tree;
-  int m = 0;            // alive particles
-  int nn = 0;           // nodes
-  uint32_t* order = nullptr;  // sorted position -> particle index
-  WalkNode* nodes = nullptr;
-  WalkNodeF32* nodes32 = nullptr;
-  int max_depth = 0;
-  uint32_t deep_overflow = 0;
-};
-
-int BuildBHTree(am_store* s, int leaf_cap, BHTree* t) {
+int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = s->n;
@@ -365,19 +423,34 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t) {
       *dst[a] = (double*)p;
     }
   }
+  SecondMoments sm{nullptr, nullptr};
+  void* featp = nullptr;
+  if (with_features) {
+    void *a, *b;
+    if ((rc = Scratch(ctx, "bh_s2p", (size_t)nn * 6 * 8, &a))) return rc;
+    if ((rc = Scratch(ctx, "bh_s2n", (size_t)nn * 6 * 8, &b))) return rc;
+    if ((rc = Scratch(ctx, "bh_feat", (size_t)nn * sizeof(FeatNode), &featp))) return rc;
+    sm.sp = (double*)a;
+    sm.sn = (double*)b;
+  }
   for (int d = nlevels - 1; d >= 0; --d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
     if (n1 <= n0) continue;
     hipLaunchKernelGGL(MomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd, mo,
                        (const uint32_t*)o.order, s->x, s->y, s->z, s->q);
+    if (with_features)
+      hipLaunchKernelGGL(SecondMomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd, mo, sm,
+                         (const uint32_t*)o.order, s->x, s->y, s->z, s->q);
     AM_HIP(hipGetLastError());
   }
   void *wn, *wn32;
   if ((rc = Scratch(ctx, "bh_walknodes", (size_t)nn * sizeof(WalkNode), &wn))) return rc;
   if ((rc = Scratch(ctx, "bh_walknodes32", (size_t)nn * sizeof(WalkNodeF32), &wn32))) return rc;
   hipLaunchKernelGGL(GatherPreorderKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, o.nd, mo,
-                     (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeF32*)wn32);
+                     (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeF32*)wn32, sm,
+                     (FeatNode*)featp);
   AM_HIP(hipGetLastError());
+  t->feat = (FeatNode*)featp;
   t->nn = nn;
   t->order = o.order;
   t->nodes = (WalkNode*)wn;
@@ -402,13 +475,30 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   const dim3 grid(Blocks(t.m, 256)), block(256);
   if (mode == 0)
     hipLaunchKernelGGL(BHWalkKernel<0>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts);
+                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       nullptr, 0, nullptr);
   else if (mode == 1)
     hipLaunchKernelGGL(BHWalkKernel<1>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts);
+                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       nullptr, 0, nullptr);
   else
     hipLaunchKernelGGL(BHWalkKernel<2>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts);
+                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       nullptr, 0, nullptr);
+  TimerEnd(ctx, slot);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* tsel, int n_targets, double* ox,
+                    double* oy, double* oz, uint32_t* rec, int rec_cap, uint32_t* rec_cnt) {
+  if (n_targets <= 0) return AM_OK;
+  am_ctx* ctx = s->ctx;
+  int slot;
+  TimerBegin(ctx, "bh_walk_f32", &slot);
+  hipLaunchKernelGGL(BHWalkKernel<0>, dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
+                     t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, theta * theta, ox, oy, oz,
+                     (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;

@@ -1,0 +1,52 @@
+// Barnes-Hut tree records shared by the walk (bh.hip) and the learned correction (mlp.hip).
+#pragma once
+
+#include "octree.hpp"
+
+namespace am {
+
+// Learned-correction features of a node (preorder), per sign s: the charge centre's offset from the
+// cube centre in half-widths (3), the normalised second moments S_s / (Q_s h^2) (6) and Q_s / e.
+struct FeatNode {
+  float fp[10];
+  float fn[10];
+  float h;  // half width in units of 2^-20 m
+  float pad;
+};
+
+struct WalkNode {
+  double cx, cy, cz, half;     // geometric cube
+  double pqx, pqy, pqz, qp;    // positive charge centre and charge
+  double nqx, nqy, nqz, qn;    // negative charge centre and |charge|
+  int b, e, skip, leaf;        // particle range in the sorted order, subtree skip pointer
+};
+struct WalkNodeF32 {  // FP32 hi/lo mirror of the monopoles in units (charge in e)
+  float4 p_hi, p_lo, n_hi, n_lo;
+};
+
+
+struct BHTree {
+  Octree tree;
+  int m = 0;            // alive particles
+  int nn = 0;           // nodes
+  uint32_t* order = nullptr;  // sorted position -> particle index
+  WalkNode* nodes = nullptr;
+  WalkNodeF32* nodes32 = nullptr;
+  FeatNode* feat = nullptr;  // only when built with features
+  int max_depth = 0;
+  uint32_t deep_overflow = 0;
+};
+
+// Builds the tree of the store's alive particles (leaf capacity leaf_cap); with_features adds the
+// per-node features of the learned correction.
+int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features = false);
+
+// FP32 Barnes-Hut force walk over the targets tsel[0..n_targets) (sorted positions; NULL: all),
+// writing forces of those targets; with rec != NULL the accepted nodes of target k are recorded in
+// rec[k * rec_cap + c] (c < rec_cap) and their number in rec_cnt[k].
+int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* tsel, int n_targets, double* ox,
+                    double* oy, double* oz, uint32_t* rec, int rec_cap, uint32_t* rec_cnt);
+
+
+
+}  // namespace am

@@ -225,6 +225,56 @@ int am_ensemble_run(am_ensemble* ens, int32_t steps, uint64_t seed, int32_t use_
 int am_ensemble_stats(am_ensemble* ens, int64_t* steps_done, int64_t* annihilated, int32_t* chunk, int64_t* n_items,
                       double* pair_interactions_per_step);
 
+/* ---- learned far-field correction of Barnes-Hut (no reference counterpart: it corrects the
+ * dual-monopole approximation of the walk in src/interaction/barnes-hut.cc:183-221) -------------
+ * For each accepted node and charge sign s the walk's monopole error is predicted by a small MLP
+ * (16 -> 64 -> 64 -> 3, SiLU; BF16 weights on MFMA, FP32 accumulation) from 13 features of the
+ * node's sign-s charges: r/R (3), h/R, (c_s - c_geo)/h (3), S_s/(Q_s h^2) (6), where r = target -
+ * charge centre, R = |r|, h = half width, S_s the second moments about c_s.  Output y is the
+ * residual in units of sigma_s Q_s h^2 / R^4 (Coulomb constant and target charge factored out). */
+typedef struct am_mlp am_mlp;
+/* New network on the context's device; weights drawn from seed (He normal). */
+int am_mlp_create(am_ctx* ctx, uint64_t seed, am_mlp** out);
+int am_mlp_destroy(am_mlp* net);
+/* Number of FP32 parameters (W1 16x64, b1, W2 64x64, b2, W3 64x3, b3; row-major [in][out]). */
+int am_mlp_num_params(void);
+int am_mlp_set_params(am_mlp* net, const float* params);
+int am_mlp_get_params(am_mlp* net, float* params);
+/* Prediction mode: 0 (the default) predicts y with the network alone; quadrupole != 0 predicts
+ * y = y_q(features) + network, where y_q = -(Qt u - 5/2 (u.Qt.u) u), Qt = 3 S - tr(S) I, is the
+ * analytic quadrupole term and the network is trained on y - y_q. */
+int am_mlp_set_baseline(am_mlp* net, int quadrupole);
+/* The analytic quadrupole term y_q of raw feature rows X[rows][16] -> Y[rows][3] (host). */
+int am_mlp_quadrupole_baseline(const float* X, int64_t rows, float* Y);
+/* Input standardisation x' = (x - mean) / std (16 each) and output scale y = out_scale * y' (3). */
+int am_mlp_set_normalization(am_mlp* net, const float* in_mean, const float* in_std, const float* out_scale);
+/* Mean-squared-error gradient of a batch of normalised inputs X[rows][16] against normalised
+ * targets Y[rows][3] (forward with BF16-rounded weights and activations): grad[num_params], loss. */
+int am_mlp_gradients(am_mlp* net, const float* X, const float* Y, int32_t rows, float* grad, float* loss);
+/* Adam training (lr, beta 0.9/0.999) on raw features X[rows][16] and labels Y[rows][3] with the
+ * network's normalisation (targets (Y - y_q) / out_scale in the quadrupole mode): `epochs` passes of
+ * `batch`-row minibatches in a seeded shuffled order; losses (optional) receives the mean normalised
+ * loss per epoch. */
+int am_mlp_train(am_mlp* net, const float* X, const float* Y, int64_t rows, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float* losses);
+/* MFMA inference on raw features X[rows][16] -> predictions Y[rows][3] (host arrays). */
+int am_mlp_infer(am_mlp* net, const float* X, int32_t rows, float* Y);
+/* Training rows from the store's cloud: Barnes-Hut walk at theta over n_targets particles (every
+ * stride-th in the tree order from offset), up to rec_cap accepted nodes each, one row per node and
+ * charge sign present: raw features X[16], the exact residual in units of the label scale Y[3],
+ * and, each optional (NULL skips), the target particle index, the node (DFS preorder position, as
+ * in am_bh_tree_download) and the label scale sigma_s Q_s h^2 / R^4 (e / (2^-20 m)^2).  Outputs
+ * hold 2*n_targets*rec_cap rows; *rows_out receives the number written. */
+int am_bh_learned_training_data(am_store* store, double theta, int32_t n_targets, int32_t stride, int32_t offset,
+                                int32_t rec_cap, float* X, float* Y, int32_t* target, int32_t* node, float* scale,
+                                int64_t* rows_out);
+/* ComputeBarnesHutCoulombForces (AM_FP32 walk) plus the learned correction of every accepted node.
+ * A row whose |y| exceeds ybound or whose h/R exceeds amax takes the exact residual instead.
+ * rec_cap: nodes recorded per particle (later ones keep the plain monopole).  stats (optional, 3):
+ * rows corrected by the model, rows that fell back, particles whose node list overflowed. */
+int am_compute_barnes_hut_learned_forces(am_store* store, double theta, am_mlp* net, int32_t rec_cap, float ybound,
+                                         float amax, int64_t* stats);
+
 /* ---- diagnostics (no reference counterpart: checks of the device building blocks) ------------ */
 /* Exclusive prefix sum of n uint32 on the device; out receives n + 1 values (out[n] = total). */
 int am_debug_exclusive_scan(am_ctx* ctx, const uint32_t* in, int64_t n, uint32_t* out);

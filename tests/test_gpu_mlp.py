@@ -1,0 +1,223 @@
+"""GPU checks of the learned Barnes-Hut correction against the oracle (oracle/mlp_ref.py and the
+FP64 direct sum): MFMA inference, training gradients, training data (features, labels, the
+reconstruction of the exact force from the labels), the exact fallback, and what a trained network
+gains over the plain walk.
+
+Tolerances: inference 1e-2 of the output range (bf16 weights and activations on both sides; the
+accumulation order differs); gradients 2e-3 relative L2 (FP32 accumulation vs FP64); forces
+1e-5 relative (FP32 pair arithmetic)."""
+import numpy as np
+import pytest
+
+import antimatter_amd as am
+from antimatter_amd import mlp
+import oracle
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+UNIT = 2.0 ** 20
+E = oracle.E_CHARGE
+K = oracle.K_COULOMB
+
+
+def random_net(seed):
+    rng = np.random.default_rng(seed)
+    p = M.unpack(np.zeros(M.N_PARAMS))
+    p["W1"] = rng.normal(0, 0.5, p["W1"].shape)
+    p["b1"] = rng.normal(0, 0.1, p["b1"].shape)
+    p["W2"] = rng.normal(0, 0.2, p["W2"].shape)
+    p["b2"] = rng.normal(0, 0.1, p["b2"].shape)
+    p["W3"] = rng.normal(0, 0.3, p["W3"].shape)
+    p["b3"] = rng.normal(0, 0.1, p["b3"].shape)
+    return M.pack(p).astype(np.float32)
+
+
+def pm_cloud(n, seed, radii=(2e-3, 2e-3, 8e-3), temperature=4.0):
+    """50/50 positrons and electrons in the ellipsoidal cloud of the Barnes-Hut workload."""
+    pops = {"positron": n // 2, "electron": n - n // 2}
+    temps = {"positron": temperature, "electron": temperature}
+    return am.initial_conditions(pops, temps, radii, seed)
+
+
+def test_layout_matches_oracle():
+    assert mlp.num_params() == M.N_PARAMS == 5443
+    sl = mlp.param_slices()
+    off = 0
+    for name, shape in M.SHAPES:
+        assert sl[name] == (off, shape)
+        off += int(np.prod(shape))
+
+
+@pytest.mark.parametrize("rows,quadrupole", [(1, False), (31, False), (32, False), (100, True), (4096, False),
+                                             (4096, True)])
+def test_mfma_inference_matches_oracle(ctx, rows, quadrupole):
+    net = mlp.Mlp(ctx, seed=5, quadrupole=quadrupole)
+    w = random_net(11)
+    net.params = w
+    rng = np.random.default_rng(rows)
+    mean = rng.normal(0, 0.3, 16).astype(np.float32)
+    std = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    scale = np.array([0.5, 2.0, 1.5], np.float32)
+    net.set_normalization(mean, std, scale)
+    X = rng.normal(0, 1.5, (rows, 16)).astype(np.float32)
+    y = net.infer(X)
+    Xn = (X - mean) / std  # float32, as on the device
+    ref = M.forward(w.astype(np.float64), Xn, rounding=True)[0] * scale
+    if quadrupole:
+        ref = ref + M.quadrupole_from_features(X)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"rows={rows}: max err {err:.2e} of the output range")
+    assert err < 1e-2
+    if rows >= 100:
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 3e-3
+    net.close()
+
+
+def test_gradients_match_oracle(ctx):
+    net = mlp.Mlp(ctx, seed=5, quadrupole=False)
+    w = random_net(12)
+    net.params = w
+    rng = np.random.default_rng(3)
+    for rows in (5, 200, 1000):
+        X = rng.normal(size=(rows, 16)).astype(np.float32)
+        Y = rng.normal(size=(rows, 3)).astype(np.float32)
+        g, loss = net.gradients(X, Y)
+        rloss, rg = M.loss_and_grad(w.astype(np.float64), X, Y, rounding=True)
+        assert loss == pytest.approx(rloss, rel=1e-4)
+        err = np.linalg.norm(g - rg) / np.linalg.norm(rg)
+        print(f"rows={rows}: gradient rel L2 {err:.2e}")
+        assert err < 2e-3
+        for name, (off, shape) in mlp.param_slices().items():
+            size = int(np.prod(shape))
+            a, b = g[off:off + size], rg[off:off + size]
+            assert np.linalg.norm(a - b) <= 5e-3 * np.linalg.norm(b) + 1e-7, name
+    net.close()
+
+
+def test_training_fits_and_is_deterministic(ctx):
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(20000, 16)).astype(np.float32)
+    A = rng.normal(size=(16, 3))
+    Y = np.tanh(X @ A * 0.5).astype(np.float32) + 0.1 * (X[:, :3] ** 2)
+    params = []
+    for _ in range(2):
+        net = mlp.Mlp(ctx, seed=9, quadrupole=False)
+        net.fit_normalization(X, Y)
+        losses = net.train(X, Y, epochs=8, batch=512, lr=3e-3, seed=4)
+        print("losses", losses)
+        assert losses[-1] < 0.1 * losses[0]
+        params.append(net.params)
+        # MFMA inference reproduces the fit the training kernels reached
+        pred = net.infer(X[:4000])
+        mse = float((((pred - Y[:4000]) / net.out_scale) ** 2).mean())
+        print(f"normalised inference MSE {mse:.4f}")
+        assert mse < 1.2 * losses[-1]
+        net.close()
+    np.testing.assert_array_equal(params[0], params[1])
+
+
+@pytest.fixture(scope="module")
+def small_cloud(ctx):
+    pos, vel, sp = pm_cloud(4096, seed=21)
+    s = am.Store(ctx, len(sp))
+    s.upload(pos, vel, sp)
+    yield pos, sp, s
+    s.close()
+
+
+def test_training_rows_match_oracle(ctx, small_cloud):
+    pos, sp, s = small_cloud
+    theta = 0.9
+    m, q = oracle.mass_charge(sp)
+    qe = q / E
+    s.compute_barnes_hut_coulomb_forces(theta, am.FP32)  # same tree as the data build
+    f_bh = s.forces()
+    tree = s.bh_tree()
+    order, begin, end = tree["order"], tree["begin"], tree["end"]
+    X, Y, T, N, S = mlp.training_data(s, theta, n_targets=64, stride=64, offset=3, rec_cap=4096)
+    assert len(X) > 1000
+    assert np.all(X[:, 13:] == 0)
+    pu = pos * UNIT
+    rng = np.random.default_rng(0)
+    for r in rng.choice(len(X), 300, replace=False):
+        i, c = T[r], N[r]
+        members = order[begin[c]:end[c]]
+        assert i not in members  # the MAC at theta < 1.15 never accepts a node holding the target
+        f = X[r]
+        sign = 0 if S[r] > 0 else 1
+        sel = members[(qe[members] > 0) if sign == 0 else (qe[members] < 0)]
+        Q, cs, S6 = M.second_moments(pu[sel], np.abs(qe[sel]))
+        rvec = pu[i] - cs
+        R = np.linalg.norm(rvec)
+        np.testing.assert_allclose(f[0:3], rvec / R, atol=2e-6)
+        h = f[3] * R
+        # the label scale sigma Q h^2 / R^4 gives the same half width as the h/R feature
+        h2 = np.sqrt(abs(S[r]) * R ** 4 / Q)
+        assert h2 == pytest.approx(h, rel=1e-5)
+        np.testing.assert_allclose(f[7:13] * h * h, S6 / Q, rtol=1e-4, atol=1e-5 * h * h)
+        assert np.all(np.abs(f[4:7]) <= 1.0 + 1e-6)
+        res = M.node_residual(pu[i], pu[members], qe[members], sign)
+        np.testing.assert_allclose(Y[r] * S[r], res, rtol=1e-5, atol=1e-6 * np.abs(res).max())
+    # labels reconstruct the exact force: F_BH + sum over the target's rows = F_direct
+    tgt = np.unique(T)
+    ref = oracle.allpairs_forces_subset(pos, q, None, tgt)
+    corr = np.zeros((len(tgt), 3))
+    idx = np.searchsorted(tgt, T)
+    np.add.at(corr, idx, (Y * S[:, None]).astype(np.float64))
+    rebuilt = f_bh[tgt] + K * q[tgt, None] * E * UNIT * UNIT * corr
+    err_bh = np.linalg.norm(f_bh[tgt] - ref) / np.linalg.norm(ref)
+    err = np.linalg.norm(rebuilt - ref) / np.linalg.norm(ref)
+    print(f"plain BH error {err_bh:.2e}; rebuilt from labels {err:.2e}")
+    assert err < 1e-5 < err_bh
+
+
+def test_fallback_everywhere_gives_the_direct_sum(ctx, small_cloud):
+    pos, sp, s = small_cloud
+    m, q = oracle.mass_charge(sp)
+    net = mlp.Mlp(ctx, seed=1)
+    ref = oracle.allpairs_forces(pos, q)
+    for ybound, amax in ((-1.0, 10.0), (100.0, 0.0)):
+        st = mlp.barnes_hut_learned_forces(s, 0.9, net, rec_cap=4096, ybound=ybound, amax=amax)
+        f = s.forces()
+        err = np.linalg.norm(f - ref) / np.linalg.norm(ref)
+        print(st, f"err {err:.2e}")
+        assert st["model_rows"] == 0 and st["fallback_rows"] > 0 and st["overflowed"] == 0
+        assert err < 1e-5
+    net.close()
+
+
+def test_trained_network_beats_plain_barnes_hut(ctx):
+    """Trained on four clouds, evaluated on a fifth (seeds differ), at theta = 0.6: the learned
+    correction lowers the force error of the plain walk at the same theta at least twofold, and
+    reaches the accuracy of the analytic quadrupole term (the quadrupole mode with zero weights)
+    from the residuals alone."""
+    theta = 0.6
+    n = 32768
+    net = mlp.Mlp(ctx, seed=3)
+    clouds = [pm_cloud(n, seed) for seed in (31, 32, 34, 35)]
+    rows, losses = mlp.train_on_clouds(ctx, clouds, theta, net, targets_per_cloud=512, rec_cap=4096, epochs=30,
+                                       batch=1024, lr=2e-3, seed=1)
+    print(f"{rows} rows; losses {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.2 * losses[0]
+    quad_only = mlp.Mlp(ctx, seed=3, quadrupole=True)
+    quad_only.params = np.zeros(mlp.num_params(), np.float32)
+    pos, vel, sp = pm_cloud(n, 33)
+    s = am.Store(ctx, n)
+    s.upload(pos, vel, sp)
+    m, q = oracle.mass_charge(sp)
+    tgt = np.random.default_rng(1).choice(n, 512, replace=False)
+    ref = oracle.allpairs_forces_subset(pos, q, None, tgt)
+    s.compute_barnes_hut_coulomb_forces(theta, am.FP32)
+    e_plain = np.linalg.norm(s.forces()[tgt] - ref) / np.linalg.norm(ref)
+    mlp.barnes_hut_learned_forces(s, theta, quad_only, rec_cap=4096, ybound=1e30, amax=1e30)
+    e_quad = np.linalg.norm(s.forces()[tgt] - ref) / np.linalg.norm(ref)
+    st = mlp.barnes_hut_learned_forces(s, theta, net, rec_cap=4096, ybound=1e30, amax=1e30)
+    e_learn = np.linalg.norm(s.forces()[tgt] - ref) / np.linalg.norm(ref)
+    print(f"plain {e_plain:.3e} quadrupole only {e_quad:.3e} learned {e_learn:.3e} stats {st}")
+    assert st["overflowed"] == 0 and st["fallback_rows"] == 0
+    assert e_learn < 0.5 * e_plain
+    assert e_learn < 1.25 * e_quad
+    s.close()
+    net.close()
+    quad_only.close()
