@@ -45,10 +45,16 @@ __host__ __device__ inline uint16_t Bf16Bits(float f) {
   memcpy(&u, &r, 4);
   return (uint16_t)(u >> 16);
 }
-__device__ inline float Sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+// SiLU and its derivative, shared by the training and inference kernels.
+__device__ inline float SigmoidFast(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float Silu(float z) { return z * SigmoidFast(z); }
+__device__ inline float SiluGrad(float z) {
+  const float s = SigmoidFast(z);
+  return s * (1.f + z * (1.f - s));
+}
 
 struct MlpNorm {
-  float in_mean[kIn], in_std[kIn], out_scale[kOut];
+  float in_mean[kIn], in_std[kIn], in_istd[kIn], out_scale[kOut];  // in_istd = 1 / in_std
 };
 
 // ---- training ------------------------------------------------------------------------------------
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(256) void MlpTrainTileKernel(int rows, const float*
     float acc = w[kOffB1 + j];
     for (int k = 0; k < kIn; ++k) acc = fmaf(xs[r][k], Bf16Round(w[kOffW1 + k * kH + j]), acc);
     z1[r][j] = acc;
-    a1[r][j] = Bf16Round(acc * Sigmoid(acc));
+    a1[r][j] = Bf16Round(Silu(acc));
   }
   __syncthreads();
   for (int e = t; e < kTrainRows * kH; e += 256) {
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(256) void MlpTrainTileKernel(int rows, const float*
     float acc = w[kOffB2 + j];
     for (int k = 0; k < kH; ++k) acc = fmaf(a1[r][k], Bf16Round(w[kOffW2 + k * kH + j]), acc);
     z2[r][j] = acc;
-    a2[r][j] = Bf16Round(acc * Sigmoid(acc));
+    a2[r][j] = Bf16Round(Silu(acc));
   }
   __syncthreads();
   float l = 0.f;
@@ -120,8 +126,7 @@ __global__ __launch_bounds__(256) void MlpTrainTileKernel(int rows, const float*
     const int r = e / kH, j = e % kH;
     float acc = 0.f;
     for (int o = 0; o < kOut; ++o) acc = fmaf(dp[r][o], Bf16Round(w[kOffW3 + j * kOut + o]), acc);
-    const float s = Sigmoid(z2[r][j]);
-    dz2[r][j] = acc * s * (1.f + z2[r][j] * (1.f - s));
+    dz2[r][j] = acc * SiluGrad(z2[r][j]);
   }
   __syncthreads();
   for (int e = t; e < kH * kH; e += 256) {
@@ -139,8 +144,7 @@ __global__ __launch_bounds__(256) void MlpTrainTileKernel(int rows, const float*
     const int r = e / kH, i = e % kH;
     float acc = 0.f;
     for (int j = 0; j < kH; ++j) acc = fmaf(dz2[r][j], Bf16Round(w[kOffW2 + i * kH + j]), acc);
-    const float s = Sigmoid(z1[r][i]);
-    dz1[r][i] = acc * s * (1.f + z1[r][i] * (1.f - s));
+    dz1[r][i] = acc * SiluGrad(z1[r][i]);
   }
   __syncthreads();
   for (int e = t; e < kIn * kH; e += 256) {
@@ -184,33 +188,43 @@ __global__ void AdamKernel(float* __restrict__ w, const float* __restrict__ grad
 }
 
 // ---- MFMA inference --------------------------------------------------------------------------------
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// v_mfma_f32_32x32x16_bf16 (gfx950): A 32x16, B 16x32, C/D 32x32 FP32.  Lane l (n = l%32, h = l/32)
+// holds A[n][8h+j] and B[8h+j][n] (j = 0..7) and D[8(v/4) + 4h + v%4][n] (v = 0..15).
+// The network runs transposed, H^T = W^T X^T, with the 32 rows of a batch as the N dimension: the
+// weights are the A operand, and a layer's output fragment is the next layer's B operand once the
+// hidden units are taken in the order in which D holds them,
+//     U(ks, h, j) = 16 ks + 8 (j/4) + 4 h + j%4    (k-step ks = 0..3, B position 8h + j),
+// i.e. B_ks[j] = D_{ks/2}[8 (ks%2) + j].  Activations never leave registers.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// B-operand fragments of the three weight matrices for v_mfma_f32_32x32x8bf16_1k: fragment
-// (layer, nblock, kblock) holds, for lane l, W[kb*8 + 4*(l/32) + j][nb*32 + l%32], j = 0..3.
-constexpr int kFragL1 = 2 * 2, kFragL2 = 2 * 8, kFragL3 = 1 * 8, kFrags = kFragL1 + kFragL2 + kFragL3;  // 28
+constexpr int kFragL1 = 2, kFragL2 = 8, kFragL3 = 4, kFrags = kFragL1 + kFragL2 + kFragL3;  // 14
 
+__host__ __device__ inline int HiddenOrder(int ks, int h, int j) { return 16 * ks + 8 * (j / 4) + 4 * h + (j % 4); }
+
+// A-operand fragment f for lane l (8 bf16): layer 1 (f = mb): W1[8h+j][32 mb + n]; layer 2
+// (f = 2 + 4 mb + ks): W2[U(ks,h,j)][32 mb + n]; layer 3 (f = 10 + ks): W3[U(ks,h,j)][n], n < 3.
 __global__ void MlpFragKernel(const float* __restrict__ w, uint16_t* __restrict__ frag) {
   const int f = blockIdx.x, l = threadIdx.x;  // 64 lanes
-  int layer, nb, kb;
-  if (f < kFragL1) { layer = 0; nb = f / 2; kb = f % 2; }
-  else if (f < kFragL1 + kFragL2) { layer = 1; nb = (f - kFragL1) / 8; kb = (f - kFragL1) % 8; }
-  else { layer = 2; nb = 0; kb = f - kFragL1 - kFragL2; }
-  for (int j = 0; j < 4; ++j) {
-    const int k = kb * 8 + 4 * (l / 32) + j;
-    const int n = nb * 32 + l % 32;
+  const int n = l % 32, h = l / 32;
+  for (int j = 0; j < 8; ++j) {
     float v = 0.f;
-    if (layer == 0) v = w[kOffW1 + k * kH + n];
-    else if (layer == 1) v = w[kOffW2 + k * kH + n];
-    else v = n < kOut ? w[kOffW3 + k * kOut + n] : 0.f;
-    frag[((size_t)f * 64 + l) * 4 + j] = Bf16Bits(v);
+    if (f < kFragL1) {
+      v = w[kOffW1 + (8 * h + j) * kH + 32 * f + n];
+    } else if (f < kFragL1 + kFragL2) {
+      const int mb = (f - kFragL1) / 4, ks = (f - kFragL1) % 4;
+      v = w[kOffW2 + HiddenOrder(ks, h, j) * kH + 32 * mb + n];
+    } else {
+      const int ks = f - kFragL1 - kFragL2;
+      v = n < kOut ? w[kOffW3 + HiddenOrder(ks, h, j) * kOut + n] : 0.f;
+    }
+    frag[((size_t)f * 64 + l) * 8 + j] = Bf16Bits(v);
   }
 }
 
 struct MlpDev {
-  const uint16_t* frag;  // kFrags x 64 lanes x 4 bf16
-  const float* w;        // FP32 master (biases used in FP32)
+  const uint16_t* frag;  // kFrags x 64 lanes x 8 bf16
+  const float* w;        // FP32 master (biases are used in FP32)
   MlpNorm norm;
   int baseline;          // add the analytic quadrupole term to the network output
 };
@@ -233,92 +247,94 @@ __host__ __device__ inline void QuadBaseline(const float* f, float y[3]) {
   y[2] = 2.5f * uqu * uz - quz;
 }
 
-// Runs the network on the 32 rows staged in LDS (xin[32][16], normalised, FP32) for one wave; the
-// outputs land in yout[32][4].  h: LDS scratch of 32 x 64 floats.
-__device__ inline void MlpForward32(const MlpDev& net, const bf16x4* wf, float (*xin)[kIn], float (*h)[kH],
-                                    float (*yout)[4]) {
+// Per-wave network state: weight fragments in registers, FP32 biases of the hidden layers in LDS.
+struct MlpWave {
+  bf16x8 wf[kFrags];
+  float b3[kOut], oscale[kOut];
+};
+
+__device__ inline void MlpWaveLoad(const MlpDev& net, float* bias_lds, MlpWave& s) {
   const int l = threadIdx.x & 63;
-  const int col = l & 31, khalf = 4 * (l >> 5);
-  // layer 1: K = 16 (2 k-blocks), N = 64 (2 n-blocks)
-  f32x16 acc[2];
-  for (int nb = 0; nb < 2; ++nb) {
-    for (int v = 0; v < 16; ++v) acc[nb][v] = 0.f;
-    for (int kb = 0; kb < 2; ++kb) {
-      bf16x4 a;
-      for (int j = 0; j < 4; ++j) a[j] = (short)Bf16Bits(xin[col][kb * 8 + khalf + j]);
-      acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wf[nb * 2 + kb], acc[nb], 0, 0, 0);
-    }
+  bias_lds[l] = net.w[kOffB1 + l];
+  bias_lds[kH + l] = net.w[kOffB2 + l];
+  const bf16x8* p = reinterpret_cast<const bf16x8*>(net.frag);
+  for (int f = 0; f < kFrags; ++f) s.wf[f] = p[f * 64 + l];
+  for (int o = 0; o < kOut; ++o) {
+    s.b3[o] = net.w[kOffB3 + o];
+    s.oscale[o] = net.norm.out_scale[o];
   }
-  __builtin_amdgcn_s_barrier();
-  for (int nb = 0; nb < 2; ++nb)
-    for (int v = 0; v < 16; ++v) {
-      const int row = 8 * (v / 4) + khalf + (v % 4), c = nb * 32 + col;
-      const float z = acc[nb][v] + net.w[kOffB1 + c];
-      h[row][c] = z * Sigmoid(z);
-    }
-  __syncthreads();
-  // layer 2: K = 64 (8 k-blocks), N = 64
-  for (int nb = 0; nb < 2; ++nb) {
-    for (int v = 0; v < 16; ++v) acc[nb][v] = 0.f;
-    for (int kb = 0; kb < 8; ++kb) {
-      bf16x4 a;
-      for (int j = 0; j < 4; ++j) a[j] = (short)Bf16Bits(h[col][kb * 8 + khalf + j]);
-      acc[nb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wf[kFragL1 + nb * 8 + kb], acc[nb], 0, 0, 0);
-    }
-  }
-  __syncthreads();
-  for (int nb = 0; nb < 2; ++nb)
-    for (int v = 0; v < 16; ++v) {
-      const int row = 8 * (v / 4) + khalf + (v % 4), c = nb * 32 + col;
-      const float z = acc[nb][v] + net.w[kOffB2 + c];
-      h[row][c] = z * Sigmoid(z);
-    }
-  __syncthreads();
-  // layer 3: K = 64, N = 3 (padded to 32)
-  f32x16 o;
-  for (int v = 0; v < 16; ++v) o[v] = 0.f;
-  for (int kb = 0; kb < 8; ++kb) {
-    bf16x4 a;
-    for (int j = 0; j < 4; ++j) a[j] = (short)Bf16Bits(h[col][kb * 8 + khalf + j]);
-    o = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, wf[kFragL1 + kFragL2 + kb], o, 0, 0, 0);
-  }
-  if (col < kOut)
-    for (int v = 0; v < 16; ++v) {
-      const int row = 8 * (v / 4) + khalf + (v % 4);
-      yout[row][col] = (o[v] + net.w[kOffB3 + col]) * net.norm.out_scale[col];
-    }
   __syncthreads();
 }
 
-__device__ inline void LoadFrags(const MlpDev& net, bf16x4* wf) {
-  const int l = threadIdx.x & 63;
-  for (int f = 0; f < kFrags; ++f) {
-    const uint16_t* p = net.frag + ((size_t)f * 64 + l) * 4;
-    for (int j = 0; j < 4; ++j) wf[f][j] = (short)p[j];
+// This lane's B operand of layer 1: raw features f (16) of row n, standardised, features 8h..8h+7.
+__device__ inline bf16x8 MlpInput(const MlpNorm& nm, const float f[kIn], bool ok) {
+  const int h = (threadIdx.x & 63) >> 5;
+  bf16x8 xb;
+  for (int j = 0; j < 8; ++j) {
+    const float fa = h ? f[8 + j] : f[j];
+    const float mu = h ? nm.in_mean[8 + j] : nm.in_mean[j];
+    const float is = h ? nm.in_istd[8 + j] : nm.in_istd[j];
+    xb[j] = (__bf16)(ok ? (fa - mu) * is : 0.f);
   }
+  return xb;
+}
+
+__device__ inline void HiddenFromBias(const float* bias, f32x16& acc) {
+  const int h = (threadIdx.x & 63) >> 5;
+  for (int g = 0; g < 4; ++g) {
+    const float4 b = *reinterpret_cast<const float4*>(bias + 8 * g + 4 * h);
+    acc[4 * g + 0] = b.x;
+    acc[4 * g + 1] = b.y;
+    acc[4 * g + 2] = b.z;
+    acc[4 * g + 3] = b.w;
+  }
+}
+
+__device__ inline void SiluPack(const f32x16 acc[2], bf16x8 hb[4]) {
+  for (int ks = 0; ks < 4; ++ks)
+    for (int j = 0; j < 8; ++j) hb[ks][j] = (__bf16)Silu(acc[ks >> 1][8 * (ks & 1) + j]);
+}
+
+// One batch of 32 rows (row n in lanes n and n + 32).  Returns the de-normalised outputs of row n
+// in lane n (lanes 32..63 return unspecified values).
+__device__ inline void MlpForwardRegs(const MlpWave& s, const float* bias_lds, const bf16x8 xb, float y[kOut]) {
+  f32x16 acc[2];
+  for (int mb = 0; mb < 2; ++mb) {
+    HiddenFromBias(bias_lds + 32 * mb, acc[mb]);
+    acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s.wf[mb], xb, acc[mb], 0, 0, 0);
+  }
+  bf16x8 hb[4];
+  SiluPack(acc, hb);
+  for (int mb = 0; mb < 2; ++mb) {
+    HiddenFromBias(bias_lds + kH + 32 * mb, acc[mb]);
+    for (int ks = 0; ks < 4; ++ks)
+      acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s.wf[kFragL1 + 4 * mb + ks], hb[ks], acc[mb], 0, 0, 0);
+  }
+  SiluPack(acc, hb);
+  f32x16 o;
+  for (int v = 0; v < 16; ++v) o[v] = 0.f;
+  for (int ks = 0; ks < 4; ++ks)
+    o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s.wf[kFragL1 + kFragL2 + ks], hb[ks], o, 0, 0, 0);
+  for (int k = 0; k < kOut; ++k) y[k] = (o[k] + s.b3[k]) * s.oscale[k];
 }
 
 // Batched inference (raw features in, de-normalised predictions out), one wave per 32 rows.
 __global__ __launch_bounds__(64) void MlpInferKernel(int rows, const float* __restrict__ X, MlpDev net,
                                                      float* __restrict__ Yout) {
-  __shared__ float xin[32][kIn];
-  __shared__ float h[32][kH];
-  __shared__ float yo[32][4];
-  bf16x4 wf[kFrags];
-  LoadFrags(net, wf);
-  const int l = threadIdx.x;
-  const int r0 = blockIdx.x * 32;
+  __shared__ float4 bias4[2 * kH / 4];
+  float* bias_lds = reinterpret_cast<float*>(bias4);
+  MlpWave ws;
+  MlpWaveLoad(net, bias_lds, ws);
+  const int l = threadIdx.x, n = l & 31;
+  const int r = blockIdx.x * 32 + n;
+  float f[kIn];
+  for (int k = 0; k < kIn; ++k) f[k] = r < rows ? X[(size_t)r * kIn + k] : 0.f;
   float yb[3] = {0.f, 0.f, 0.f};
-  if (l < 32) {
-    float f[kIn];
-    for (int k = 0; k < kIn; ++k) f[k] = r0 + l < rows ? X[(size_t)(r0 + l) * kIn + k] : 0.f;
-    for (int k = 0; k < kIn; ++k) xin[l][k] = (f[k] - net.norm.in_mean[k]) / net.norm.in_std[k];
-    if (net.baseline) QuadBaseline(f, yb);
-  }
-  __syncthreads();
-  MlpForward32(net, wf, xin, h, yo);
-  if (l < 32 && r0 + l < rows)
-    for (int o = 0; o < kOut; ++o) Yout[(size_t)(r0 + l) * kOut + o] = yo[l][o] + yb[o];
+  if (net.baseline) QuadBaseline(f, yb);
+  float y[kOut];
+  MlpForwardRegs(ws, bias_lds, MlpInput(net.norm, f, true), y);
+  if (l < 32 && r < rows)
+    for (int o = 0; o < kOut; ++o) Yout[(size_t)r * kOut + o] = y[o] + yb[o];
 }
 
 // ---- features, labels and the correction ------------------------------------------------------------
@@ -442,12 +458,11 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
                                                        MlpDev net, float ybound, float amax,
                                                        double* __restrict__ fx, double* __restrict__ fy,
                                                        double* __restrict__ fz, unsigned long long* __restrict__ stats) {
-  __shared__ float xin[32][kIn];
-  __shared__ float h[32][kH];
-  __shared__ float yo[32][4];
-  bf16x4 wf[kFrags];
-  LoadFrags(net, wf);
-  const int l = threadIdx.x;
+  __shared__ float4 bias4[2 * kH / 4];
+  float* bias_lds = reinterpret_cast<float*>(bias4);
+  MlpWave ws;
+  MlpWaveLoad(net, bias_lds, ws);
+  const int l = threadIdx.x, n = l & 31;
   unsigned long long n_model = 0, n_fallback = 0, n_over = 0;
   // Persistent over targets: the weight fragments stay in registers for the whole launch.
   for (int k = blockIdx.x; k < m; k += gridDim.x) {
@@ -459,23 +474,24 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
     const int nrows = 2 * cnt;
     double cx = 0.0, cy = 0.0, cz = 0.0;
     for (int base = 0; base < nrows; base += 32) {
-      const int row = base + l;
+      const int row = base + n;  // lanes n and n + 32 carry the same row
       bool ok = false;
       float R = 1.f, Qs = 0.f, hh = 0.f;
       float yb[3] = {0.f, 0.f, 0.f};
+      float f[kIn];
+      for (int a = 0; a < kIn; ++a) f[a] = 0.f;
       int c = 0;
-      if (l < 32 && row < nrows) {
+      if (row < nrows) {
         c = (int)rec[(size_t)k * rec_cap + row / 2];
-        float f[kIn];
         ok = RowFeatures(th, tl, nodes32[c], feat[c], row & 1, f, R, Qs, hh);
-        for (int a = 0; a < kIn; ++a) xin[l][a] = ok ? (f[a] - net.norm.in_mean[a]) / net.norm.in_std[a] : 0.f;
         if (ok && net.baseline) QuadBaseline(f, yb);
       }
-      __syncthreads();
-      MlpForward32(net, wf, xin, h, yo);
+      float y[kOut];
+      MlpForwardRegs(ws, bias_lds, MlpInput(net.norm, f, ok), y);
+      ok = ok && l < 32;
       float corr[3] = {0.f, 0.f, 0.f};
       if (ok) {
-        const float p0 = yo[l][0] + yb[0], p1 = yo[l][1] + yb[1], p2 = yo[l][2] + yb[2];
+        const float p0 = y[0] + yb[0], p1 = y[1] + yb[1], p2 = y[2] + yb[2];
         const float a = hh / R;
         const float mag = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
         if (mag <= ybound && a <= amax && mag == mag) {
@@ -567,7 +583,7 @@ int am_mlp_create(am_ctx* ctx, uint64_t seed, am_mlp** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&net->grad, kParams * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&net->adam_m, kParams * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&net->adam_v, kParams * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&net->frag, (size_t)kFrags * 64 * 4 * 2);
+  if (e == hipSuccess) e = hipMalloc((void**)&net->frag, (size_t)kFrags * 64 * 8 * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMalloc((void**)&net->loss, 16);
   if (e != hipSuccess) {
     am_mlp_destroy(net);
@@ -591,6 +607,7 @@ int am_mlp_create(am_ctx* ctx, uint64_t seed, am_mlp** out) {
   for (int k = 0; k < kIn; ++k) {
     net->norm.in_mean[k] = 0.f;
     net->norm.in_std[k] = 1.f;
+    net->norm.in_istd[k] = 1.f;
   }
   for (int k = 0; k < kOut; ++k) net->norm.out_scale[k] = 1.f;
   int rc;
@@ -648,6 +665,7 @@ int am_mlp_set_normalization(am_mlp* net, const float* in_mean, const float* in_
     if (!(in_std[k] > 0.f)) return Fail(AM_ERR_INVALID, "am_mlp_set_normalization: std must be > 0");
     net->norm.in_mean[k] = in_mean[k];
     net->norm.in_std[k] = in_std[k];
+    net->norm.in_istd[k] = 1.f / in_std[k];
   }
   for (int k = 0; k < kOut; ++k) net->norm.out_scale[k] = out_scale[k];
   return AM_OK;
@@ -706,7 +724,7 @@ int am_mlp_train(am_mlp* net, const float* X, const float* Y, int64_t rows, int 
   std::vector<float> xn((size_t)rows * kIn), yn((size_t)rows * kOut);
   for (int64_t r = 0; r < rows; ++r) {
     for (int k = 0; k < kIn; ++k)
-      xn[r * kIn + k] = (X[r * kIn + k] - net->norm.in_mean[k]) / net->norm.in_std[k];
+      xn[r * kIn + k] = (X[r * kIn + k] - net->norm.in_mean[k]) * net->norm.in_istd[k];  // as on the device
     float yb[3] = {0.f, 0.f, 0.f};
     if (net->baseline) QuadBaseline(X + r * kIn, yb);  // the network learns what the quadrupole leaves
     for (int k = 0; k < kOut; ++k) yn[r * kOut + k] = (Y[r * kOut + k] - yb[k]) / net->norm.out_scale[k];
