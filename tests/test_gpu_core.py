@@ -115,6 +115,30 @@ def test_allpairs_fp32_sorted_path_with_dead_and_neutral(ctx):
         s.close()
 
 
+def test_allpairs_sorted_path_non_unit_charges(ctx):
+    """Tiles holding charges other than +-e (antihelium, caller overrides) take the generic pair
+    loop; unit-charge tiles fold the sign into the difference.  Both against the oracle, with the
+    target order of the tiles partitioned by sign."""
+    rng = np.random.default_rng(41)
+    n = 9000
+    pos = rng.normal(0, 1, (n, 3)) * np.array([1e-3, 1e-3, 3e-3])
+    sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    sp[:300] = 5  # antihelium (-2e) in the first tiles of some regions
+    m, q = oracle.mass_charge(sp)
+    s = upload(ctx, pos, np.zeros_like(pos), sp)
+    s.compute_allpairs_coulomb_forces(am.FP32)
+    ref = oracle.allpairs_forces(pos, q)
+    assert rel_l2(s.forces(), ref) < 1e-5
+    q2 = q.copy()
+    q2[::17] *= 0.37  # fractional charges via the upload override
+    s.upload(pos, np.zeros_like(pos), sp, charge=q2)
+    for _ in range(2):
+        s.compute_allpairs_coulomb_forces(am.FP32)
+        ref2 = oracle.allpairs_forces(pos, q2)
+        assert rel_l2(s.forces(), ref2) < 1e-5
+    s.close()
+
+
 def test_allpairs_edge_cases(ctx):
     s = am.Store(ctx, 16)
     s.upload(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.uint8))
