@@ -9,6 +9,17 @@ Boris push, with every input resident in HBM.  Energy diagnostics and the force-
 outside the timed region.
 
 Prints one JSON line (rank 0).  --gpus N > 1 exits: this build targets one GPU.
+
+Other workloads (--workload), each on its full configuration with its own JSON line:
+  barnes-hut      N = 1,048,576 positrons + electrons (50/50), radii (2e-3, 2e-3, 8e-3) m, 4 K,
+                  B0 = 1 T, U0 = 10 V; per step Barnes-Hut theta = 0.7 (FP32 walk) + push +
+                  annihilation (5e-6 m, p = 0.5)
+  fast-multipole  the same cloud and step with FMM order 4, theta = 0.5 (FP32 near field)
+  bh-learned      the same cloud and step with the learned correction at theta = 0.9, trained
+                  first (outside the timed region) on 16 clouds of 262,144 particles
+  ensemble        512 independent traps (N log-uniform in 1,024..32,768), one hipGraph-replayed
+                  step of all-pairs forces + push with cooling + annihilation
+--copy-calibration BYTES runs a device copy of that size (the reference for HBM byte counters).
 """
 from __future__ import annotations
 
@@ -65,6 +76,18 @@ def cpu_baseline(pos, vel, sp, trap, dt, steps=4):
                        f"push, median {med:.3f} s/step, OpenMP")
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of nbytes (five launches): the known-size reference that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a profiled kernel into HBM bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}), flush=True)
+
+
 def read_traffic():
     """HBM bytes per all-pairs launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", "allpairs_f32_n65536_pmc.json")
@@ -76,19 +99,261 @@ def read_traffic():
         return None
 
 
+# ---- other workloads --------------------------------------------------------------------------
+N_TREE = 1 << 20
+TREE_CONFIG = ("N=1048576 positrons + electrons (50/50), radii (2e-3,2e-3,8e-3) m, 4 K, B0=1 T, U0=10 V, "
+               "z0=rho0=0.01 m; annihilation 5e-6 m, p=0.5")
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def tree_cloud(n=N_TREE, seed=42, temperature=4.0):
+    import antimatter_amd as am
+    pops = {"positron": n // 2, "electron": n - n // 2}
+    temps = {"positron": temperature, "electron": temperature}
+    pos, vel, sp = am.initial_conditions(pops, temps, (2e-3, 2e-3, 8e-3), seed)
+    return pos, vel, sp, am.make_trap(1.0, 10.0, 0.01, 0.01), am.derive_dt(pops, 1.0)
+
+
+def timed_steps(ctx, step, steps, warmup):
+    for k in range(warmup):
+        step(k + 1)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k + 1)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    return elapsed
+
+
+def force_errors(am, store, f):
+    """Relative force errors of f against the FP64 direct sum on the device (the all-pairs FP64
+    kernel, itself bit-identical to the oracle on the parity tests)."""
+    store.compute_allpairs_coulomb_forces(am.FP64)
+    ref = store.forces()
+    per = np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+    return float(np.linalg.norm(f - ref) / np.linalg.norm(ref)), float(np.median(per)), float(per.max())
+
+
+def kernel_avg(ctx, name):
+    ms, launches = ctx.kernel_time(name)
+    return ms / max(launches, 1), launches
+
+
+def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
+    import antimatter_amd as am
+    from antimatter_amd import mlp
+    log(f"{kind}: building the cloud")
+    pos, vel, sp, trap, dt = tree_cloud()
+    n = len(sp)
+    ctx = am.Context(0)
+    store = am.Store(ctx, n)
+    store.upload(pos, vel, sp)
+    extra = {}
+    if kind == "barnes-hut":
+        theta = 0.7
+
+        def forces():
+            store.compute_barnes_hut_coulomb_forces(theta, am.FP32)
+        kernel, label = "bh_walk_f32", f"Barnes-Hut theta={theta}, FP32 walk"
+    elif kind == "fast-multipole":
+        def forces():
+            store.compute_fast_multipole_coulomb_forces(4, 0.5, am.FP32)
+        kernel, label = "fmm_m2l", "FMM order 4, theta=0.5, FP32 near field"
+    else:
+        theta = 0.9
+        net = mlp.Mlp(ctx, seed=3)
+        rng = np.random.default_rng(7)
+        t0 = time.perf_counter()
+        clouds = []
+        for k in range(16):
+            temp = float(np.exp(rng.uniform(np.log(4.0), np.log(1e4))))
+            clouds.append(tree_cloud(262144, 100 + k, temp)[:3])
+        log("bh-learned: generating training rows and training")
+        rows, losses = mlp.train_on_clouds(ctx, clouds, theta, net, targets_per_cloud=512, rec_cap=2048, epochs=20,
+                                           batch=1024, lr=2e-3, seed=1)
+        extra["training"] = {"clouds": 16, "cloud_n": 262144, "rows": rows, "epochs": 20,
+                             "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                             "seconds": time.perf_counter() - t0}
+        stats = {}
+
+        def forces():
+            stats.update(mlp.barnes_hut_learned_forces(store, theta, net, rec_cap=1024, ybound=1e30, amax=1e30))
+        kernel, label = "bh_learned_correct", f"Barnes-Hut theta={theta} + learned correction (MLP 16-64-64-3)"
+
+    def step(k):
+        forces()
+        store.push(trap, dt, 0.0, 0.0, 42, k)
+        store.annihilation(5e-6, 0.5, 42, k)
+
+    log(f"{kind}: timing {steps} steps")
+    elapsed = timed_steps(ctx, step, steps, warmup)
+    avg_ms, _ = kernel_avg(ctx, kernel)
+    parts = {name: kernel_avg(ctx, name)[0] for name in
+             ("bh_walk_f32", "bh_learned_correct", "fmm_upward", "fmm_traverse", "fmm_m2l", "fmm_downward",
+              "fmm_l2p_p2p", "push", "annihilation") if ctx.kernel_time(name)[1]}
+    forces()
+    f = store.forces()
+    log(f"{kind}: FP64 direct reference")
+    err, med, mx = force_errors(am, store, f)
+    out = {
+        "metric": f"particle-steps/s, {label}, force error vs FP64 direct",
+        "value": n * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (reference initial-condition generators, seed 42)",
+        "config": {"workload": f"{label}; push + annihilation per step; {TREE_CONFIG}", "n_particles": n},
+        "force_rel_err_l2": err, "force_rel_err_median": med, "force_rel_err_max": mx,
+        "kernel_ms": parts,
+    }
+    if kind == "barnes-hut":
+        counts = store.barnes_hut_forces_counted(theta, am.FP32)
+        inter = float(counts.sum())
+        out["interactions_per_step"] = inter
+        flops = FLOP_PER_PAIR * inter
+        out["roofline"] = {"bound": "valu", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": flops / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                           "traffic": None,
+                           "note": "20 flop per interaction (leaf pair or accepted monopole) over the walk kernel"}
+    elif kind == "fast-multipole":
+        st = store.fmm_stats()
+        flops = 420.0 * st["m2l_pairs"]  # 210 multiply-adds of the order-4 M2L contraction per cell pair
+        out["fmm_stats"] = st
+        out["roofline"] = {"bound": "valu", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": flops / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                           "traffic": None,
+                           "note": "M2L kernel, FP64; 420 flop (the contraction only) per cell pair"}
+    else:
+        rows = stats["model_rows"] + stats["fallback_rows"]
+        flops = 2.0 * (16 * 64 + 64 * 64 + 64 * 3) * rows
+        out["learned"] = dict(stats, **extra)
+        out["roofline"] = {"bound": "mfma", "achieved": flops / (avg_ms * 1e-3) / 1e12,
+                           "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": flops / (avg_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                           "note": "correction kernel; 10624 flop (the network's multiply-adds) per (target, node, sign) row"}
+    if with_cpu and kind != "bh-learned":
+        log(f"{kind}: CPU baseline")
+        out["cpu_baseline"] = tree_cpu_baseline(kind, pos, sp, cpu_steps)
+    store.close()
+    ctx.close()
+    return out
+
+
+def tree_cpu_baseline(kind, pos, sp, cpu_steps):
+    """The oracle's OpenMP FP64 solver on a bounded sample: one force evaluation over the first
+    262,144 particles of the cloud (the solvers' cost is near-linear in N)."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    oracle.set_num_threads(threads)
+    k = 262144
+    m, q = oracle.mass_charge(sp[:k])
+    t0 = time.perf_counter()
+    if kind == "barnes-hut":
+        oracle.bh_forces(pos[:k], q, None, 0.7)
+    else:
+        oracle.fmm_forces(pos[:k], q, None, 4, 0.5)
+    t = time.perf_counter() - t0
+    return dict(value=k / t, unit="particle-steps/s (force evaluation only)", cores=oracle.max_threads(),
+                kind="port", sample=f"one force evaluation of the first {k} particles of the same cloud, {t:.2f} s")
+
+
+def ensemble_workload(steps, warmup, with_cpu, cpu_steps):
+    import antimatter_amd as am
+    pos, vel, sp, traps, params = am.make_config5()
+    n = len(sp)
+    ctx = am.Context(0)
+    store = am.Store(ctx, n)
+    store.upload(pos, vel, sp)
+    ens = am.Ensemble(store, traps)
+    chosen, times = ens.autotune()
+    ens.run(warmup, seed=42, use_graph=True)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    ens.run(steps, seed=42, use_graph=True)  # one captured step, replayed `steps` times
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    st = ens.stats()
+    avg_ms = ctx.kernel_time("ensemble_steps")[0] / steps  # the timer brackets the whole replay
+    pairs = st["pair_interactions_per_step"]
+    flops = FLOP_PER_PAIR * pairs
+    out = {
+        "metric": "particle-steps/s, batched ensemble of 512 traps (one hipGraph-replayed step)",
+        "value": n * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (reference initial-condition generators, seed 42 + trap)",
+        "config": {"workload": "512 traps, N log-uniform 1024..32768, 4 species Dirichlet mix, T log-uniform "
+                               "4..1e4 K, B0 1..5 T, U0 5..100 V, radii 1e-4..2e-3 m; all-pairs FP32 + FP64 Boris + "
+                               "cooling (bath 4 K) + annihilation (5e-6 m, p=1)",
+                   "n_particles": n, "n_traps": len(traps)},
+        "autotune": {"chosen_chunk": chosen, "times_ms": [float(t) for t in times]},
+        "pair_interactions_per_s": pairs * steps / elapsed,
+        "annihilated": st["annihilated"],
+        "roofline": {"bound": "valu", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "traffic": None,
+                     "note": "whole step (forces, push, contacts) timed by HIP events; 20 flop per pair"},
+    }
+    if with_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        oracle.set_num_threads(threads)
+        t0 = time.perf_counter()
+        done = 0
+        for tr in traps[:64]:
+            b, e = tr.begin, tr.end
+            m, q = oracle.mass_charge(sp[b:e])
+            oracle.allpairs_forces(pos[b:e], q)
+            done += e - b
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=done / t, unit="particle-steps/s (forces only)", cores=oracle.max_threads(),
+                                   kind="port", sample=f"FP64 all-pairs forces of the first 64 traps ({done} particles), "
+                                                       f"{t:.2f} s")
+    ens.close()
+    store.close()
+    ctx.close()
+    return out
+
+
+OTHER_WORKLOADS = {
+    "barnes-hut": lambda s, w, c, cs: tree_workload("barnes-hut", s, w, c, cs),
+    "fast-multipole": lambda s, w, c, cs: tree_workload("fast-multipole", s, w, c, cs),
+    "bh-learned": lambda s, w, c, cs: tree_workload("bh-learned", s, w, c, cs),
+    "ensemble": ensemble_workload,
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--workload", default="all-pairs",
+                    choices=["all-pairs", "barnes-hut", "fast-multipole", "bh-learned", "ensemble"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the all-pairs path would shard targets across GPUs with an all-gather of positions).",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.workload != "all-pairs":
+        out = OTHER_WORKLOADS[args.workload](args.steps, args.warmup, not args.no_cpu_baseline, args.cpu_steps)
+        print(json.dumps(out), flush=True)
+        return
 
     import antimatter_amd as am
     pos, vel, sp, trap, dt = workload()
