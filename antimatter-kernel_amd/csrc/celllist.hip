@@ -6,17 +6,19 @@
 //
 // A cell list sorts particles by the 63-bit key (cx, cy, cz) of their cell of edge h (slightly
 // larger than the interaction radius, so every pair within the radius lies in the 27-cell
-// stencil) with the stable radix sort; a query binary-searches each stencil cell's range.
+// stencil) with the stable radix sort; a query looks each stencil cell's run up in a cell table
+// (open addressing, built once per call).
 // Cell coordinates are clamped to +-2^20 cells (clamping is 1-Lipschitz, so pairs within the
 // radius stay within the stencil; only far-flung particles share boundary cells).
 #include <algorithm>
 #include <cmath>
 
 #include "am_internal.hpp"
+#include "celltable.hpp"
 
 namespace am {
 
-constexpr uint64_t kDeadKey = ~0ull;  // annihilated particles sort last and match no cell
+constexpr uint64_t kDeadKey = kCellEmpty;  // annihilated particles sort last and match no cell
 
 __device__ inline int64_t CellCoord(double v, double inv_h) {
   const double c = floor(v * inv_h);
@@ -42,20 +44,47 @@ __global__ void CellKeyKernel(int64_t n, const double* __restrict__ x, const dou
   keys[i] = CellKey(CellCoord(x[i], inv_h), CellCoord(y[i], inv_h), CellCoord(z[i], inv_h));
 }
 
-__device__ inline int64_t LowerBound(const uint64_t* __restrict__ keys, int64_t n, uint64_t k) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (keys[mid] < k) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+__global__ void CellTableClearKernel(uint64_t size, uint64_t* __restrict__ hkeys) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size) hkeys[h] = kCellEmpty;
 }
 
-// Calls f(j) for every particle j in the 27 cells around particle i's cell (including i).
+// One thread per sorted position; the first position of each run inserts the run.
+__global__ void CellTableInsertKernel(int64_t n, const uint64_t* __restrict__ skeys, uint64_t* __restrict__ hkeys,
+                                      uint2* __restrict__ hrange, uint64_t mask) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t key = skeys[k];
+  if (key == kCellEmpty || (k > 0 && skeys[k - 1] == key)) return;
+  int64_t e = k + 1;
+  while (e < n && skeys[e] == key) ++e;
+  uint64_t h = CellHashMix(key) & mask;
+  for (;;) {  // a free slot exists: at most n of the >= 2n slots are ever taken
+    const unsigned long long prev =
+        atomicCAS(reinterpret_cast<unsigned long long*>(hkeys + h), (unsigned long long)kCellEmpty,
+                  (unsigned long long)key);
+    if (prev == kCellEmpty) {
+      hrange[h] = make_uint2((uint32_t)k, (uint32_t)(e - k));
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+int LaunchCellTable(hipStream_t st, const uint64_t* skeys, int64_t n, uint64_t* hkeys, uint2* hrange, uint64_t size) {
+  hipLaunchKernelGGL(CellTableClearKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size, hkeys);
+  if (n > 0)
+    hipLaunchKernelGGL(CellTableInsertKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, skeys, hkeys, hrange,
+                       size - 1);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
+}
+
+// Calls f(j) for every particle j in the 27 cells around particle i's cell (including i), cells
+// in (dz, dy, dx) order and each cell's particles in sorted order.
 template <typename F>
-__device__ inline void ForEachStencil(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx, int64_t n,
-                                      double px, double py, double pz, double inv_h, F&& f) {
+__device__ inline void ForEachStencil(const CellTable& tab, const uint32_t* __restrict__ idx, double px, double py,
+                                      double pz, double inv_h, F&& f) {
   const int64_t cx = CellCoord(px, inv_h), cy = CellCoord(py, inv_h), cz = CellCoord(pz, inv_h);
   const int64_t lim = (int64_t)1 << 21;
   for (int dz = -1; dz <= 1; ++dz) {
@@ -67,8 +96,8 @@ __device__ inline void ForEachStencil(const uint64_t* __restrict__ keys, const u
       for (int dx = -1; dx <= 1; ++dx) {
         const int64_t x = cx + dx;
         if (x < 0 || x >= lim) continue;
-        const uint64_t key = CellKey(x, y, z);
-        for (int64_t k = LowerBound(keys, n, key); k < n && keys[k] == key; ++k) f((int64_t)idx[k]);
+        const uint2 r = CellLookup(tab, CellKey(x, y, z));
+        for (uint32_t k = r.x; k < r.x + r.y; ++k) f((int64_t)idx[k]);
       }
     }
   }
@@ -78,6 +107,7 @@ struct CellList {
   uint64_t* keys = nullptr;
   uint32_t* idx = nullptr;
   double inv_h = 0.0;
+  CellTable tab;
 };
 
 static int BuildCellList(am_store* s, double radius, const char* tag, CellList* cl) {
@@ -94,15 +124,24 @@ static int BuildCellList(am_store* s, double radius, const char* tag, CellList* 
   hipLaunchKernelGGL(CellKeyKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z, s->flags,
                      cl->inv_h, cl->keys, cl->idx);
   AM_HIP(hipGetLastError());
-  return RadixSortPairs(ctx, cl->keys, cl->idx, n, 64);
+  if ((rc = RadixSortPairs(ctx, cl->keys, cl->idx, n, 64))) return rc;
+  const uint64_t size = CellTableSize(n);
+  void *hk, *hr;
+  std::string hkn = std::string(tag) + "_hkeys", hrn = std::string(tag) + "_hrange";
+  if ((rc = Scratch(ctx, hkn.c_str(), (size_t)size * 8, &hk))) return rc;
+  if ((rc = Scratch(ctx, hrn.c_str(), (size_t)size * sizeof(uint2), &hr))) return rc;
+  if ((rc = LaunchCellTable(ctx->stream, cl->keys, n, (uint64_t*)hk, (uint2*)hr, size))) return rc;
+  cl->tab.keys = (const uint64_t*)hk;
+  cl->tab.range = (const uint2*)hr;
+  cl->tab.mask = size - 1;
+  return AM_OK;
 }
 
 // coulomb_interaction.cc:60-73 (pair term r_ij * (k q_i q_j / d2 / distance)).
 __global__ __launch_bounds__(256) void StandardForceKernel(int64_t n, const double* __restrict__ x,
                                                            const double* __restrict__ y, const double* __restrict__ z,
                                                            const double* __restrict__ q,
-                                                           const uint8_t* __restrict__ flags,
-                                                           const uint64_t* __restrict__ keys,
+                                                           const uint8_t* __restrict__ flags, CellTable tab,
                                                            const uint32_t* __restrict__ idx, double inv_h, double rc2,
                                                            double* __restrict__ fx, double* __restrict__ fy,
                                                            double* __restrict__ fz) {
@@ -113,7 +152,7 @@ __global__ __launch_bounds__(256) void StandardForceKernel(int64_t n, const doub
   const double px = x[i], py = y[i], pz = z[i];
   const bool live = !(flags[i] & kFlagAnnihilated) && qi != 0.0 && isfinite(px) && isfinite(py) && isfinite(pz);
   if (live) {
-    ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+    ForEachStencil(tab, idx, px, py, pz, inv_h, [&](int64_t j) {
       if (j == i) return;
       const double qj = q[j];
       if (qj == 0.0) return;
@@ -142,8 +181,7 @@ __global__ __launch_bounds__(256) void StandardEnergyKernel(int64_t n, const dou
                                                             const double* __restrict__ y,
                                                             const double* __restrict__ z,
                                                             const double* __restrict__ q,
-                                                            const uint8_t* __restrict__ flags,
-                                                            const uint64_t* __restrict__ keys,
+                                                            const uint8_t* __restrict__ flags, CellTable tab,
                                                             const uint32_t* __restrict__ idx, double inv_h, double rc,
                                                             double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,7 +191,7 @@ __global__ __launch_bounds__(256) void StandardEnergyKernel(int64_t n, const dou
   if (!(flags[i] & kFlagAnnihilated) && qi != 0.0) {
     const double px = x[i], py = y[i], pz = z[i];
     const double rc2 = rc * rc;
-    ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+    ForEachStencil(tab, idx, px, py, pz, inv_h, [&](int64_t j) {
       if (j == i) return;
       const double qj = q[j];
       if (qj == 0.0) return;
@@ -181,7 +219,7 @@ int StandardEnergyTerms(am_store* s, double rc, double* out) {
   int rc_ = BuildCellList(s, rc, "cl_energy", &cl);
   if (rc_) return rc_;
   hipLaunchKernelGGL(StandardEnergyKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, s->ctx->stream, s->n, s->x, s->y,
-                     s->z, s->q, s->flags, cl.keys, cl.idx, cl.inv_h, rc, out);
+                     s->z, s->q, s->flags, cl.tab, cl.idx, cl.inv_h, rc, out);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -192,11 +230,11 @@ template <typename F>
 __device__ inline void ForEachCandidate(int64_t i, int64_t n, const double* __restrict__ x,
                                         const double* __restrict__ y, const double* __restrict__ z,
                                         const uint8_t* __restrict__ species, const uint8_t* __restrict__ flags,
-                                        const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
-                                        double inv_h, double ra2, F&& f) {
+                                        const CellTable& tab, const uint32_t* __restrict__ idx, double inv_h,
+                                        double ra2, F&& f) {
   const double px = x[i], py = y[i], pz = z[i];
   const int si = species[i];
-  ForEachStencil(keys, idx, n, px, py, pz, inv_h, [&](int64_t j) {
+  ForEachStencil(tab, idx, px, py, pz, inv_h, [&](int64_t j) {
     if (j == i || (flags[j] & kFlagAnnihilated)) return;
     if (!IsPair(si, species[j])) return;
     const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
@@ -207,14 +245,14 @@ __device__ inline void ForEachCandidate(int64_t i, int64_t n, const double* __re
 
 __global__ void ContactCountKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                    const double* __restrict__ z, const uint8_t* __restrict__ species,
-                                   const uint8_t* __restrict__ flags, const uint64_t* __restrict__ keys,
+                                   const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    uint32_t* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c = 0;
   if (!(flags[i] & kFlagAnnihilated))
-    ForEachCandidate(i, n, x, y, z, species, flags, keys, idx, inv_h, ra2, [&](int64_t) { ++c; });
+    ForEachCandidate(i, n, x, y, z, species, flags, tab, idx, inv_h, ra2, [&](int64_t) { ++c; });
   count[i] = c;
 }
 
@@ -222,7 +260,7 @@ __global__ void ContactCountKernel(int64_t n, const double* __restrict__ x, cons
 // short segment), as (i, j) pairs.
 __global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                    const double* __restrict__ z, const uint8_t* __restrict__ species,
-                                   const uint8_t* __restrict__ flags, const uint64_t* __restrict__ keys,
+                                   const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    const uint32_t* __restrict__ off, uint2* __restrict__ pairs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,7 +268,7 @@ __global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, cons
   const uint32_t b = off[i], e = off[i + 1];
   if (b == e) return;
   uint32_t w = b;
-  ForEachCandidate(i, n, x, y, z, species, flags, keys, idx, inv_h, ra2, [&](int64_t j) {
+  ForEachCandidate(i, n, x, y, z, species, flags, tab, idx, inv_h, ra2, [&](int64_t j) {
     uint32_t k = w++;
     while (k > b && pairs[k - 1].y > (uint32_t)j) {
       pairs[k] = pairs[k - 1];
@@ -335,7 +373,7 @@ int am_compute_standard_coulomb_forces(am_store* s, double cutoff_radius, int pr
   int slot;
   TimerBegin(ctx, "standard", &slot);
   hipLaunchKernelGGL(StandardForceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, s->x, s->y, s->z,
-                     s->q, s->flags, cl.keys, cl.idx, cl.inv_h, cutoff_radius * cutoff_radius, s->fx, s->fy, s->fz);
+                     s->q, s->flags, cl.tab, cl.idx, cl.inv_h, cutoff_radius * cutoff_radius, s->fx, s->fy, s->fz);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;
@@ -360,7 +398,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
   int slot;
   TimerBegin(ctx, "annihilation", &slot);
   hipLaunchKernelGGL(ContactCountKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
-                     s->species, s->flags, cl.keys, cl.idx, cl.inv_h, ra2, (uint32_t*)cnt);
+                     s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (uint32_t*)cnt);
   AM_HIP(hipGetLastError());
   if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)off, n))) return rc;
   uint32_t total = 0;
@@ -373,7 +411,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
     if ((rc = Scratch(ctx, "contact_gpass", (size_t)total, &gpass))) return rc;
     if ((rc = Scratch(ctx, "contact_gmark", (size_t)total, &gmark))) return rc;
     hipLaunchKernelGGL(ContactWriteKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
-                       s->species, s->flags, cl.keys, cl.idx, cl.inv_h, ra2, (const uint32_t*)off, (uint2*)pairs);
+                       s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (const uint32_t*)off, (uint2*)pairs);
     AM_HIP(hipGetLastError());
     hipLaunchKernelGGL(ContactResolveKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
                        (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked,

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <vector>
 
+#include "celltable.hpp"
 #include "physics.hpp"
 
 namespace am {
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void EnsemblePushKernel(int64_t n, double* __r
 
 // ---- contact detection within each trap ---------------------------------------------------------
 constexpr int kECellBits = 18;
-constexpr uint64_t kEDeadKey = ~0ull;
+constexpr uint64_t kEDeadKey = kCellEmpty;
 
 __device__ inline int64_t ECoord(double v, double inv_h) {
   const double c = floor(v * inv_h);
@@ -181,7 +182,7 @@ __device__ inline void EnsForEachCandidate(int64_t i, int64_t n, const double* _
                                            const double* __restrict__ y, const double* __restrict__ z,
                                            const uint8_t* __restrict__ species, const uint8_t* __restrict__ flags,
                                            const uint32_t* __restrict__ trap, const TrapRec* __restrict__ traps,
-                                           const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx, F&& f) {
+                                           const CellTable& tab, const uint32_t* __restrict__ idx, F&& f) {
   const uint32_t t = trap[i];
   const TrapRec& tr = traps[t];
   const double px = x[i], py = y[i], pz = z[i];
@@ -194,14 +195,8 @@ __device__ inline void EnsForEachCandidate(int64_t i, int64_t n, const double* _
       for (int dx = -1; dx <= 1; ++dx) {
         const int64_t ax = cx + dx, ay = cy + dy, az = cz + dz;
         if (ax < 0 || ay < 0 || az < 0 || ax >= lim || ay >= lim || az >= lim) continue;
-        const uint64_t key = EKey(t, ax, ay, az);
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (keys[mid] < key) lo = mid + 1;
-          else hi = mid;
-        }
-        for (int64_t k = lo; k < n && keys[k] == key; ++k) {
+        const uint2 r = CellLookup(tab, EKey(t, ax, ay, az));
+        for (uint32_t k = r.x; k < r.x + r.y; ++k) {
           const int64_t j = idx[k];
           if (j == i || (flags[j] & kFlagAnnihilated)) continue;
           if (!IsPair(si, species[j])) continue;
@@ -215,21 +210,21 @@ __device__ inline void EnsForEachCandidate(int64_t i, int64_t n, const double* _
 __global__ void EnsContactCountKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                       const double* __restrict__ z, const uint8_t* __restrict__ species,
                                       const uint8_t* __restrict__ flags, const uint32_t* __restrict__ trap,
-                                      const TrapRec* __restrict__ traps, const uint64_t* __restrict__ keys,
+                                      const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, uint32_t* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t c = 0;
   const uint32_t t = trap[i];
   if (t < (uint32_t)kMaxTraps && !(flags[i] & kFlagAnnihilated) && traps[t].annihilation)
-    EnsForEachCandidate(i, n, x, y, z, species, flags, trap, traps, keys, idx, [&](int64_t) { ++c; });
+    EnsForEachCandidate(i, n, x, y, z, species, flags, trap, traps, tab, idx, [&](int64_t) { ++c; });
   count[i] = c;
 }
 
 __global__ void EnsContactWriteKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                       const double* __restrict__ z, const uint8_t* __restrict__ species,
                                       const uint8_t* __restrict__ flags, const uint32_t* __restrict__ trap,
-                                      const TrapRec* __restrict__ traps, const uint64_t* __restrict__ keys,
+                                      const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
                                       int64_t cap, uint2* __restrict__ pairs, uint32_t* __restrict__ overflow) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,7 +236,7 @@ __global__ void EnsContactWriteKernel(int64_t n, const double* __restrict__ x, c
     return;
   }
   uint32_t w = b;
-  EnsForEachCandidate(i, n, x, y, z, species, flags, trap, traps, keys, idx, [&](int64_t j) {
+  EnsForEachCandidate(i, n, x, y, z, species, flags, trap, traps, tab, idx, [&](int64_t j) {
     uint32_t k = w++;
     while (k > b && pairs[k - 1].y > (uint32_t)j) {
       pairs[k] = pairs[k - 1];
@@ -346,6 +341,9 @@ struct am_ensemble {
   // contact buffers
   uint64_t* ckeys = nullptr;
   uint32_t* cidx = nullptr;
+  uint64_t* hkeys = nullptr;  // cell table of the sorted contact keys
+  uint2* hrange = nullptr;
+  uint64_t hsize = 0;
   uint32_t* ccount = nullptr;
   uint32_t* coff = nullptr;
   uint2* cpairs = nullptr;
@@ -441,14 +439,19 @@ static int EnsEnqueueStep(am_ensemble* e, uint64_t seed) {
                        (const uint32_t*)s->trap, (const TrapRec*)e->d_traps, e->ckeys, e->cidx);
     AM_HIP(hipGetLastError());
     if ((rc = RadixSortPairs(ctx, e->ckeys, e->cidx, n, 64))) return rc;
+    if ((rc = LaunchCellTable(st, e->ckeys, n, e->hkeys, e->hrange, e->hsize))) return rc;
+    CellTable tab;
+    tab.keys = e->hkeys;
+    tab.range = e->hrange;
+    tab.mask = e->hsize - 1;
     hipLaunchKernelGGL(EnsContactCountKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->x, s->y, s->z,
-                       s->species, s->flags, (const uint32_t*)s->trap, (const TrapRec*)e->d_traps,
-                       (const uint64_t*)e->ckeys, (const uint32_t*)e->cidx, e->ccount);
+                       s->species, s->flags, (const uint32_t*)s->trap, (const TrapRec*)e->d_traps, tab,
+                       (const uint32_t*)e->cidx, e->ccount);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, e->ccount, e->coff, n))) return rc;
     hipLaunchKernelGGL(EnsContactWriteKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->x, s->y, s->z,
-                       s->species, s->flags, (const uint32_t*)s->trap, (const TrapRec*)e->d_traps,
-                       (const uint64_t*)e->ckeys, (const uint32_t*)e->cidx, (const uint32_t*)e->coff,
+                       s->species, s->flags, (const uint32_t*)s->trap, (const TrapRec*)e->d_traps, tab,
+                       (const uint32_t*)e->cidx, (const uint32_t*)e->coff,
                        e->contact_cap, e->cpairs, e->overflow);
     hipLaunchKernelGGL(EnsResolveKernel, dim3(e->n_traps), dim3(256), 0, st, (const TrapRec*)e->d_traps,
                        (const uint32_t*)e->coff, (const uint2*)e->cpairs, e->contact_cap, seed,
@@ -531,6 +534,11 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
     e->ckeys = (uint64_t*)p;
     if ((rc = EnsAlloc(e, (size_t)n * 4, &p))) return fail(rc);
     e->cidx = (uint32_t*)p;
+    e->hsize = CellTableSize(n);
+    if ((rc = EnsAlloc(e, (size_t)e->hsize * 8, &p))) return fail(rc);
+    e->hkeys = (uint64_t*)p;
+    if ((rc = EnsAlloc(e, (size_t)e->hsize * sizeof(uint2), &p))) return fail(rc);
+    e->hrange = (uint2*)p;
     if ((rc = EnsAlloc(e, (size_t)n * 4, &p))) return fail(rc);
     e->ccount = (uint32_t*)p;
     if ((rc = EnsAlloc(e, (size_t)(n + 1) * 4, &p))) return fail(rc);
