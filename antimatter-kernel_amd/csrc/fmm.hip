@@ -23,6 +23,7 @@
 #include <memory>
 #include <vector>
 
+#include "celltable.hpp"
 #include "octree.hpp"
 
 namespace am {
@@ -228,6 +229,7 @@ __global__ void SortedSrcKernel(int m, const uint32_t* __restrict__ order, const
 
 struct Cell {  // preorder cell record
   double cx, cy, cz, radius;
+  double half;  // cube half width (the root's times 2^-depth, exactly)
   int b, e, leaf, nchild;
   int child[8];
 };
@@ -266,6 +268,7 @@ __global__ void GatherCellsKernel(int nn, NodeArrays nd, const uint32_t* __restr
   c.cy = nd.cy[node];
   c.cz = nd.cz[node];
   c.radius = rad[node];
+  c.half = nd.half[node];
   c.b = nd.b[node];
   c.e = nd.e[node];
   c.leaf = nd.leaf[node];
@@ -407,101 +410,282 @@ __global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int
   off[t] = lo;
 }
 
-// M2L (fast-multipole.cc:528-555): one workgroup of kM2LWaves waves per target cell; wave w takes
-// sources w, w + kM2LWaves, ... of the cell's list.  The index tables live in LDS (orders up to
-// kM2LLdsOrder; larger orders read them from global memory).  Per source the wave builds the
-// derivative tensor D of 1/r degree by degree in its LDS slice (the recurrence of :207-232), then
-// lane gamma contracts sum_alpha (-1)^|alpha| M_alpha D_{alpha+gamma}.  The waves' partial sums
-// are combined in wave order at the end, so the result is deterministic.
-constexpr int kM2LWaves = 4;
-constexpr int kM2LLdsOrder = 5;  // keeps the workgroup under 64 KB of LDS
+// ---- M2L (fast-multipole.cc:528-555) by displacement class ---------------------------------------
+// Cell centres sit on the octree lattice, so the displacement between the two cells of an M2L pair
+// is an integer multiple q of the finer cell's half width h_f = root_half * 2^-d.  The key (d, q)
+// fixes the displacement exactly, and the derivative tensor of 1/r depends on nothing else.  Pairs
+// are classed by key in an open-addressing table, each class's tensor is built once, and the
+// per-pair work is the contraction sum_alpha (-1)^|alpha| M_alpha D_{alpha+gamma}.  A pair whose key
+// does not fit (|q| >= 2^18) or that finds no slot within the probe limit builds its own tensor.
+constexpr int kQBits = 19;
+constexpr int64_t kQOff = (int64_t)1 << (kQBits - 1);
+constexpr uint64_t kQMask = ((uint64_t)1 << kQBits) - 1;
+constexpr uint64_t kClassEmpty = ~0ull;
+constexpr uint32_t kNoClass = 0xffffffffu;
+constexpr int kClassProbes = 64;
+constexpr int kM2LWaves = 4;     // waves per workgroup, one target cell each
+constexpr int kM2LLdsOrder = 5;  // orders whose index tables fit in LDS beside the wave slices
 
+__device__ inline void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Key (d: 5 bits, q: 3 x 19 bits) of the displacement T - S; false if it does not fit.
+__device__ inline bool M2LClassKey(const Cell& T, const Cell& S, int root_exp, uint64_t* key) {
+  const double hf = fmin(T.half, S.half);
+  const int d = root_exp - ilogb(hf);
+  if (d < 0 || d > 31) return false;
+  const double qd[3] = {(T.cx - S.cx) / hf, (T.cy - S.cy) / hf, (T.cz - S.cz) / hf};
+  uint64_t k = (uint64_t)d;
+  for (int a = 0; a < 3; ++a) {
+    const double r = rint(qd[a]);
+    if (!(fabs(r) < (double)kQOff) || fabs(qd[a] - r) > 1e-6) return false;
+    k = (k << kQBits) | (uint64_t)((int64_t)r + kQOff);
+  }
+  *key = k;
+  return true;
+}
+
+// The displacement of a key: q * h_f, exact.
+__device__ inline void M2LClassDisplacement(uint64_t key, double root_half, double xs[3]) {
+  const int d = (int)(key >> (3 * kQBits));
+  const double hf = ldexp(root_half, -d);
+  xs[0] = (double)((int64_t)((key >> (2 * kQBits)) & kQMask) - kQOff) * hf;
+  xs[1] = (double)((int64_t)((key >> kQBits) & kQMask) - kQOff) * hf;
+  xs[2] = (double)((int64_t)(key & kQMask) - kQOff) * hf;
+}
+
+__global__ void ClassClearKernel(uint64_t size, uint64_t* __restrict__ hkeys) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size) hkeys[h] = kClassEmpty;
+}
+
+// Inserts every pair's key; pair_slot[q] = its table slot, or kNoClass.
+__global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
+                                     uint64_t* __restrict__ hkeys, uint64_t mask, uint32_t* __restrict__ pair_slot) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const uint64_t pk = pairs[q];
+  const int root_exp = ilogb(cells[0].half);
+  uint64_t key;
+  uint32_t slot = kNoClass;
+  if (M2LClassKey(cells[pk >> 32], cells[(uint32_t)pk], root_exp, &key)) {
+    uint64_t h = CellHashMix(key) & mask;
+    for (int probe = 0; probe < kClassProbes; ++probe) {
+      const uint64_t cur = hkeys[h];
+      if (cur == key) {
+        slot = (uint32_t)h;
+        break;
+      }
+      if (cur == kClassEmpty) {
+        const unsigned long long prev =
+            atomicCAS(reinterpret_cast<unsigned long long*>(hkeys + h), (unsigned long long)kClassEmpty,
+                      (unsigned long long)key);
+        if (prev == kClassEmpty || prev == key) {
+          slot = (uint32_t)h;
+          break;
+        }
+      }
+      h = (h + 1) & mask;
+    }
+  }
+  pair_slot[q] = slot;
+}
+
+__global__ void ClassOccupiedKernel(uint64_t size, const uint64_t* __restrict__ hkeys, uint32_t* __restrict__ occ) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size) occ[h] = hkeys[h] != kClassEmpty ? 1u : 0u;
+}
+
+// ids[h] (exclusive scan of occupancy) numbers the classes in slot order; cls_key[id] = key.
+__global__ void ClassListKernel(uint64_t size, const uint64_t* __restrict__ hkeys, const uint32_t* __restrict__ ids,
+                                uint64_t* __restrict__ cls_key) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size && hkeys[h] != kClassEmpty) cls_key[ids[h]] = hkeys[h];
+}
+
+// Class id of every pair (kNoClass for unclassed pairs).
+__global__ void PairClassKernel(int64_t np, const uint32_t* __restrict__ pair_slot, const uint32_t* __restrict__ ids,
+                                uint32_t* __restrict__ pair_cls) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const uint32_t slot = pair_slot[q];
+  pair_cls[q] = slot == kNoClass ? kNoClass : ids[slot];
+}
+
+// Derivative tensor D_k (k < nd) of 1/r at xs into the wave's LDS slice, degree by degree (the
+// recurrence of fast-multipole.cc:207-232).
+__device__ inline void DerivTensorWave(const double xs[3], const FmmTab& t, const int* __restrict__ rec,
+                                       double* __restrict__ Dw) {
+  const int lane = threadIdx.x & 63;
+  const double inv_r2 = 1.0 / (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
+  if (lane == 0) Dw[0] = sqrt(inv_r2);
+  for (int g = 1; g <= 2 * t.P; ++g) {
+    WaveSync();
+    for (int k = t.deg_start[g] + lane; k < t.deg_start[g + 1]; k += 64) {
+      const int* rr = rec + 10 * k;
+      const int i = rr[0];
+      const double xi = i == 0 ? xs[0] : (i == 1 ? xs[1] : xs[2]);
+      double sum = (2 * rr[1 + i] - 1) * xi * Dw[rr[4 + i]];
+      if (rr[7 + i] >= 0) sum += (rr[1 + i] - 1) * (rr[1 + i] - 1) * Dw[rr[7 + i]];
+      for (int j = 0; j < 3; ++j) {
+        if (j == i || rr[1 + j] == 0) continue;
+        sum += 2.0 * rr[1 + j] * xs[j] * Dw[rr[4 + j]];
+        if (rr[7 + j] >= 0) sum += rr[1 + j] * (rr[1 + j] - 1) * Dw[rr[7 + j]];
+      }
+      Dw[k] = -sum * inv_r2;
+    }
+  }
+  WaveSync();
+}
+
+// One wave per class: the tensor of the class's displacement into Dc[class][nd].
+__global__ __launch_bounds__(64) void ClassTensorKernel(int n_cls, const uint64_t* __restrict__ cls_key,
+                                                        const Cell* __restrict__ cells, FmmTab t,
+                                                        double* __restrict__ Dc) {
+  extern __shared__ double lds[];
+  const int c = blockIdx.x;
+  if (c >= n_cls) return;
+  double xs[3];
+  M2LClassDisplacement(cls_key[c], cells[0].half, xs);
+  DerivTensorWave(xs, t, t.rec, lds);
+  for (int k = threadIdx.x; k < t.nd; k += 64) Dc[(size_t)c * t.nd + k] = lds[k];
+}
+
+// One wave per target cell (kM2LWaves per workgroup), sources of its list in order.  Per pair the
+// class tensor (or, unclassed, the pair's own) and the multipole are staged in the wave's LDS slice
+// and lane gamma contracts; the next pair's tensor and multipole are fetched into registers while
+// the current one is contracted.  Per-target sums are in list order: deterministic.
+// DPL: tensor entries per lane, ceil(nd / 64) rounded up to 3, 5, 8 or 16 (orders 4, 5, 6, 8).
+constexpr int kMPerLane = (kFmmMaxNm + 63) / 64;  // 3
+constexpr int kRegNm = 36;  // register contraction for nm <= 35 (orders <= 4), padded to 4
+
+__device__ __forceinline__ double ReadLaneF64(double v, int src_lane) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src_lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src_lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <int DPL, bool REG>
 __global__ __launch_bounds__(64 * kM2LWaves) void M2LKernel(int nn, const Cell* __restrict__ cells,
                                                              const uint64_t* __restrict__ list,
-                                                             const int64_t* __restrict__ off, FmmTab t,
+                                                             const int64_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ pair_cls,
+                                                             const double* __restrict__ Dc, FmmTab t,
                                                              const double* __restrict__ M, double* __restrict__ L) {
-  // Dynamic LDS sized for the order: D[w][nd], Ms[w][nm], part[w][nm], sign[nm] (doubles), then
-  // m2l[nm*nm] and rec[nd*10] (ints, when the order allows).
+  // Dynamic LDS: per wave D[nd] and Ms[nm] (doubles); sign[nm]; then, for orders <= kM2LLdsOrder,
+  // the m2l index table [nm][nm] and the recurrence table [nd][10] (ints).
   extern __shared__ double lds[];
   const int nm = t.nm, nd = t.nd;
-  double* D0 = lds;
-  double* Ms0 = D0 + kM2LWaves * nd;
-  double* part0 = Ms0 + kM2LWaves * nm;
-  double* s_sign = part0 + kM2LWaves * nm;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* s_sign = lds + kM2LWaves * (nd + nm);
   int* s_m2l = (int*)(s_sign + nm);
   int* s_rec = s_m2l + nm * nm;
-  const int p = blockIdx.x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t b = off[p], e = off[p + 1];
-  if (b == e) {
-    for (int g = threadIdx.x; g < nm; g += 64 * kM2LWaves) L[(size_t)p * nm + g] = 0.0;
-    return;
-  }
   const bool lds_tab = t.P <= kM2LLdsOrder;
   if (lds_tab) {
     for (int k = threadIdx.x; k < nm * nm; k += 64 * kM2LWaves) s_m2l[k] = t.m2l[k];
-    for (int k = threadIdx.x; k < t.nd * 10; k += 64 * kM2LWaves) s_rec[k] = t.rec[k];
+    for (int k = threadIdx.x; k < nd * 10; k += 64 * kM2LWaves) s_rec[k] = t.rec[k];
   }
   for (int k = threadIdx.x; k < nm; k += 64 * kM2LWaves) s_sign[k] = t.sign[k];
+  __syncthreads();  // the only workgroup barrier: waves run independently from here
+  const int p = blockIdx.x * kM2LWaves + wave;
+  if (p >= nn) return;
   const int* m2l = lds_tab ? s_m2l : t.m2l;
   const int* rec = lds_tab ? s_rec : t.rec;
-  double* Dw = D0 + wave * nd;
-  double* Mw = Ms0 + wave * nm;
+  double* Dw = lds + wave * (nd + nm);
+  double* Mw = Dw + nd;
   const Cell T = cells[p];
   double acc[3] = {0.0, 0.0, 0.0};
-  const int64_t count = e - b;
-  const int64_t rounds = (count + kM2LWaves - 1) / kM2LWaves;
-  for (int64_t r = 0; r < rounds; ++r) {
-    const int64_t q = b + r * kM2LWaves + wave;
-    const bool active = q < e;
-    __syncthreads();
-    double xs[3] = {0.0, 0.0, 0.0}, inv_r2 = 0.0;
-    if (active) {
-      const int s = (int)(uint32_t)list[q];
-      const Cell& S = cells[s];
-      xs[0] = T.cx - S.cx;
-      xs[1] = T.cy - S.cy;
-      xs[2] = T.cz - S.cz;
-      for (int k = lane; k < nm; k += 64) Mw[k] = s_sign[k] * M[(size_t)s * nm + k];
-      inv_r2 = 1.0 / (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2]);
-      if (lane == 0) Dw[0] = sqrt(inv_r2);
+  const int64_t b = off[p], e = off[p + 1];
+  double dreg[DPL], mreg[kMPerLane];
+  int ixr[REG ? kRegNm : 1];
+  if (REG)
+#pragma unroll
+    for (int al = 0; al < (REG ? kRegNm : 1); ++al) ixr[al] = (lane < nm && al < nm) ? m2l[lane * nm + al] : 0;
+  double msig = 0.0;  // REG: this lane's signed multipole coefficient of the current pair
+  int s_cur = 0;
+  uint32_t c_cur = kNoClass;
+  auto fetch = [&](int64_t q, int& s, uint32_t& c) {
+    s = (int)(uint32_t)list[q];
+    c = pair_cls[q];
+#pragma unroll
+    for (int r = 0; r < kMPerLane; ++r) {
+      const int k = lane + 64 * r;
+      mreg[r] = k < nm ? M[(size_t)s * nm + k] : 0.0;
     }
-    for (int g = 1; g <= 2 * t.P; ++g) {
-      __syncthreads();
-      if (!active) continue;
-      for (int k = t.deg_start[g] + lane; k < t.deg_start[g + 1]; k += 64) {
-        const int* rr = rec + 10 * k;
-        const int i = rr[0];
-        double sum = (2 * rr[1 + i] - 1) * xs[i] * Dw[rr[4 + i]];
-        if (rr[7 + i] >= 0) sum += (rr[1 + i] - 1) * (rr[1 + i] - 1) * Dw[rr[7 + i]];
-        for (int j = 0; j < 3; ++j) {
-          if (j == i || rr[1 + j] == 0) continue;
-          sum += 2.0 * rr[1 + j] * xs[j] * Dw[rr[4 + j]];
-          if (rr[7 + j] >= 0) sum += rr[1 + j] * (rr[1 + j] - 1) * Dw[rr[7 + j]];
-        }
-        Dw[k] = -sum * inv_r2;
+    if (c != kNoClass) {
+      const double* src = Dc + (size_t)c * nd;
+#pragma unroll
+      for (int r = 0; r < DPL; ++r) {
+        const int k = lane + 64 * r;
+        if (k < nd) dreg[r] = src[k];
       }
     }
-    __syncthreads();
-    if (!active) continue;
+  };
+  if (b < e) fetch(b, s_cur, c_cur);
+  for (int64_t q = b; q < e; ++q) {
+    if (REG) {
+      msig = lane < nm ? s_sign[lane] * mreg[0] : 0.0;
+    } else {
+#pragma unroll
+      for (int r = 0; r < kMPerLane; ++r) {
+        const int k = lane + 64 * r;
+        if (k < nm) Mw[k] = s_sign[k] * mreg[r];
+      }
+    }
+    if (c_cur != kNoClass) {
+#pragma unroll
+      for (int r = 0; r < DPL; ++r) {
+        const int k = lane + 64 * r;
+        if (k < nd) Dw[k] = dreg[r];
+      }
+      WaveSync();
+    } else {
+      const Cell& S = cells[s_cur];
+      const double xs[3] = {T.cx - S.cx, T.cy - S.cy, T.cz - S.cz};
+      DerivTensorWave(xs, t, rec, Dw);
+    }
+    if (q + 1 < e) fetch(q + 1, s_cur, c_cur);  // in flight during the contraction
+    if (REG) {
+      // nm <= 35: lane g holds its index row in registers; multipole coefficient al is broadcast
+      // from lane al (after the sign) with readlane, so only the tensor gather touches LDS.
+      if (lane < nm) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int al = 0; al < kRegNm; al += 4) {
+          if (al + 0 < nm) a0 += ReadLaneF64(msig, al + 0) * Dw[ixr[al + 0]];
+          if (al + 1 < nm) a1 += ReadLaneF64(msig, al + 1) * Dw[ixr[al + 1]];
+          if (al + 2 < nm) a2 += ReadLaneF64(msig, al + 2) * Dw[ixr[al + 2]];
+          if (al + 3 < nm) a3 += ReadLaneF64(msig, al + 3) * Dw[ixr[al + 3]];
+        }
+        acc[0] += ((a0 + a1) + (a2 + a3)) * t.inv_fact[lane];
+      }
+      WaveSync();
+      continue;
+    }
     for (int k = 0; k < 3; ++k) {
       const int g = lane + 64 * k;
-      if (g >= nm) continue;
+      if (g >= nm) break;
       const int* ix = m2l + g * nm;
-      double a = 0.0;
-      for (int al = 0; al < nm; ++al) a += Mw[al] * Dw[ix[al]];
-      acc[k] += a * t.inv_fact[g];
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int al = 0;
+      for (; al + 4 <= nm; al += 4) {
+        a0 += Mw[al] * Dw[ix[al]];
+        a1 += Mw[al + 1] * Dw[ix[al + 1]];
+        a2 += Mw[al + 2] * Dw[ix[al + 2]];
+        a3 += Mw[al + 3] * Dw[ix[al + 3]];
+      }
+      for (; al < nm; ++al) a0 += Mw[al] * Dw[ix[al]];
+      acc[k] += ((a0 + a1) + (a2 + a3)) * t.inv_fact[g];
     }
+    WaveSync();  // the slice is rewritten by the next pair
   }
   for (int k = 0; k < 3; ++k) {
     const int g = lane + 64 * k;
-    if (g < nm) part0[wave * nm + g] = acc[k];
-  }
-  __syncthreads();
-  for (int g = threadIdx.x; g < nm; g += 64 * kM2LWaves) {
-    double v = 0.0;
-    for (int w = 0; w < kM2LWaves; ++w) v += part0[w * nm + g];
-    L[(size_t)p * nm + g] = v;
+    if (g < nm) L[(size_t)p * nm + g] = acc[k];
   }
 }
 
@@ -653,7 +837,7 @@ __global__ void ZeroNeutralForceKernel(int64_t n, const uint8_t* __restrict__ fl
 }
 
 struct FmmRunStats {
-  int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0;
+  int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0, m2l_classes = 0;
   int max_depth = 0;
 };
 static FmmRunStats g_fmm_stats;
@@ -811,11 +995,53 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   AM_HIP(hipGetLastError());
 
   TimerBegin(ctx, "fmm_m2l", &slot);
-  const size_t m2l_lds = sizeof(double) * ((size_t)kM2LWaves * (tab.nd + 2 * tab.nm) + tab.nm) +
-                         (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
-  hipLaunchKernelGGL(M2LKernel, dim3(nn), dim3(64 * kM2LWaves), m2l_lds, st, nn, (const Cell*)cells,
-                     (const uint64_t*)m2l,
-                     (const int64_t*)off_m2l, tab, (const double*)M, (double*)L);
+  {
+    // classes of the M2L pairs (table of at most 2^22 slots, load factor <= 1/2 for up to 2^21 keys)
+    uint64_t tsize = 1024;
+    while (tsize < 2 * (uint64_t)std::max<int64_t>(n_m2l, 1) && tsize < ((uint64_t)1 << 22)) tsize <<= 1;
+    void *hk, *pslot, *occ, *ids;
+    if ((rc = Scratch(ctx, "fmm_cls_keys", tsize * 8, &hk))) return rc;
+    if ((rc = Scratch(ctx, "fmm_pair_slot", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &pslot))) return rc;
+    if ((rc = Scratch(ctx, "fmm_cls_occ", tsize * 4, &occ))) return rc;
+    if ((rc = Scratch(ctx, "fmm_cls_ids", (tsize + 1) * 4, &ids))) return rc;
+    hipLaunchKernelGGL(ClassClearKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize, (uint64_t*)hk);
+    if (n_m2l > 0)
+      hipLaunchKernelGGL(M2LClassInsertKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                         (const uint64_t*)m2l, (const Cell*)cells, (uint64_t*)hk, tsize - 1, (uint32_t*)pslot);
+    hipLaunchKernelGGL(ClassOccupiedKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize,
+                       (const uint64_t*)hk, (uint32_t*)occ);
+    AM_HIP(hipGetLastError());
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)occ, (uint32_t*)ids, (int64_t)tsize))) return rc;
+    uint32_t n_cls = 0;
+    AM_HIP(hipMemcpyAsync(&n_cls, (uint32_t*)ids + tsize, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    void *ckey, *Dc;
+    if ((rc = Scratch(ctx, "fmm_cls_key", (size_t)std::max<uint32_t>(n_cls, 1) * 8, &ckey))) return rc;
+    if ((rc = Scratch(ctx, "fmm_cls_tensor", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 8, &Dc))) return rc;
+    hipLaunchKernelGGL(ClassListKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize,
+                       (const uint64_t*)hk, (const uint32_t*)ids, (uint64_t*)ckey);
+    if (n_cls > 0)
+      hipLaunchKernelGGL(ClassTensorKernel, dim3(n_cls), dim3(64), sizeof(double) * tab.nd, st, (int)n_cls,
+                         (const uint64_t*)ckey, (const Cell*)cells, tab, (double*)Dc);
+    const size_t m2l_lds =
+        sizeof(double) * ((size_t)kM2LWaves * (tab.nd + tab.nm) + tab.nm) +
+        (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
+    if (n_m2l > 0)
+      hipLaunchKernelGGL(PairClassKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (const uint32_t*)pslot,
+                         (const uint32_t*)ids, (uint32_t*)pslot);  // in place: slot -> class id
+    const int dpl = (tab.nd + 63) / 64;
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(Blocks(nn, kM2LWaves)), dim3(64 * kM2LWaves), m2l_lds, st, nn,
+                         (const Cell*)cells, (const uint64_t*)m2l, (const int64_t*)off_m2l, (const uint32_t*)pslot,
+                         (const double*)Dc, tab, (const double*)M, (double*)L);
+    };
+    if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
+    else if (dpl <= 3) launch(M2LKernel<3, false>);
+    else if (dpl <= 5) launch(M2LKernel<5, false>);
+    else if (dpl <= 8) launch(M2LKernel<8, false>);
+    else launch(M2LKernel<16, false>);
+    g_fmm_stats.m2l_classes = n_cls;
+  }
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   TimerBegin(ctx, "fmm_downward", &slot);
