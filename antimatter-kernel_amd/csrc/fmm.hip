@@ -718,6 +718,25 @@ __global__ __launch_bounds__(64) void L2LKernel(int n0, NodeArrays nd, const uin
 
 // L2P + P2P (fast-multipole.cc:580-661) and unscaling (:663-674): one wave per leaf; lane l owns
 // particle b + (l & 31) of the current 32-particle chunk and the sources of parity l >> 5.
+// Near-field sources of every target leaf as one contiguous index range: pair q (sorted by target)
+// contributes its source cell's particles at soff[q].
+__global__ void P2PCountKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
+                               uint32_t* __restrict__ cnt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const Cell& S = cells[(uint32_t)pairs[q]];
+  cnt[q] = (uint32_t)(S.e - S.b);
+}
+
+__global__ void P2PIndexKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
+                               const uint32_t* __restrict__ soff, uint32_t* __restrict__ sidx) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const Cell& S = cells[(uint32_t)pairs[q]];
+  const uint32_t o = soff[q];
+  for (int j = S.b; j < S.e; ++j) sidx[o + (uint32_t)(j - S.b)] = (uint32_t)j;
+}
+
 __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restrict__ cells,
                                                    const uint64_t* __restrict__ list,
                                                    const int64_t* __restrict__ off, FmmTab t,
@@ -725,9 +744,12 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
                                                    const uint32_t* __restrict__ order,
                                                    const FmmFrame* __restrict__ fr, int want_forces,
                                                    const float4* __restrict__ sp32,
+                                                   const uint32_t* __restrict__ soff,
+                                                   const uint32_t* __restrict__ sidx,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz, double* __restrict__ phi_out) {
   __shared__ double lt[kFmmMaxNm];
+  __shared__ float4 s_hi[64], s_lo[64];  // FP32 near field: the staged chunk of sources
   const int p = blockIdx.x;
   const Cell T = cells[p];
   if (!T.leaf) return;
@@ -760,28 +782,40 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
       }
     }
     double fx32 = 0.0, fy32 = 0.0, fz32 = 0.0;  // FP32 near field, e/unit^2
-    if (valid && sp32 != nullptr && want_forces) {
-      const float4 th = sp32[2 * i], tl = sp32[2 * i + 1];
-      for (int64_t q = lb; q < le; ++q) {
-        const Cell& S = cells[(int)(uint32_t)list[q]];
-        float ax = 0.f, ay = 0.f, az = 0.f;
-        for (int j = S.b + half; j < S.e; j += 2) {
-          if (j == i) continue;
-          const float4 sh = sp32[2 * j];
-          const float4 sl = sp32[2 * j + 1];
-          const float dx = (th.x - sh.x) + (tl.x - sl.x);
-          const float dy = (th.y - sh.y) + (tl.y - sl.y);
-          const float dz = (th.z - sh.z) + (tl.z - sl.z);
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
-          const float ri = __builtin_amdgcn_rsqf(r2);
-          const float c = (sh.w * ri) * (ri * ri);
-          ax = fmaf(c, dx, ax);
-          ay = fmaf(c, dy, ay);
-          az = fmaf(c, dz, az);
+    if (sp32 != nullptr && want_forces) {
+      // The leaf's near sources [soff[lb], soff[le]) in chunks of 64: lane l fetches source l of the
+      // next chunk while the current one is summed; half h takes sources 32h..32h+31 of a chunk.
+      // A source equal to the target contributes exactly 0 (softened coincident pair); padding
+      // entries past the end have charge 0.
+      const int tid = threadIdx.x;
+      const float4 th = sp32[2 * (valid ? i : T.b)], tl = sp32[2 * (valid ? i : T.b) + 1];
+      const uint32_t a = soff[lb], z = soff[le];
+      float4 nh = make_float4(0.f, 0.f, 0.f, 0.f), nl = nh;
+      if (a + tid < z) {
+        const uint32_t j = sidx[a + tid];
+        nh = sp32[2 * j];
+        nl = sp32[2 * j + 1];
+      }
+      for (uint32_t k0 = a; k0 < z; k0 += 64) {
+        __syncthreads();
+        s_hi[tid] = nh;
+        s_lo[tid] = nl;
+        __syncthreads();
+        nh = make_float4(0.f, 0.f, 0.f, 0.f);
+        nl = nh;
+        if (k0 + 64 + tid < z) {
+          const uint32_t j = sidx[k0 + 64 + tid];
+          nh = sp32[2 * j];
+          nl = sp32[2 * j + 1];
         }
-        fx32 += ax;
-        fy32 += ay;
-        fz32 += az;
+        float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll 8
+        for (int u = 0; u < 32; ++u) PairF32(th, tl, s_hi[32 * half + u], s_lo[32 * half + u], ax, ay, az);
+        if (valid) {
+          fx32 += ax;
+          fy32 += ay;
+          fz32 += az;
+        }
       }
     } else if (valid) {
       for (int64_t q = lb; q < le; ++q) {
@@ -1064,10 +1098,30 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = Scratch(ctx, "fmm_phi", (size_t)m * 8, &phi))) return rc;
   }
   TimerBegin(ctx, "fmm_l2p_p2p", &slot);
+  void *soff = nullptr, *sidx = nullptr;
+  if (sp32) {
+    if (n_p2p >= ((int64_t)1 << 32) / 64) return Fail(AM_ERR_CAPACITY, "FMM: too many near-field cell pairs");
+    void* cnt;
+    if ((rc = Scratch(ctx, "fmm_p2p_cnt", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &cnt))) return rc;
+    if ((rc = Scratch(ctx, "fmm_p2p_soff", (size_t)(n_p2p + 1) * 4, &soff))) return rc;
+    if (n_p2p > 0)
+      hipLaunchKernelGGL(P2PCountKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (const uint64_t*)p2p,
+                         (const Cell*)cells, (uint32_t*)cnt);
+    AM_HIP(hipGetLastError());
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)soff, n_p2p))) return rc;
+    uint32_t total = 0;
+    AM_HIP(hipMemcpyAsync(&total, (uint32_t*)soff + n_p2p, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    if ((rc = Scratch(ctx, "fmm_p2p_sidx", (size_t)std::max<uint32_t>(total, 1) * 4, &sidx))) return rc;
+    if (n_p2p > 0)
+      hipLaunchKernelGGL(P2PIndexKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (const uint64_t*)p2p,
+                         (const Cell*)cells, (const uint32_t*)soff, (uint32_t*)sidx);
+    AM_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(L2PP2PKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)p2p,
                      (const int64_t*)off_p2p, tab, (const double*)L, (const double4*)sp, (const uint32_t*)o.order,
-                     (const FmmFrame*)frame, want_forces ? 1 : 0, (const float4*)sp32, s->fx, s->fy, s->fz,
-                     (double*)phi);
+                     (const FmmFrame*)frame, want_forces ? 1 : 0, (const float4*)sp32, (const uint32_t*)soff,
+                     (const uint32_t*)sidx, s->fx, s->fy, s->fz, (double*)phi);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   g_fmm_stats.cells = nn;
