@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
   int next = want ? 0 : nn;
   double ax = 0.0, ay = 0.0, az = 0.0;
   int64_t cnt = 0;
-  int c = WaveMin(next);
+  int c = __builtin_amdgcn_readfirstlane(WaveMin(next));  // wave-uniform: node records come in by scalar loads
   while (c < nn) {
     const WalkNode nd = nodes[c];
     const bool mine = next == c;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         }
       }
     }
-    c = __ballot(opened) ? c + 1 : WaveMin(next);
+    c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : WaveMin(next));
   }
   if (!valid) return;
   if (rec_cnt) rec_cnt[k] = nrec;
