@@ -272,7 +272,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
                                                      const uint32_t* __restrict__ order,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, const double* __restrict__ q,
-                                                     const float4* __restrict__ src, double theta2,
+                                                     const float4* __restrict__ src,
+                                                     const float4* __restrict__ ssrc, double theta2,
                                                      double* __restrict__ out_x, double* __restrict__ out_y,
                                                      double* __restrict__ out_z, int64_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec,
@@ -280,7 +281,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < m;
   const int kc = valid ? k : m - 1;
-  const uint32_t i = order[tsel ? tsel[kc] : kc];
+  const int ks = tsel ? (int)tsel[kc] : kc;  // the target's position in the tree order
+  const uint32_t i = order[ks];
   uint32_t nrec = 0;
   const double px = x[i], py = y[i], pz = z[i], qi = q[i];
   const float4 th = src[2 * i], tl = src[2 * i + 1];
@@ -300,13 +302,13 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         next = nd.skip;
       } else if (nd.leaf) {
         if (MODE == 0) {
+          // the leaf's sources are contiguous in the tree-ordered copy of the FP32 mirror
           float fx = 0.f, fy = 0.f, fz = 0.f;
           for (int kk = nd.b; kk < nd.e; ++kk) {
-            const uint32_t j = order[kk];
-            if (j == i) continue;
-            const float4 sh = src[2 * j];
+            if (kk == ks) continue;
+            const float4 sh = ssrc[2 * kk];
             if (sh.w == 0.f) continue;
-            PairF32(th, tl, sh, src[2 * j + 1], fx, fy, fz);
+            PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
             ++cnt;
           }
           ax += fx;
@@ -390,6 +392,16 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
+// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position).
+__global__ void GatherTreeSrcKernel(int m, const uint32_t* __restrict__ order, const float4* __restrict__ src,
+                                    float4* __restrict__ ssrc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = order[k];
+  ssrc[2 * k] = src[2 * i];
+  ssrc[2 * k + 1] = src[2 * i + 1];
+}
+
 int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
@@ -450,6 +462,12 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
                      (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeF32*)wn32, sm,
                      (FeatNode*)featp);
   AM_HIP(hipGetLastError());
+  void* ssrc;
+  if ((rc = Scratch(ctx, "bh_ssrc", (size_t)m * 2 * sizeof(float4), &ssrc))) return rc;
+  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
+                     (const float4*)s->src, (float4*)ssrc);
+  AM_HIP(hipGetLastError());
+  t->ssrc = (float4*)ssrc;
   t->feat = (FeatNode*)featp;
   t->nn = nn;
   t->order = o.order;
@@ -475,15 +493,15 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   const dim3 grid(Blocks(t.m, 256)), block(256);
   if (mode == 0)
     hipLaunchKernelGGL(BHWalkKernel<0>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
                        nullptr, 0, nullptr);
   else if (mode == 1)
     hipLaunchKernelGGL(BHWalkKernel<1>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
                        nullptr, 0, nullptr);
   else
     hipLaunchKernelGGL(BHWalkKernel<2>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, theta2, ox, oy, oz, counts, nullptr,
+                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
                        nullptr, 0, nullptr);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
@@ -497,7 +515,8 @@ int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* 
   int slot;
   TimerBegin(ctx, "bh_walk_f32", &slot);
   hipLaunchKernelGGL(BHWalkKernel<0>, dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
-                     t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, theta * theta, ox, oy, oz,
+                     t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
+                     ox, oy, oz,
                      (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());

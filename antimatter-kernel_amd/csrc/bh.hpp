@@ -33,6 +33,7 @@ struct BHTree {
   WalkNode* nodes = nullptr;
   WalkNodeF32* nodes32 = nullptr;
   FeatNode* feat = nullptr;  // only when built with features
+  float4* ssrc = nullptr;    // FP32 hi/lo mirror in tree order (2 per sorted position)
   int max_depth = 0;
   uint32_t deep_overflow = 0;
 };
