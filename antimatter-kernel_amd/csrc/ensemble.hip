@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "celltable.hpp"
+#include "pairtile.hpp"
 #include "physics.hpp"
 
 namespace am {
@@ -35,20 +36,22 @@ struct TrapRec {
   double b0, u0, d, dt, tau_scale, bath_temp, contact_radius, probability;
   double inv_h;  // contact cell scale
   int annihilation;
-  int chunks;
-  int64_t part_base;
+  int chunks;         // source chunks of the force work items
+  int64_t part_base;  // the trap's partial sums: [chunk][3][ntile * kTile]
+  int tile0, ntile;   // the trap's tiles in the sorted layout (ntile even: whole 256-target blocks)
+  int64_t sorted0;    // first rank of the trap's particles in the sorted keys
 };
 
+// Force work item: a 256-target block of a trap against one chunk of the trap's source tiles.
 struct EnsItem {
   int trap;
-  int t0;  // first target (store index)
-  int nt;  // targets in the block (<= 256)
-  int s0, s1;  // source range (store indices)
-  int chunk;
+  int blk;    // target block (tiles 2 blk, 2 blk + 1 of the trap)
+  int chunk;  // source tiles [chunk * tiles_per_chunk, ...)
+  int cost;
 };
 
 constexpr int kEnsThreads = 256;
-constexpr int kEnsTile = 128;
+constexpr int kEnsTile = kTile;
 constexpr int kMaxTraps = 1024;  // trap id in 10 key bits
 
 __host__ __device__ inline uint64_t TrapSeed(uint64_t seed, uint32_t t) { return seed + (uint64_t)t * 0x9E3779B97F4A7C15ull; }
@@ -57,10 +60,105 @@ __global__ void EnsResetKernel(unsigned* __restrict__ counter) { *counter = 0u; 
 
 __global__ void EnsStepKernel(uint64_t* __restrict__ step) { *step += 1ull; }
 
+// ---- forces: per-trap Morton-sorted tiles (the all-pairs kernel's tile machinery) -----------------
+// Every step: each trap's bounding box, 30-bit Morton keys with the trap id above them, one radix
+// sort, the per-trap sorted layout (trap t's particles at tiles tile0_t.., padded to whole
+// 256-target blocks), tile preparation, then the persistent force kernel over (trap, block,
+// source chunk) items and the fixed-order chunk reduction.
+constexpr int kEnsKeyBits = 41;  // 11 trap bits (kMaxTraps = particles outside every trap) + 30
+
+__global__ __launch_bounds__(256) void EnsBoxKernel(const TrapRec* __restrict__ traps, const double* __restrict__ x,
+                                                    const double* __restrict__ y, const double* __restrict__ z,
+                                                    double* __restrict__ box) {
+  __shared__ double lo[3][256], hi[3][256];
+  const TrapRec& tr = traps[blockIdx.x];
+  const int t = threadIdx.x;
+  double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = tr.begin + t; i < tr.end; i += 256) {
+    const double p[3] = {x[i], y[i], z[i]};
+    for (int d = 0; d < 3; ++d) {
+      l[d] = fmin(l[d], p[d]);
+      h[d] = fmax(h[d], p[d]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    lo[d][t] = l[d];
+    hi[d][t] = h[d];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int d = 0; d < 3; ++d) {
+        lo[d][t] = fmin(lo[d][t], lo[d][t + w]);
+        hi[d][t] = fmax(hi[d][t], hi[d][t + w]);
+      }
+    __syncthreads();
+  }
+  if (t < 3) {
+    const bool any = lo[t][0] <= hi[t][0];
+    box[6 * blockIdx.x + t] = any ? lo[t][0] : 0.0;
+    box[6 * blockIdx.x + 3 + t] = any ? hi[t][0] : 0.0;
+  }
+}
+
+__global__ void EnsSortKeyKernel(int64_t n, int n_traps, const uint32_t* __restrict__ trap,
+                                 const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, const double* __restrict__ box,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vals[i] = (uint32_t)i;
+  const uint32_t t = trap[i];
+  if (t >= (uint32_t)n_traps) {
+    keys[i] = (uint64_t)kMaxTraps << 30;
+    return;
+  }
+  const double* b = box + 6 * t;
+  const double ext = fmax(fmax(b[3] - b[0], b[4] - b[1]), fmax(b[5] - b[2], 1e-300));
+  const double sc = 1023.999 / ext;
+  const double p[3] = {x[i], y[i], z[i]};
+  uint32_t g[3];
+  for (int d = 0; d < 3; ++d) {
+    double c = (p[d] - b[d]) * sc;
+    c = (c == c) ? fmin(fmax(c, 0.0), 1023.0) : 0.0;
+    g[d] = (uint32_t)c;
+  }
+  keys[i] = ((uint64_t)t << 30) | (uint64_t)(Spread10(g[0]) | (Spread10(g[1]) << 1) | (Spread10(g[2]) << 2));
+}
+
+// Rank r of the sorted keys -> the trap's sorted layout.
+__global__ void EnsScatterPermKernel(int64_t n_in, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                     const TrapRec* __restrict__ traps, uint32_t* __restrict__ perm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_in) return;
+  const TrapRec& tr = traps[keys[r] >> 30];
+  perm[(int64_t)tr.tile0 * kEnsTile + (r - tr.sorted0)] = vals[r];
+}
+
+__global__ __launch_bounds__(kEnsTile) void EnsTilePrepKernel(const int* __restrict__ tile_trap,
+                                                              const TrapRec* __restrict__ traps,
+                                                              uint32_t* __restrict__ perm, const double* __restrict__ x,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ z,
+                                                              const float4* __restrict__ src, float4* __restrict__ shl,
+                                                              float4* __restrict__ sloc, TileRec* __restrict__ tiles) {
+  const int T = blockIdx.x;
+  const TrapRec& tr = traps[tile_trap[T]];
+  const int nvalid = (int)((tr.end - tr.begin) - (int64_t)(T - tr.tile0) * kEnsTile);
+  TilePrepBody(T * kEnsTile, nvalid, T, perm, x, y, z, src, shl, sloc, tiles);
+}
+
 __global__ __launch_bounds__(kEnsThreads) void EnsembleForceKernel(const EnsItem* __restrict__ items, int n_items,
                                                                    unsigned* __restrict__ counter,
                                                                    const TrapRec* __restrict__ traps,
-                                                                   const float4* __restrict__ src,
+                                                                   int tiles_per_chunk,
+                                                                   const uint32_t* __restrict__ perm,
+                                                                   const double* __restrict__ x,
+                                                                   const double* __restrict__ y,
+                                                                   const double* __restrict__ z,
+                                                                   const float4* __restrict__ shl,
+                                                                   const float4* __restrict__ sloc,
+                                                                   const TileRec* __restrict__ tiles,
                                                                    double* __restrict__ part) {
   __shared__ int s_item;
   for (;;) {
@@ -70,57 +168,58 @@ __global__ __launch_bounds__(kEnsThreads) void EnsembleForceKernel(const EnsItem
     const int it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= n_items) return;
     const EnsItem w = items[it];
-    const int lt = threadIdx.x;
-    const bool valid = lt < w.nt;
-    const int i = w.t0 + (valid ? lt : 0);
-    const float4 th = src[2 * i], tl = src[2 * i + 1];
+    const TrapRec& tr = traps[w.trap];
+    const int N = (int)(tr.end - tr.begin);
+    const int local = w.blk * kEnsThreads + (int)threadIdx.x;
+    const int64_t k = (int64_t)tr.tile0 * kEnsTile + local;
+    const int tb = tr.tile0 + 2 * w.blk;
+    const BlockBox bb = MakeBlockBox(tiles[tb], tiles[tb + 1], w.blk * kEnsThreads + kEnsTile >= N);
+    const float4 th = shl[2 * k], tl = shl[2 * k + 1];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    if (local < N) {
+      const uint32_t i = perm[k];
+      lx = (float)(x[i] * kUnitScale - bb.bc[0]);
+      ly = (float)(y[i] * kUnitScale - bb.bc[1]);
+      lz = (float)(z[i] * kUnitScale - bb.bc[2]);
+    }
+    const int t0 = tr.tile0 + w.chunk * tiles_per_chunk;
+    const int t1 = min(tr.tile0 + tr.ntile, t0 + tiles_per_chunk);
     double ax = 0.0, ay = 0.0, az = 0.0;
-    int j = w.s0;
-    for (; j + kEnsTile <= w.s1; j += kEnsTile) {
+    for (int T = t0; T < t1; ++T) {
       float px = 0.f, py = 0.f, pz = 0.f;
-      const float4* s = src + 2 * j;
-#pragma unroll 8
-      for (int k = 0; k < kEnsTile; ++k) PairF32(th, tl, s[2 * k], s[2 * k + 1], px, py, pz);
+      TileSum(tiles[T], T * kEnsTile, bb, th, tl, lx, ly, lz, shl, sloc, px, py, pz);
       ax += (double)px;
       ay += (double)py;
       az += (double)pz;
     }
-    {
-      float px = 0.f, py = 0.f, pz = 0.f;
-      for (; j < w.s1; ++j) PairF32(th, tl, src[2 * j], src[2 * j + 1], px, py, pz);
-      ax += (double)px;
-      ay += (double)py;
-      az += (double)pz;
-    }
-    if (valid) {
-      const TrapRec& tr = traps[w.trap];
-      const int64_t N = tr.end - tr.begin;
-      const int64_t l = i - tr.begin;
-      double* out = part + tr.part_base + (int64_t)w.chunk * 3 * N;
-      out[l] = ax;
-      out[N + l] = ay;
-      out[2 * N + l] = az;
-    }
+    const int64_t npad = (int64_t)tr.ntile * kEnsTile;
+    double* out = part + tr.part_base + (int64_t)w.chunk * 3 * npad;
+    out[local] = ax;
+    out[npad + local] = ay;
+    out[2 * npad + local] = az;
   }
 }
 
-__global__ void EnsembleReduceKernel(int64_t n, const uint32_t* __restrict__ trap, const TrapRec* __restrict__ traps,
+// One thread per position of the sorted layout: the fixed-order sum over chunks, scaled into N.
+__global__ void EnsembleReduceKernel(int64_t n_pos, const int* __restrict__ tile_trap,
+                                     const TrapRec* __restrict__ traps, const uint32_t* __restrict__ perm,
                                      const double* __restrict__ part, const double* __restrict__ q,
                                      const uint8_t* __restrict__ flags, double* __restrict__ fx,
                                      double* __restrict__ fy, double* __restrict__ fz) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t t = trap[i];
-  if (t >= (uint32_t)kMaxTraps) return;  // particle outside every trap: untouched
-  const TrapRec& tr = traps[t];
-  const int64_t N = tr.end - tr.begin, l = i - tr.begin;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_pos) return;
+  const TrapRec& tr = traps[tile_trap[k / kEnsTile]];
+  const int64_t local = k - (int64_t)tr.tile0 * kEnsTile;
+  if (local >= tr.end - tr.begin) return;
+  const int64_t npad = (int64_t)tr.ntile * kEnsTile;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int c = 0; c < tr.chunks; ++c) {
-    const double* p = part + tr.part_base + (int64_t)c * 3 * N;
-    sx += p[l];
-    sy += p[N + l];
-    sz += p[2 * N + l];
+    const double* p = part + tr.part_base + (int64_t)c * 3 * npad;
+    sx += p[local];
+    sy += p[npad + local];
+    sz += p[2 * npad + local];
   }
+  const uint32_t i = perm[k];
   const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0;
   const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
   fx[i] = scale * sx;
@@ -344,6 +443,18 @@ struct am_ensemble {
   uint64_t* hkeys = nullptr;  // cell table of the sorted contact keys
   uint2* hrange = nullptr;
   uint64_t hsize = 0;
+  // force buffers: sort keys, sorted layout, tiles
+  uint64_t* fkeys = nullptr;
+  uint32_t* fvals = nullptr;
+  double* tbox = nullptr;
+  uint32_t* perm = nullptr;
+  float4* shl = nullptr;
+  float4* sloc = nullptr;
+  TileRec* tiles = nullptr;
+  int* tile_trap = nullptr;
+  int n_tiles = 0;
+  int64_t n_in = 0;  // particles inside traps
+  int tiles_per_chunk = 32;
   uint32_t* ccount = nullptr;
   uint32_t* coff = nullptr;
   uint2* cpairs = nullptr;
@@ -365,32 +476,31 @@ static int EnsAlloc(am_ensemble* e, size_t bytes, void** p) {
   return AM_OK;
 }
 
-// Work items and partial-buffer layout for a source-chunk length.
+// Work items and partial-buffer layout for a source-chunk length (rounded to whole tiles).
 static int EnsBuildItems(am_ensemble* e, int chunk) {
   am_ctx* ctx = e->store->ctx;
+  const int tpc = std::max(1, (chunk + kEnsTile - 1) / kEnsTile);
   std::vector<EnsItem> items;
   int64_t base = 0;
   for (int t = 0; t < e->n_traps; ++t) {
     TrapRec& tr = e->traps[t];
     const int64_t N = tr.end - tr.begin;
-    tr.chunks = N > 0 ? (int)((N + chunk - 1) / chunk) : 0;
+    tr.chunks = N > 0 ? (tr.ntile + tpc - 1) / tpc : 0;
     tr.part_base = base;
-    base += 3 * N * tr.chunks;
-    for (int64_t t0 = 0; t0 < N; t0 += kEnsThreads)
+    base += 3 * (int64_t)tr.ntile * kEnsTile * tr.chunks;
+    for (int b = 0; 2 * b < tr.ntile; ++b)
       for (int c = 0; c < tr.chunks; ++c) {
         EnsItem w;
         w.trap = t;
-        w.t0 = (int)(tr.begin + t0);
-        w.nt = (int)std::min<int64_t>(kEnsThreads, N - t0);
-        w.s0 = (int)(tr.begin + (int64_t)c * chunk);
-        w.s1 = (int)std::min<int64_t>(tr.end, tr.begin + (int64_t)(c + 1) * chunk);
+        w.blk = b;
         w.chunk = c;
+        const int nt = (int)std::min<int64_t>(kEnsThreads, N - (int64_t)b * kEnsThreads);
+        const int ns = std::min(tpc, tr.ntile - c * tpc);
+        w.cost = nt * ns;
         items.push_back(w);
       }
   }
-  std::stable_sort(items.begin(), items.end(), [](const EnsItem& a, const EnsItem& b) {
-    return (int64_t)a.nt * (a.s1 - a.s0) > (int64_t)b.nt * (b.s1 - b.s0);
-  });
+  std::stable_sort(items.begin(), items.end(), [](const EnsItem& a, const EnsItem& b) { return a.cost > b.cost; });
   void* p;
   int rc;
   if ((rc = Scratch(ctx, "ens_items", std::max<size_t>(items.size(), 1) * sizeof(EnsItem), &p))) return rc;
@@ -398,7 +508,8 @@ static int EnsBuildItems(am_ensemble* e, int chunk) {
   if ((rc = Scratch(ctx, "ens_part", (size_t)std::max<int64_t>(base, 1) * 8, &p))) return rc;
   e->part = (double*)p;
   e->n_items = (int)items.size();
-  e->chunk = chunk;
+  e->chunk = tpc * kEnsTile;
+  e->tiles_per_chunk = tpc;
   if (!items.empty())
     AM_HIP(hipMemcpyAsync(e->d_items, items.data(), items.size() * sizeof(EnsItem), hipMemcpyHostToDevice,
                           ctx->stream));
@@ -413,12 +524,32 @@ static int EnsForce(am_ensemble* e) {
   am_store* s = e->store;
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
+  const int64_t n = s->n;
+  int rc;
+  hipLaunchKernelGGL(EnsBoxKernel, dim3(e->n_traps), dim3(256), 0, st, (const TrapRec*)e->d_traps, s->x, s->y, s->z,
+                     e->tbox);
+  hipLaunchKernelGGL(EnsSortKeyKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, e->n_traps,
+                     (const uint32_t*)s->trap, s->x, s->y, s->z, (const double*)e->tbox, e->fkeys, e->fvals);
+  AM_HIP(hipGetLastError());
+  if ((rc = RadixSortPairs(ctx, e->fkeys, e->fvals, n, kEnsKeyBits))) return rc;
+  if (e->n_in > 0)
+    hipLaunchKernelGGL(EnsScatterPermKernel, dim3(Blocks(e->n_in, 256)), dim3(256), 0, st, e->n_in,
+                       (const uint64_t*)e->fkeys, (const uint32_t*)e->fvals, (const TrapRec*)e->d_traps, e->perm);
+  if (e->n_tiles > 0)
+    hipLaunchKernelGGL(EnsTilePrepKernel, dim3(e->n_tiles), dim3(kEnsTile), 0, st, (const int*)e->tile_trap,
+                       (const TrapRec*)e->d_traps, e->perm, s->x, s->y, s->z, (const float4*)s->src, e->shl, e->sloc,
+                       e->tiles);
   const int grid = std::max(1, std::min(e->n_items, (ctx->num_cus > 0 ? ctx->num_cus : 256) * 8));
   hipLaunchKernelGGL(EnsResetKernel, dim3(1), dim3(1), 0, st, e->counter);
   hipLaunchKernelGGL(EnsembleForceKernel, dim3(grid), dim3(kEnsThreads), 0, st, (const EnsItem*)e->d_items,
-                     e->n_items, e->counter, (const TrapRec*)e->d_traps, (const float4*)s->src, e->part);
-  hipLaunchKernelGGL(EnsembleReduceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, st, s->n, (const uint32_t*)s->trap,
-                     (const TrapRec*)e->d_traps, (const double*)e->part, s->q, s->flags, s->fx, s->fy, s->fz);
+                     e->n_items, e->counter, (const TrapRec*)e->d_traps, e->tiles_per_chunk, (const uint32_t*)e->perm,
+                     s->x, s->y, s->z, (const float4*)e->shl, (const float4*)e->sloc, (const TileRec*)e->tiles,
+                     e->part);
+  const int64_t n_pos = (int64_t)e->n_tiles * kEnsTile;
+  if (n_pos > 0)
+    hipLaunchKernelGGL(EnsembleReduceKernel, dim3(Blocks(n_pos, 256)), dim3(256), 0, st, n_pos,
+                       (const int*)e->tile_trap, (const TrapRec*)e->d_traps, (const uint32_t*)e->perm,
+                       (const double*)e->part, s->q, s->flags, s->fx, s->fy, s->fz);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -503,6 +634,12 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
     r.probability = d.probability;
     r.annihilation = d.annihilation ? 1 : 0;
     r.inv_h = d.annihilation ? 1.0 / (d.contact_radius * (1.0 + 1e-9)) : 0.0;
+    const int64_t N = d.end - d.begin;
+    r.tile0 = e->n_tiles;
+    r.ntile = (int)(2 * ((N + 2 * kEnsTile - 1) / (2 * kEnsTile)));
+    r.sorted0 = e->n_in;
+    e->n_tiles += r.ntile;
+    e->n_in += N;
     e->traps.push_back(r);
     e->any_annihilation |= r.annihilation != 0;
     for (int64_t i = d.begin; i < d.end; ++i) tid[i] = (uint32_t)t;
@@ -517,6 +654,31 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
   };
   if ((rc = EnsAlloc(e, sizeof(TrapRec) * n_traps, &p))) return fail(rc);
   e->d_traps = (TrapRec*)p;
+  {
+    const int64_t n_pos = (int64_t)e->n_tiles * kEnsTile;
+    if (n_pos >= ((int64_t)1 << 31)) return fail(Fail(AM_ERR_CAPACITY, "am_ensemble_create: too many particles"));
+    if ((rc = EnsAlloc(e, (size_t)n * 8, &p))) return fail(rc);
+    e->fkeys = (uint64_t*)p;
+    if ((rc = EnsAlloc(e, (size_t)n * 4, &p))) return fail(rc);
+    e->fvals = (uint32_t*)p;
+    if ((rc = EnsAlloc(e, (size_t)n_traps * 6 * 8, &p))) return fail(rc);
+    e->tbox = (double*)p;
+    if ((rc = EnsAlloc(e, (size_t)n_pos * 4, &p))) return fail(rc);
+    e->perm = (uint32_t*)p;
+    if ((rc = EnsAlloc(e, (size_t)n_pos * 2 * sizeof(float4), &p))) return fail(rc);
+    e->shl = (float4*)p;
+    if ((rc = EnsAlloc(e, (size_t)n_pos * sizeof(float4), &p))) return fail(rc);
+    e->sloc = (float4*)p;
+    if ((rc = EnsAlloc(e, (size_t)e->n_tiles * sizeof(TileRec), &p))) return fail(rc);
+    e->tiles = (TileRec*)p;
+    if ((rc = EnsAlloc(e, (size_t)e->n_tiles * 4, &p))) return fail(rc);
+    e->tile_trap = (int*)p;
+    std::vector<int> tt((size_t)e->n_tiles);
+    for (int t = 0; t < n_traps; ++t)
+      for (int k = 0; k < e->traps[t].ntile; ++k) tt[e->traps[t].tile0 + k] = t;
+    if (!tt.empty() && hipMemcpy(e->tile_trap, tt.data(), tt.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(AM_ERR_HIP);
+  }
   if ((rc = EnsAlloc(e, 64, &p))) return fail(rc);
   e->counter = (unsigned*)p;
   e->step = (uint64_t*)((char*)p + 8);
