@@ -95,7 +95,7 @@ SIGNATURES = {
     "am_bh_tree_stats": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
-    "am_fmm_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "am_fmm_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces_ex": (_int, [_vp, _int, _d, _int]),
     "am_lorentz_force": (_int, [_vp, _vp, _d]),
     "am_cyclotron_cooling": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
@@ -341,9 +341,10 @@ class Store:
     def fmm_stats(self):
         v = [ctypes.c_int64() for _ in range(4)]
         d = ctypes.c_int32()
-        _check(lib().am_fmm_stats(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(d)))
+        c = ctypes.c_int64()
+        _check(lib().am_fmm_stats(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(d), ctypes.byref(c)))
         return dict(sources=v[0].value, cells=v[1].value, m2l_pairs=v[2].value, p2p_cell_pairs=v[3].value,
-                    max_depth=d.value)
+                    max_depth=d.value, m2l_classes=c.value)
 
     # ---- behaviours ----
     def lorentz_force(self, field: TrapField, dt: float):

@@ -396,6 +396,17 @@ __global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, con
   }
 }
 
+// (T << 32 | S) <-> (T << cb | S) with cb bits per cell index: the list sorts on 2 cb key bits.
+__global__ void PairKeyPackKernel(int64_t n, int cb, uint64_t* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = ((keys[i] >> 32) << cb) | (keys[i] & 0xffffffffull);
+}
+
+__global__ void PairKeyUnpackKernel(int64_t n, int cb, uint64_t* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = ((keys[i] >> cb) << 32) | (keys[i] & (((uint64_t)1 << cb) - 1));
+}
+
 // CSR offsets of target cells in a (T << 32 | S)-sorted pair list.
 __global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int64_t np, int64_t* __restrict__ off) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -686,6 +697,188 @@ __global__ __launch_bounds__(64 * kM2LWaves) void M2LKernel(int nn, const Cell* 
   for (int k = 0; k < 3; ++k) {
     const int g = lane + 64 * k;
     if (g < nm) L[(size_t)p * nm + g] = acc[k];
+  }
+}
+
+// FP32 tensors of the classes (the FP32-precision path).
+__global__ void TensorToF32Kernel(int64_t n, const double* __restrict__ Dc, float* __restrict__ Dc32) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) Dc32[k] = (float)Dc[k];
+}
+
+// ---- M2L on MFMA (the FP32-precision path, orders <= 5) -------------------------------------------
+// For displacement class c the M2L is one fixed matrix: L_gamma += sum_alpha A_c[gamma][alpha] M_alpha
+// with A_c[gamma][alpha] = (-1)^|alpha| D_{alpha+gamma} / gamma!.  Pairs sorted by class run as dense
+// products A_c [M_S1 .. M_S16] on v_mfma_f32_16x16x4f32, 16 pairs per product; A_c stays in
+// registers while consecutive groups share the class.  Each pair's result goes to its slot of the
+// target-sorted list and M2LSumKernel adds a target's slots in list order, so the sum is
+// deterministic.  FP32 rounding (~1e-7 relative) is far below the order-4/5 truncation error.
+// Operand layout (16x16x4): lane l = (j, h) with j = l%16, h = l/16 holds A[j][h] and B[h][j];
+// the result D[4h + v][j], v = 0..3.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+constexpr int kM2LGroupsPerWave = 16;  // groups of 16 class-sorted pairs per wave
+
+__global__ void MultipoleToF32Kernel(int64_t n, const double* __restrict__ M, float* __restrict__ M32) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) M32[k] = (float)M[k];
+}
+
+// Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is the sort key
+// (a stable sort of a few bits), the pair index the value.
+__global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair_cls, uint32_t n_cls,
+                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  keys[q] = pair_cls[q] == kNoClass ? n_cls : pair_cls[q];
+  vals[q] = (uint32_t)q;
+}
+
+__device__ inline uint32_t WaveMinU32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+  return v;
+}
+
+template <int NRB, int NKS>
+__global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t* __restrict__ skeys,
+                                                     const uint32_t* __restrict__ svals,
+                                                     const uint64_t* __restrict__ list, uint32_t n_cls,
+                                                     const float* __restrict__ Dc32, const Cell* __restrict__ cells,
+                                                     FmmTab t, const float* __restrict__ M32,
+                                                     float* __restrict__ out) {
+  // Dynamic LDS: per wave Dd[nd] (doubles, unclassed pairs) and Df[nd] (floats); then the m2l
+  // table [nm][nm] and the recurrence table [nd][10] (ints).
+  extern __shared__ double lds[];
+  const int nm = t.nm, nd = t.nd;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slice = nd + (nd + 1) / 2 + 8 * nm + 8;  // Dd, Df, the 16 x nm result rows, their pairs
+  int* s_m2l = (int*)(lds + 4 * slice);
+  int* s_rec = s_m2l + nm * nm;
+  for (int k = threadIdx.x; k < nm * nm; k += 256) s_m2l[k] = t.m2l[k];
+  for (int k = threadIdx.x; k < nd * 10; k += 256) s_rec[k] = t.rec[k];
+  __syncthreads();  // the only workgroup barrier
+  double* Dd = lds + wave * slice;
+  float* Df = reinterpret_cast<float*>(Dd + nd);
+  float* stage = Df + 2 * ((nd + 1) / 2);          // [16][nm] results of the current product
+  int* stage_q = reinterpret_cast<int*>(stage + 16 * nm);  // [16] pair of each row, -1: not written
+  const int j = lane & 15, h = lane >> 4;
+  const int64_t n_groups = (np + 15) / 16;
+  const int64_t g0 = ((int64_t)blockIdx.x * 4 + wave) * kM2LGroupsPerWave;
+  const int64_t g1 = min(n_groups, g0 + kM2LGroupsPerWave);
+  // A_c fragments and the coefficients that scale them: A[g][al] = sign[al] / g! * D[m2l[g][al]]
+  float afrag[NRB][NKS];
+  uint32_t cur = 0xffffffffu;
+  auto build_a = [&]() {  // A_c fragments from the tensor in Df
+    WaveSync();
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        const int g = rb * 16 + j, al = ks * 4 + h;
+        afrag[rb][ks] = (g < nm && al < nm) ? (float)(t.sign[al] * t.inv_fact[g]) * Df[s_m2l[g * nm + al]] : 0.f;
+      }
+    WaveSync();
+  };
+  auto product = [&](bool sel, int S, uint32_t q) {  // A_c [M_S] for the selected columns
+    float bv[NKS];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int al = ks * 4 + h;
+      bv[ks] = (sel && al < nm) ? M32[(size_t)S * nm + al] : 0.f;
+    }
+    f32x4v acc[NRB];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv[ks], acc[rb], 0, 0, 0);
+    // through LDS, so that each pair's row of nm results is written with consecutive addresses
+    if (sel) {
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int g = rb * 16 + 4 * h + v;
+          if (g < nm) stage[j * nm + g] = acc[rb][v];
+        }
+    }
+    if (h == 0) stage_q[j] = sel ? (int)q : -1;
+    WaveSync();
+    for (int e = lane; e < 16 * nm; e += 64) {
+      const int row = e / nm, g = e - row * nm;
+      const int qr = stage_q[row];
+      if (qr >= 0) out[(size_t)qr * nm + g] = stage[e];
+    }
+    WaveSync();
+  };
+  // the group's column: sorted position r -> (class, pair q, source cell S); prefetched one ahead
+  auto column = [&](int64_t grp, uint32_t& cls, uint32_t& q, int& S, bool& valid) {
+    const int64_t r = grp * 16 + j;
+    valid = grp < g1 && r < np;
+    cls = valid ? (uint32_t)skeys[r] : 0xffffffffu;
+    q = valid ? svals[r] : 0u;
+    S = valid ? (int)(uint32_t)list[q] : 0;
+  };
+  uint32_t n_cls_, n_q;
+  int n_S;
+  bool n_valid;
+  column(g0, n_cls_, n_q, n_S, n_valid);
+  for (int64_t grp = g0; grp < g1; ++grp) {
+    const uint32_t cls = n_cls_, q = n_q;
+    const int S = n_S;
+    const bool valid = n_valid;
+    column(grp + 1, n_cls_, n_q, n_S, n_valid);  // in flight during this group
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(cls);  // column 0 is always valid here
+    if (c0 != n_cls && __ballot(valid && cls != c0) == 0) {  // one class: the common case
+      if (c0 != cur) {
+        for (int k = lane; k < nd; k += 64) Df[k] = Dc32[(size_t)c0 * nd + k];
+        cur = c0;
+        build_a();
+      }
+      product(valid, S, q);
+      continue;
+    }
+    bool done = !valid;
+    for (;;) {  // several classes (or unclassed pairs) in the group: one class at a time
+      const uint32_t c = __builtin_amdgcn_readfirstlane(WaveMinU32(done ? 0xffffffffu : cls));
+      if (c == 0xffffffffu) break;
+      bool sel = !done && cls == c;
+      if (c == n_cls) {  // unclassed: one pair at a time with its own tensor
+        const int jj = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(sel)) - 1) & 15;
+        sel = sel && j == jj;
+        const int Sj = __shfl(S, jj);
+        const uint32_t qj = (uint32_t)__shfl((int)q, jj);
+        const Cell& T = cells[(int)(list[qj] >> 32)];
+        const Cell& Sc = cells[Sj];
+        const double xs[3] = {T.cx - Sc.cx, T.cy - Sc.cy, T.cz - Sc.cz};
+        DerivTensorWave(xs, t, s_rec, Dd);
+        for (int k = lane; k < nd; k += 64) Df[k] = (float)Dd[k];
+        cur = 0xffffffffu;
+        build_a();
+      } else if (c != cur) {
+        for (int k = lane; k < nd; k += 64) Df[k] = Dc32[(size_t)c * nd + k];
+        cur = c;
+        build_a();
+      }
+      product(sel, S, q);
+      done = done || sel;
+    }
+  }
+}
+
+// L of each target: its pairs' results in list order, summed in FP64.
+__global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm,
+                                                   const float* __restrict__ out, double* __restrict__ L) {
+  const int p = blockIdx.x;
+  const int64_t b = off[p], e = off[p + 1];
+  for (int g = threadIdx.x; g < nm; g += 64) {
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight independent chains, fixed order
+    int64_t q = b;
+    for (; q + 8 <= e; q += 8)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] += (double)out[(size_t)(q + u) * nm + g];
+    for (; q < e; ++q) a[0] += (double)out[(size_t)q * nm + g];
+    L[(size_t)p * nm + g] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
 }
 
@@ -1017,8 +1210,21 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   void *v1, *v2, *off_m2l, *off_p2p;
   if ((rc = Scratch(ctx, "fmm_v1", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &v1))) return rc;
   if ((rc = Scratch(ctx, "fmm_v2", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &v2))) return rc;
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, 64))) return rc;
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, 64))) return rc;
+  // sort both lists by (target, source) on compact keys T << cb | S, then restore T << 32 | S
+  int cb = 1;
+  while ((1 << cb) < nn) ++cb;
+  if (n_m2l > 0)
+    hipLaunchKernelGGL(PairKeyPackKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, cb, (uint64_t*)m2l);
+  if (n_p2p > 0)
+    hipLaunchKernelGGL(PairKeyPackKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, cb, (uint64_t*)p2p);
+  AM_HIP(hipGetLastError());
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, 2 * cb))) return rc;
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, 2 * cb))) return rc;
+  if (n_m2l > 0)
+    hipLaunchKernelGGL(PairKeyUnpackKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, cb, (uint64_t*)m2l);
+  if (n_p2p > 0)
+    hipLaunchKernelGGL(PairKeyUnpackKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, cb, (uint64_t*)p2p);
+  AM_HIP(hipGetLastError());
   if ((rc = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return rc;
   if ((rc = Scratch(ctx, "fmm_off_p2p", (size_t)(nn + 1) * 8, &off_p2p))) return rc;
   hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)m2l, n_m2l,
@@ -1069,7 +1275,40 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                          (const Cell*)cells, (const uint64_t*)m2l, (const int64_t*)off_m2l, (const uint32_t*)pslot,
                          (const double*)Dc, tab, (const double*)M, (double*)L);
     };
-    if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
+    if (precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0) {
+      void *Dc32, *M32, *ckeys, *cvals, *outp;
+      if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
+      if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * tab.nm * 4, &M32))) return rc;
+      if ((rc = Scratch(ctx, "fmm_cls_sortkeys", (size_t)n_m2l * 8, &ckeys))) return rc;
+      if ((rc = Scratch(ctx, "fmm_cls_sortvals", (size_t)n_m2l * 4, &cvals))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * tab.nm * 4, &outp))) return rc;
+      const int64_t nt = (int64_t)n_cls * tab.nd;
+      if (nt > 0)
+        hipLaunchKernelGGL(TensorToF32Kernel, dim3(Blocks(nt, 256)), dim3(256), 0, st, nt, (const double*)Dc,
+                           (float*)Dc32);
+      hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * tab.nm, 256)), dim3(256), 0, st,
+                         (int64_t)nn * tab.nm, (const double*)M, (float*)M32);
+      hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                         (const uint32_t*)pslot, n_cls, (uint64_t*)ckeys, (uint32_t*)cvals);
+      AM_HIP(hipGetLastError());
+      int cls_bits = 1;
+      while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
+      if ((rc = RadixSortPairs(ctx, (uint64_t*)ckeys, (uint32_t*)cvals, n_m2l, cls_bits))) return rc;
+      const size_t lds_mf = sizeof(double) * ((size_t)4 * (tab.nd + (tab.nd + 1) / 2 + 8 * tab.nm + 8)) +
+                            sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10);
+      const int64_t n_groups = (n_m2l + 15) / 16;
+      const int64_t waves = (n_groups + kM2LGroupsPerWave - 1) / kM2LGroupsPerWave;
+      auto mf = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(Blocks(waves, 4)), dim3(256), lds_mf, st, n_m2l, (const uint64_t*)ckeys,
+                           (const uint32_t*)cvals, (const uint64_t*)m2l, n_cls, (const float*)Dc32, (const Cell*)cells, tab,
+                           (const float*)M32, (float*)outp);
+      };
+      if (tab.nm <= 36) mf(M2LMfmaKernel<3, 9>);
+      else mf(M2LMfmaKernel<4, 14>);
+      hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm,
+                         (const float*)outp, (double*)L);
+      AM_HIP(hipGetLastError());
+    } else if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
     else if (dpl <= 3) launch(M2LKernel<3, false>);
     else if (dpl <= 5) launch(M2LKernel<5, false>);
     else if (dpl <= 8) launch(M2LKernel<8, false>);
@@ -1173,13 +1412,14 @@ int am_compute_fast_multipole_coulomb_forces_ex(am_store* s, int order, double t
 
 // FMM statistics of the most recent run: sources, cells, M2L cell pairs, P2P cell pairs, depth.
 int am_fmm_stats(am_store* s, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs, int64_t* p2p_cell_pairs,
-                 int32_t* max_depth) {
+                 int32_t* max_depth, int64_t* m2l_classes) {
   if (!s) return Fail(AM_ERR_INVALID, "am_fmm_stats: NULL store");
   if (n_sources) *n_sources = g_fmm_stats.m;
   if (n_cells) *n_cells = g_fmm_stats.cells;
   if (m2l_pairs) *m2l_pairs = g_fmm_stats.m2l_pairs;
   if (p2p_cell_pairs) *p2p_cell_pairs = g_fmm_stats.p2p_cell_pairs;
   if (max_depth) *max_depth = g_fmm_stats.max_depth;
+  if (m2l_classes) *m2l_classes = g_fmm_stats.m2l_classes;
   return AM_OK;
 }
 

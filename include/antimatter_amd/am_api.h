@@ -163,9 +163,10 @@ int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double 
  * coordinates with FP64 per-cell accumulation (expansions stay FP64). */
 int am_compute_fast_multipole_coulomb_forces_ex(am_store* store, int order, double theta, int precision);
 /* Statistics of the most recent FMM call (the sizes FastMultipoleSolver::Compute builds,
- * fast-multipole.cc:397-526): sources, cells, M2L cell pairs, P2P cell pairs, deepest level. */
+ * fast-multipole.cc:397-526): sources, cells, M2L cell pairs, P2P cell pairs, deepest level, and the
+ * number of distinct M2L displacement classes (any pointer may be NULL). */
 int am_fmm_stats(am_store* store, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs,
-                 int64_t* p2p_cell_pairs, int32_t* max_depth);
+                 int64_t* p2p_cell_pairs, int32_t* max_depth, int64_t* m2l_classes);
 
 /* ---- behaviours ---------------------------------------------------------------------------- */
 /* LorentzForce::Run for every alive agent (src/forces/lorentz.cc:37-71): F = qE + F_coul,

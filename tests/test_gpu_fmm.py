@@ -83,6 +83,6 @@ def test_fmm_fp32_near_field(store):
     store.compute_fast_multipole_coulomb_forces(4, 0.5, am.FP32)
     f32 = store.forces()
     ref = oracle.fmm_forces(pos, q, None, 4, 0.5)
-    assert rel_l2(f32, ref) < 1e-6  # FP32 near-field arithmetic on top of the method
+    assert rel_l2(f32, ref) < 1e-6  # FP32 near field and M2L arithmetic on top of the method
     store.compute_fast_multipole_coulomb_forces(4, 0.0, am.FP32)
     assert rel_l2(store.forces(), oracle.allpairs_forces(pos, q)) < 1e-5
