@@ -1303,8 +1303,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                            (const uint32_t*)cvals, (const uint64_t*)m2l, n_cls, (const float*)Dc32, (const Cell*)cells, tab,
                            (const float*)M32, (float*)outp);
       };
+      int mslot;
+      TimerBegin(ctx, "fmm_m2l_mfma", &mslot);
       if (tab.nm <= 36) mf(M2LMfmaKernel<3, 9>);
       else mf(M2LMfmaKernel<4, 14>);
+      TimerEnd(ctx, mslot);
       hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm,
                          (const float*)outp, (double*)L);
       AM_HIP(hipGetLastError());
