@@ -30,7 +30,6 @@ namespace am {
 
 constexpr int kFmmMaxOrder = 8;
 constexpr int kFmmMaxNm = (kFmmMaxOrder + 1) * (kFmmMaxOrder + 2) * (kFmmMaxOrder + 3) / 6;  // 165
-constexpr int kFmmMaxNd = (2 * kFmmMaxOrder + 1) * (2 * kFmmMaxOrder + 2) * (2 * kFmmMaxOrder + 3) / 6;  // 969
 
 // Device view of the expansion tables (FmmTables, fast-multipole.cc:41-178).
 struct FmmTab {
@@ -909,40 +908,18 @@ __global__ __launch_bounds__(64) void L2LKernel(int n0, NodeArrays nd, const uin
   }
 }
 
-// L2P + P2P (fast-multipole.cc:580-661) and unscaling (:663-674): one wave per leaf; lane l owns
-// particle b + (l & 31) of the current 32-particle chunk and the sources of parity l >> 5.
-// Near-field sources of every target leaf as one contiguous index range: pair q (sorted by target)
-// contributes its source cell's particles at soff[q].
-__global__ void P2PCountKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
-                               uint32_t* __restrict__ cnt) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  const Cell& S = cells[(uint32_t)pairs[q]];
-  cnt[q] = (uint32_t)(S.e - S.b);
-}
-
-__global__ void P2PIndexKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
-                               const uint32_t* __restrict__ soff, uint32_t* __restrict__ sidx) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  const Cell& S = cells[(uint32_t)pairs[q]];
-  const uint32_t o = soff[q];
-  for (int j = S.b; j < S.e; ++j) sidx[o + (uint32_t)(j - S.b)] = (uint32_t)j;
-}
-
+// FP64 L2P + P2P (fast-multipole.cc:580-661) and unscaling (:663-674), also the potential
+// energy: one wave per leaf; lane l owns particle b + (l & 31) of the current 32-particle chunk
+// and the sources of parity l >> 5.
 __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restrict__ cells,
                                                    const uint64_t* __restrict__ list,
                                                    const int64_t* __restrict__ off, FmmTab t,
                                                    const double* __restrict__ L, const double4* __restrict__ sp,
                                                    const uint32_t* __restrict__ order,
                                                    const FmmFrame* __restrict__ fr, int want_forces,
-                                                   const float4* __restrict__ sp32,
-                                                   const uint32_t* __restrict__ soff,
-                                                   const uint32_t* __restrict__ sidx,
                                                    double* __restrict__ fx, double* __restrict__ fy,
                                                    double* __restrict__ fz, double* __restrict__ phi_out) {
   __shared__ double lt[kFmmMaxNm];
-  __shared__ float4 s_hi[64], s_lo[64];  // FP32 near field: the staged chunk of sources
   const int p = blockIdx.x;
   const Cell T = cells[p];
   if (!T.leaf) return;
@@ -974,43 +951,7 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
         ez = -gz;
       }
     }
-    double fx32 = 0.0, fy32 = 0.0, fz32 = 0.0;  // FP32 near field, e/unit^2
-    if (sp32 != nullptr && want_forces) {
-      // The leaf's near sources [soff[lb], soff[le]) in chunks of 64: lane l fetches source l of the
-      // next chunk while the current one is summed; half h takes sources 32h..32h+31 of a chunk.
-      // A source equal to the target contributes exactly 0 (softened coincident pair); padding
-      // entries past the end have charge 0.
-      const int tid = threadIdx.x;
-      const float4 th = sp32[2 * (valid ? i : T.b)], tl = sp32[2 * (valid ? i : T.b) + 1];
-      const uint32_t a = soff[lb], z = soff[le];
-      float4 nh = make_float4(0.f, 0.f, 0.f, 0.f), nl = nh;
-      if (a + tid < z) {
-        const uint32_t j = sidx[a + tid];
-        nh = sp32[2 * j];
-        nl = sp32[2 * j + 1];
-      }
-      for (uint32_t k0 = a; k0 < z; k0 += 64) {
-        __syncthreads();
-        s_hi[tid] = nh;
-        s_lo[tid] = nl;
-        __syncthreads();
-        nh = make_float4(0.f, 0.f, 0.f, 0.f);
-        nl = nh;
-        if (k0 + 64 + tid < z) {
-          const uint32_t j = sidx[k0 + 64 + tid];
-          nh = sp32[2 * j];
-          nl = sp32[2 * j + 1];
-        }
-        float ax = 0.f, ay = 0.f, az = 0.f;
-#pragma unroll 8
-        for (int u = 0; u < 32; ++u) PairF32(th, tl, s_hi[32 * half + u], s_lo[32 * half + u], ax, ay, az);
-        if (valid) {
-          fx32 += ax;
-          fy32 += ay;
-          fz32 += az;
-        }
-      }
-    } else if (valid) {
+    if (valid) {
       for (int64_t q = lb; q < le; ++q) {
         const Cell& S = cells[(int)(uint32_t)list[q]];
         for (int j = S.b + half; j < S.e; j += 2) {
@@ -1036,22 +977,286 @@ __global__ __launch_bounds__(64) void L2PP2PKernel(int nn, const Cell* __restric
     ey += __shfl_down(ey, 32);
     ez += __shfl_down(ez, 32);
     pot += __shfl_down(pot, 32);
-    fx32 += __shfl_down(fx32, 32);
-    fy32 += __shfl_down(fy32, 32);
-    fz32 += __shfl_down(fz32, 32);
     if (valid && half == 0) {
       const uint32_t o = order[i];
       if (want_forces) {
         const double s = fr->force_scale * si.w;
-        const double s32 = kCoulomb * si.w * kElementaryCharge * kInvUnit2;
-        fx[o] = s * ex + s32 * fx32;
-        fy[o] = s * ey + s32 * fy32;
-        fz[o] = s * ez + s32 * fz32;
+        fx[o] = s * ex;
+        fy[o] = s * ey;
+        fz[o] = s * ez;
       } else {
         phi_out[i] = si.w * pot;  // q_i phi_i in tree order
       }
     }
   }
+}
+
+// L2P gradient for order P (fast-multipole.cc:663-674): the local expansion is
+// phi(x) = sum_g L_g x^g over the multi-indices of total degree <= P in the table order (degree
+// ascending, then a and b descending), and grad phi is summed over g in that order with the same
+// products as the runtime-order loop above; unrolled, with the coefficients read as wave-uniform
+// scalars.
+template <int P>
+__device__ __forceinline__ void L2PGrad(const double* __restrict__ L, double x, double y, double z, double& gx,
+                                        double& gy, double& gz) {
+  double px[P + 1], py[P + 1], pz[P + 1];
+  px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k <= P; ++k) {
+    px[k] = px[k - 1] * x;
+    py[k] = py[k - 1] * y;
+    pz[k] = pz[k - 1] * z;
+  }
+  gx = gy = gz = 0.0;
+  int g = 0;
+#pragma unroll
+  for (int deg = 0; deg <= P; ++deg)
+#pragma unroll
+    for (int a = deg; a >= 0; --a)
+#pragma unroll
+      for (int b = deg - a; b >= 0; --b) {
+        const int c = deg - a - b;
+        const double l = L[g];
+        if (a > 0) gx += (double)a * l * (px[a - 1] * py[b] * pz[c]);
+        if (b > 0) gy += (double)b * l * (px[a] * py[b - 1] * pz[c]);
+        if (c > 0) gz += (double)c * l * (px[a] * py[b] * pz[c - 1]);
+        ++g;
+      }
+}
+
+// L2P forces for the FP32 path (fast-multipole.cc:663-674): one wave per leaf, kP2PWaves leaves
+// per workgroup, lane = particle of the leaf (at most 32).  Writes F = -q grad(phi_local) in
+// newtons; P2PF32Kernel then adds the near field.
+constexpr int kP2PWaves = 4;
+
+template <int P>
+__global__ __launch_bounds__(64 * kP2PWaves) void L2PForceKernel(int nn, const Cell* __restrict__ cells, int nm,
+                                                                 const double* __restrict__ L,
+                                                                 const double4* __restrict__ sp,
+                                                                 const uint32_t* __restrict__ order,
+                                                                 const FmmFrame* __restrict__ fr,
+                                                                 double* __restrict__ fx, double* __restrict__ fy,
+                                                                 double* __restrict__ fz) {
+  const int p = blockIdx.x * kP2PWaves + (threadIdx.x >> 6);
+  if (p >= nn) return;
+  const Cell& T = cells[p];
+  if (!T.leaf) return;
+  for (int i = T.b + (threadIdx.x & 63); i < T.e; i += 64) {
+    const double4 si = sp[i];
+    double gx, gy, gz;
+    L2PGrad<P>(L + (size_t)p * nm, si.x - T.cx, si.y - T.cy, si.z - T.cz, gx, gy, gz);
+    const double s = fr->force_scale * si.w;
+    const uint32_t o = order[i];
+    fx[o] = s * -gx;
+    fy[o] = s * -gy;
+    fz[o] = s * -gz;
+  }
+}
+
+// FP32 near field (the P2P of fast-multipole.cc:663-674 in the hi/lo pair form), added to the
+// L2P forces.  The work is cut into items so that no leaf with a long near list holds one wave
+// for the whole kernel: item = (target leaf p, batch of <= 32 of its targets, a run of segments
+// of <= kP2PSeg cells of its near list), at most kP2PItemsPerBatch items per batch (so the
+// partials stay bounded when the lists are very long).  P2PItemKernel sums one item per wave into
+// a partial per target; P2PReduceKernel adds the partials of a target in list order, so the
+// result does not depend on scheduling.
+//   * A segment's source cells are contiguous ranges of the tree-ordered particles.  They are
+//     staged in LDS as (first particle, exclusive prefix count), so the segment reads as one
+//     flattened source sequence: position k maps to a particle by a binary search in the prefix
+//     counts, and 64 consecutive positions are mostly consecutive particles.
+//   * A batch of nt targets takes nt_pad = the next power of two >= max(nt, 4) lanes; the
+//     64 / nt_pad groups of such lanes split every 64-source chunk (group s takes sources s,
+//     s + 64 / nt_pad, ...).  A chunk is staged in the wave's LDS slice while the next one is
+//     fetched.
+//   * Per chunk the pair sum is FP32 and is added into FP64; the lane groups are folded by a fixed
+//     xor tree.  A source equal to the target contributes exactly 0 (softened coincident pair),
+//     and so do the charge-0 padding entries.
+constexpr int kP2PSeg = 64;
+constexpr int kP2PItemsPerBatch = 16;
+
+// Segments of a near list of np cells, and segments per item.
+__host__ __device__ inline int P2PSegments(int64_t np) { return (int)((np + kP2PSeg - 1) / kP2PSeg); }
+__host__ __device__ inline int P2PSegsPerItem(int nseg) {
+  return (nseg + kP2PItemsPerBatch - 1) / kP2PItemsPerBatch;
+}
+
+// One staged chunk against a batch of NPAD target lanes: group grp takes sources grp + r * 64 / NPAD
+// (a fixed trip count, unrolled, so the LDS reads go out ahead of the arithmetic).
+template <int NPAD>
+__device__ __forceinline__ void P2PChunk(const float4* __restrict__ hi, const float4* __restrict__ lo, int grp,
+                                         const float4 th, const float4 tl, float& px, float& py, float& pz) {
+  constexpr int kGroups = 64 / NPAD;
+#pragma unroll 4
+  for (int r = 0; r < NPAD; ++r) PairF32(th, tl, hi[grp + r * kGroups], lo[grp + r * kGroups], px, py, pz);
+}
+
+struct P2PItem {
+  uint32_t leaf;
+  uint16_t batch, part;  // segments [part * G, (part + 1) * G) of the list, G = P2PSegsPerItem
+};
+
+// Items per cell: (target batches) x (items per batch); 0 for internal cells.
+__global__ void P2PItemCountKernel(int nn, const Cell* __restrict__ cells, const int64_t* __restrict__ off,
+                                   uint32_t* __restrict__ cnt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nn) return;
+  const Cell& T = cells[p];
+  const int nseg = P2PSegments(off[p + 1] - off[p]);
+  const int g = P2PSegsPerItem(nseg);
+  cnt[p] = T.leaf ? (uint32_t)((T.e - T.b + 31) / 32) * (uint32_t)((nseg + g - 1) / g) : 0u;
+}
+
+// Items of cell p at base[p]: batch-major, then in list order.
+__global__ void P2PItemListKernel(int nn, const Cell* __restrict__ cells, const int64_t* __restrict__ off,
+                                  const uint32_t* __restrict__ base, const uint32_t* __restrict__ cnt,
+                                  P2PItem* __restrict__ items) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nn || cnt[p] == 0) return;
+  const int nseg = P2PSegments(off[p + 1] - off[p]);
+  const int g = P2PSegsPerItem(nseg), ni = (nseg + g - 1) / g;
+  const int nb = (int)(cnt[p] / (uint32_t)ni);
+  P2PItem* out = items + base[p];
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < ni; ++k) out[b * ni + k] = P2PItem{(uint32_t)p, (uint16_t)b, (uint16_t)k};
+}
+
+// part[item][slot] = (sum_x, sum_y, sum_z) over the item's sources, e/unit^2, slot < batch size.
+__global__ __launch_bounds__(64 * kP2PWaves) void P2PItemKernel(
+    uint32_t n_items, const P2PItem* __restrict__ items, const Cell* __restrict__ cells,
+    const uint64_t* __restrict__ list, const int64_t* __restrict__ off, const float4* __restrict__ sp32,
+    double* __restrict__ part) {
+  __shared__ uint32_t s_first[kP2PWaves][kP2PSeg], s_pre[kP2PWaves][kP2PSeg + 1];
+  __shared__ float4 s_hi[kP2PWaves][64], s_lo[kP2PWaves][64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint32_t it = blockIdx.x * kP2PWaves + w;
+  if (it >= n_items) return;  // the wave's own exit: no workgroup barrier below
+  const P2PItem item = items[it];
+  const Cell& T = cells[item.leaf];
+  const int tb = T.b + 32 * item.batch;
+  const int nt = T.e - tb < 32 ? T.e - tb : 32;
+  const int sh = nt <= 4 ? 2 : (nt <= 8 ? 3 : (nt <= 16 ? 4 : 5));  // log2 nt_pad
+  const int slot = l & ((1 << sh) - 1), grp = l >> sh;
+  const bool valid = slot < nt;
+  const int i = tb + (valid ? slot : 0);
+  const float4 th = sp32[2 * i], tl = sp32[2 * i + 1];
+  const int64_t lb = off[item.leaf], le = off[item.leaf + 1];
+  const int gper = P2PSegsPerItem(P2PSegments(le - lb));
+  const int64_t s0 = lb + (int64_t)kP2PSeg * gper * item.part;
+  const int64_t s1 = s0 + (int64_t)kP2PSeg * gper < le ? s0 + (int64_t)kP2PSeg * gper : le;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int64_t seg = s0; seg < s1; seg += kP2PSeg) {
+    const int ns = (int)((s1 - seg) < kP2PSeg ? (s1 - seg) : kP2PSeg);
+    uint32_t first = 0, cnt = 0;
+    if (l < ns) {
+      const Cell& S = cells[(uint32_t)list[seg + l]];
+      first = (uint32_t)S.b;
+      cnt = (uint32_t)(S.e - S.b);
+    }
+    uint32_t incl = cnt;  // inclusive scan over the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d);
+      if (l >= d) incl += y;
+    }
+    const uint32_t run = __shfl(incl, 63);
+    WaveSync();  // the previous segment's tables have been read
+    s_first[w][l] = first;
+    s_pre[w][l] = incl - cnt;  // entries past the segment hold run
+    if (l == 0) s_pre[w][kP2PSeg] = run;
+    WaveSync();
+    // particle at flattened position k < run: the last q with s_pre[q] <= k
+    auto fetch = [&](uint32_t k, float4& h, float4& lo) {
+      h = make_float4(0.f, 0.f, 0.f, 0.f);
+      lo = h;
+      if (k < run) {
+        int a = 0;
+#pragma unroll
+        for (int step = kP2PSeg / 2; step > 0; step >>= 1)
+          if (s_pre[w][a + step] <= k) a += step;
+        const uint32_t j = s_first[w][a] + (k - s_pre[w][a]);
+        h = sp32[2 * j];
+        lo = sp32[2 * j + 1];
+      }
+    };
+    float4 nh, nl;
+    fetch((uint32_t)l, nh, nl);
+    for (uint32_t k0 = 0; k0 < run; k0 += 64) {
+      WaveSync();  // the previous chunk has been read
+      s_hi[w][l] = nh;
+      s_lo[w][l] = nl;
+      WaveSync();
+      fetch(k0 + 64 + (uint32_t)l, nh, nl);
+      float px = 0.f, py = 0.f, pz = 0.f;
+      switch (sh) {
+        case 2: P2PChunk<4>(s_hi[w], s_lo[w], grp, th, tl, px, py, pz); break;
+        case 3: P2PChunk<8>(s_hi[w], s_lo[w], grp, th, tl, px, py, pz); break;
+        case 4: P2PChunk<16>(s_hi[w], s_lo[w], grp, th, tl, px, py, pz); break;
+        default: P2PChunk<32>(s_hi[w], s_lo[w], grp, th, tl, px, py, pz); break;
+      }
+      ax += (double)px;
+      ay += (double)py;
+      az += (double)pz;
+    }
+  }
+  for (int d = 1 << sh; d < 64; d <<= 1) {
+    ax += __shfl_xor(ax, d);
+    ay += __shfl_xor(ay, d);
+    az += __shfl_xor(az, d);
+  }
+  if (grp == 0 && valid) {
+    double* o = part + ((size_t)it * 32 + slot) * 3;
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
+  }
+}
+
+// F_i += k_C q_i e 2^40 * (sum over the item partials of i's batch, in list order).  One thread
+// per target slot of every leaf batch.
+__global__ __launch_bounds__(64 * kP2PWaves) void P2PReduceKernel(
+    int nn, const Cell* __restrict__ cells, const int64_t* __restrict__ off, const uint32_t* __restrict__ base,
+    const uint32_t* __restrict__ cnt, const double* __restrict__ part, const double4* __restrict__ sp,
+    const uint32_t* __restrict__ order, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  const int p = blockIdx.x * kP2PWaves + (threadIdx.x >> 6);
+  if (p >= nn || cnt[p] == 0) return;
+  const Cell& T = cells[p];
+  const int nseg = P2PSegments(off[p + 1] - off[p]);
+  const int g = P2PSegsPerItem(nseg), ni = (nseg + g - 1) / g;
+  for (int i = T.b + (threadIdx.x & 63); i < T.e; i += 64) {
+    const int b = (i - T.b) / 32, slot = (i - T.b) % 32;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < ni; ++k) {
+      const double* o = part + (((size_t)base[p] + (size_t)b * ni + k) * 32 + slot) * 3;
+      sx += o[0];
+      sy += o[1];
+      sz += o[2];
+    }
+    const double s32 = kCoulomb * sp[i].w * kElementaryCharge * kInvUnit2;
+    const uint32_t o = order[i];
+    fx[o] += s32 * sx;
+    fy[o] += s32 * sy;
+    fz[o] += s32 * sz;
+  }
+}
+
+static void LaunchL2PForce(int order, hipStream_t st, int nn, const Cell* cells, int nm, const double* L,
+                           const double4* sp, const uint32_t* ord, const FmmFrame* fr, double* fx, double* fy,
+                           double* fz) {
+  const dim3 grid((nn + kP2PWaves - 1) / kP2PWaves), block(64 * kP2PWaves);
+#define AM_L2P_CASE(P)                                                                                          \
+  case P:                                                                                                       \
+    hipLaunchKernelGGL(L2PForceKernel<P>, grid, block, 0, st, nn, cells, nm, L, sp, ord, fr, fx, fy, fz); \
+    break;
+  switch (order) {
+    AM_L2P_CASE(1)
+    AM_L2P_CASE(2)
+    AM_L2P_CASE(3)
+    AM_L2P_CASE(4)
+    AM_L2P_CASE(5)
+    AM_L2P_CASE(6)
+    AM_L2P_CASE(7)
+    AM_L2P_CASE(8)
+  }
+#undef AM_L2P_CASE
 }
 
 __global__ void ZeroNeutralForceKernel(int64_t n, const uint8_t* __restrict__ flags, const double* __restrict__ q,
@@ -1340,30 +1545,38 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = Scratch(ctx, "fmm_phi", (size_t)m * 8, &phi))) return rc;
   }
   TimerBegin(ctx, "fmm_l2p_p2p", &slot);
-  void *soff = nullptr, *sidx = nullptr;
   if (sp32) {
-    if (n_p2p >= ((int64_t)1 << 32) / 64) return Fail(AM_ERR_CAPACITY, "FMM: too many near-field cell pairs");
-    void* cnt;
-    if ((rc = Scratch(ctx, "fmm_p2p_cnt", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &cnt))) return rc;
-    if ((rc = Scratch(ctx, "fmm_p2p_soff", (size_t)(n_p2p + 1) * 4, &soff))) return rc;
-    if (n_p2p > 0)
-      hipLaunchKernelGGL(P2PCountKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (const uint64_t*)p2p,
-                         (const Cell*)cells, (uint32_t*)cnt);
+    void *icnt, *ibase, *items, *part;
+    if ((rc = Scratch(ctx, "fmm_p2p_icnt", (size_t)nn * 4, &icnt))) return rc;
+    if ((rc = Scratch(ctx, "fmm_p2p_ibase", (size_t)(nn + 1) * 4, &ibase))) return rc;
+    hipLaunchKernelGGL(P2PItemCountKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, (const Cell*)cells,
+                       (const int64_t*)off_p2p, (uint32_t*)icnt);
     AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)soff, n_p2p))) return rc;
-    uint32_t total = 0;
-    AM_HIP(hipMemcpyAsync(&total, (uint32_t*)soff + n_p2p, 4, hipMemcpyDeviceToHost, st));
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)icnt, (uint32_t*)ibase, nn))) return rc;
+    uint32_t n_items = 0;
+    AM_HIP(hipMemcpyAsync(&n_items, (uint32_t*)ibase + nn, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
-    if ((rc = Scratch(ctx, "fmm_p2p_sidx", (size_t)std::max<uint32_t>(total, 1) * 4, &sidx))) return rc;
-    if (n_p2p > 0)
-      hipLaunchKernelGGL(P2PIndexKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (const uint64_t*)p2p,
-                         (const Cell*)cells, (const uint32_t*)soff, (uint32_t*)sidx);
+    if ((rc = Scratch(ctx, "fmm_p2p_items", (size_t)std::max<uint32_t>(n_items, 1) * sizeof(P2PItem), &items)))
+      return rc;
+    if ((rc = Scratch(ctx, "fmm_p2p_part", (size_t)std::max<uint32_t>(n_items, 1) * 32 * 3 * 8, &part))) return rc;
+    hipLaunchKernelGGL(P2PItemListKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, (const Cell*)cells,
+                       (const int64_t*)off_p2p, (const uint32_t*)ibase, (const uint32_t*)icnt, (P2PItem*)items);
     AM_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(L2PP2PKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)p2p,
-                     (const int64_t*)off_p2p, tab, (const double*)L, (const double4*)sp, (const uint32_t*)o.order,
-                     (const FmmFrame*)frame, want_forces ? 1 : 0, (const float4*)sp32, (const uint32_t*)soff,
-                     (const uint32_t*)sidx, s->fx, s->fy, s->fz, (double*)phi);
+    LaunchL2PForce(order, st, nn, (const Cell*)cells, tab.nm, (const double*)L, (const double4*)sp,
+                   (const uint32_t*)o.order, (const FmmFrame*)frame, s->fx, s->fy, s->fz);
+    AM_HIP(hipGetLastError());
+    if (n_items > 0)
+      hipLaunchKernelGGL(P2PItemKernel, dim3((n_items + kP2PWaves - 1) / kP2PWaves), dim3(64 * kP2PWaves), 0, st,
+                         n_items, (const P2PItem*)items, (const Cell*)cells, (const uint64_t*)p2p,
+                         (const int64_t*)off_p2p, (const float4*)sp32, (double*)part);
+    AM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(P2PReduceKernel, dim3((nn + kP2PWaves - 1) / kP2PWaves), dim3(64 * kP2PWaves), 0, st, nn,
+                       (const Cell*)cells, (const int64_t*)off_p2p, (const uint32_t*)ibase, (const uint32_t*)icnt,
+                       (const double*)part, (const double4*)sp, (const uint32_t*)o.order, s->fx, s->fy, s->fz);
+  } else
+    hipLaunchKernelGGL(L2PP2PKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const uint64_t*)p2p,
+                       (const int64_t*)off_p2p, tab, (const double*)L, (const double4*)sp, (const uint32_t*)o.order,
+                       (const FmmFrame*)frame, want_forces ? 1 : 0, s->fx, s->fy, s->fz, (double*)phi);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   g_fmm_stats.cells = nn;

@@ -86,3 +86,33 @@ def test_fmm_fp32_near_field(store):
     assert rel_l2(f32, ref) < 1e-6  # FP32 near field and M2L arithmetic on top of the method
     store.compute_fast_multipole_coulomb_forces(4, 0.0, am.FP32)
     assert rel_l2(store.forces(), oracle.allpairs_forces(pos, q)) < 1e-5
+
+
+@pytest.mark.parametrize("order,theta", [(1, 0.5), (3, 0.6), (6, 0.4), (8, 0.5)])
+def test_fmm_fp32_orders_with_dead_and_neutral(store, order, theta):
+    """The FP32 near field for every order its kernel is built for, with annihilated and neutral
+    particles (sources of charge 0 in the near lists) and leaves of every occupancy class."""
+    pos, sp, ann = cloud(7000, seed=40 + order)
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    store.compute_fast_multipole_coulomb_forces(order, theta, am.FP32)
+    f = store.forces()
+    m, q = oracle.mass_charge(sp)
+    ref = oracle.fmm_forces(pos, q, ann, order, theta)
+    live = ann == 0
+    assert rel_l2(f[live], ref[live]) < 1e-6
+    assert np.all(f[(sp == 2) & live] == 0)
+    assert np.all(f[~live] == 0)
+
+
+def test_fmm_fp32_overfull_leaf(store):
+    """45 coincident particles fill one deepest-level leaf past 32: the FP32 near field takes its
+    targets in batches of 32 (coincident pairs contribute nothing, as in the reference)."""
+    pos, sp, ann = cloud(3000, seed=47, frac_neutral=0.0, frac_dead=0.0)
+    pos = np.concatenate([pos, np.tile(np.array([[2e-4, -1e-4, 5e-4]]), (45, 1))])
+    sp = np.concatenate([sp, np.where(np.arange(45) % 2 == 0, 1, 0).astype(np.uint8)])
+    store.upload(pos, np.zeros_like(pos), sp)
+    m, q = oracle.mass_charge(sp)
+    store.compute_fast_multipole_coulomb_forces(4, 0.5, am.FP32)
+    ref = oracle.fmm_forces(pos, q, None, 4, 0.5)
+    assert rel_l2(store.forces(), ref) < 1e-6
+    np.testing.assert_allclose(store.forces()[-45:], ref[-45:], rtol=1e-5, atol=1e-6 * np.abs(ref).max())
