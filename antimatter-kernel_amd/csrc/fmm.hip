@@ -342,11 +342,14 @@ __global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uin
   }
 }
 
-// One expansion step of the dual-tree traversal (fast-multipole.cc:494-525).  kind[i]: 0 split,
-// 1 M2L, 2 P2P; cnt[i] = number of child pairs produced.
+// One expansion step of the dual-tree traversal (fast-multipole.cc:494-525).  Pair i of the
+// frontier is accepted for M2L, kept for P2P, or split; cnt[i], cnt[np + i] and cnt[2 np + i] count
+// its M2L pair, its P2P pair and its child pairs, so one scan over cnt places all three outputs.
 __global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
-                                    double theta2, uint32_t* __restrict__ n_m2l, uint32_t* __restrict__ n_p2p,
-                                    uint32_t* __restrict__ n_new) {
+                                    double theta2, uint32_t* __restrict__ cnt) {
+  uint32_t* n_m2l = cnt;
+  uint32_t* n_p2p = cnt + np;
+  uint32_t* n_new = cnt + 2 * (size_t)np;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint2 pr = pairs[i];
@@ -367,13 +370,16 @@ __global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, con
   n_new[i] = c;
 }
 
+// off = exclusive scan of TraverseCountKernel's cnt (3 np + 1 entries).
 __global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
-                                    double theta2, const uint32_t* __restrict__ o_m2l,
-                                    const uint32_t* __restrict__ o_p2p, const uint32_t* __restrict__ o_new,
-                                    uint64_t* __restrict__ m2l_out, uint64_t* __restrict__ p2p_out,
-                                    uint2* __restrict__ next) {
+                                    double theta2, const uint32_t* __restrict__ off, uint64_t* __restrict__ m2l_out,
+                                    uint64_t* __restrict__ p2p_out, uint2* __restrict__ next) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  const uint32_t* o_m2l = off;
+  const uint32_t b2 = off[np], b3 = off[2 * (size_t)np];
+  const uint32_t* o_p2p = off + np;
+  const uint32_t* o_new = off + 2 * (size_t)np;
   const uint2 pr = pairs[i];
   const Cell& T = cells[pr.x];
   const Cell& S = cells[pr.y];
@@ -384,10 +390,10 @@ __global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, con
   if (rs * rs < theta2 * dist2) {
     m2l_out[o_m2l[i]] = key;
   } else if (T.leaf && S.leaf) {
-    p2p_out[o_p2p[i]] = key;
+    p2p_out[o_p2p[i] - b2] = key;
   } else {
     const bool split_t = S.leaf || (!T.leaf && T.radius >= S.radius);
-    uint2* out = next + o_new[i];
+    uint2* out = next + (o_new[i] - b3);
     if (split_t)
       for (int k = 0; k < T.nchild; ++k) out[k] = make_uint2((uint32_t)T.child[k], pr.y);
     else
@@ -395,18 +401,31 @@ __global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, con
   }
 }
 
-// (T << 32 | S) <-> (T << cb | S) with cb bits per cell index: the list sorts on 2 cb key bits.
-__global__ void PairKeyPackKernel(int64_t n, int cb, uint64_t* __restrict__ keys) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) keys[i] = ((keys[i] >> 32) << cb) | (keys[i] & 0xffffffffull);
+// The three output counts of a traversal step: tot = (M2L pairs, P2P pairs, child pairs).
+__global__ void TraverseTotalsKernel(int np, const uint32_t* __restrict__ off, uint32_t* __restrict__ tot) {
+  if (threadIdx.x == 0) {
+    tot[0] = off[np];
+    tot[1] = off[2 * (size_t)np] - off[np];
+    tot[2] = off[3 * (size_t)np] - off[2 * (size_t)np];
+  }
 }
 
-__global__ void PairKeyUnpackKernel(int64_t n, int cb, uint64_t* __restrict__ keys) {
+// (T << 32 | S) -> key T, value S, and back: the lists are grouped by target with a stable sort on
+// the cb bits of T, so within a target the pairs keep the traversal's (deterministic) order.
+__global__ void PairSplitKernel(int64_t n, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) keys[i] = ((keys[i] >> cb) << 32) | (keys[i] & (((uint64_t)1 << cb) - 1));
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  keys[i] = k >> 32;
+  vals[i] = (uint32_t)k;
 }
 
-// CSR offsets of target cells in a (T << 32 | S)-sorted pair list.
+__global__ void PairJoinKernel(int64_t n, uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = (keys[i] << 32) | vals[i];
+}
+
+// CSR offsets of target cells in a pair list (T << 32 | S) grouped by ascending T.
 __global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int64_t np, int64_t* __restrict__ off) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > nn) return;
@@ -1359,23 +1378,19 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   int64_t np = 1;
   TimerBegin(ctx, "fmm_traverse", &slot);
   while (np > 0) {
-    void *c1, *c2, *c3, *o1, *o2, *o3;
-    if ((rc = Scratch(ctx, "fmm_c1", (size_t)np * 4, &c1))) return rc;
-    if ((rc = Scratch(ctx, "fmm_c2", (size_t)np * 4, &c2))) return rc;
-    if ((rc = Scratch(ctx, "fmm_c3", (size_t)np * 4, &c3))) return rc;
-    if ((rc = Scratch(ctx, "fmm_o1", (size_t)(np + 1) * 4, &o1))) return rc;
-    if ((rc = Scratch(ctx, "fmm_o2", (size_t)(np + 1) * 4, &o2))) return rc;
-    if ((rc = Scratch(ctx, "fmm_o3", (size_t)(np + 1) * 4, &o3))) return rc;
+    void *cnt3, *off3, *tot_dev;
+    if ((rc = Scratch(ctx, "fmm_trav_cnt", (size_t)np * 3 * 4, &cnt3))) return rc;
+    if ((rc = Scratch(ctx, "fmm_trav_off", ((size_t)np * 3 + 1) * 4, &off3))) return rc;
+    if ((rc = Scratch(ctx, "fmm_trav_tot", 16, &tot_dev))) return rc;
     hipLaunchKernelGGL(TraverseCountKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const Cell*)cells, theta2, (uint32_t*)c1, (uint32_t*)c2, (uint32_t*)c3);
+                       (const Cell*)cells, theta2, (uint32_t*)cnt3);
     AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c1, (uint32_t*)o1, np))) return rc;
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c2, (uint32_t*)o2, np))) return rc;
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)c3, (uint32_t*)o3, np))) return rc;
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt3, (uint32_t*)off3, 3 * np))) return rc;
+    hipLaunchKernelGGL(TraverseTotalsKernel, dim3(1), dim3(64), 0, st, (int)np, (const uint32_t*)off3,
+                       (uint32_t*)tot_dev);
+    AM_HIP(hipGetLastError());
     uint32_t tot[3];
-    AM_HIP(hipMemcpyAsync(&tot[0], (uint32_t*)o1 + np, 4, hipMemcpyDeviceToHost, st));
-    AM_HIP(hipMemcpyAsync(&tot[1], (uint32_t*)o2 + np, 4, hipMemcpyDeviceToHost, st));
-    AM_HIP(hipMemcpyAsync(&tot[2], (uint32_t*)o3 + np, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipMemcpyAsync(tot, tot_dev, 12, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
     // grow outputs (preserving contents)
     auto grow = [&](const char* name, void** buf, size_t& cap, size_t need, size_t used, size_t elem) -> int {
@@ -1401,8 +1416,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = grow("fmm_p2p", &p2p, cap_p2p, (size_t)n_p2p + tot[1], (size_t)n_p2p, 8))) return rc;
     if ((rc = grow("fmm_front_b", &fb, cap_fb, tot[2], 0, 8))) return rc;
     hipLaunchKernelGGL(TraverseWriteKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const Cell*)cells, theta2, (const uint32_t*)o1, (const uint32_t*)o2, (const uint32_t*)o3,
-                       (uint64_t*)m2l + n_m2l, (uint64_t*)p2p + n_p2p, (uint2*)fb);
+                       (const Cell*)cells, theta2, (const uint32_t*)off3, (uint64_t*)m2l + n_m2l,
+                       (uint64_t*)p2p + n_p2p, (uint2*)fb);
     AM_HIP(hipGetLastError());
     n_m2l += tot[0];
     n_p2p += tot[1];
@@ -1415,20 +1430,24 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   void *v1, *v2, *off_m2l, *off_p2p;
   if ((rc = Scratch(ctx, "fmm_v1", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &v1))) return rc;
   if ((rc = Scratch(ctx, "fmm_v2", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &v2))) return rc;
-  // sort both lists by (target, source) on compact keys T << cb | S, then restore T << 32 | S
+  // group both lists by target cell: stable sort on the cb bits of T, S carried as the value
   int cb = 1;
   while ((1 << cb) < nn) ++cb;
   if (n_m2l > 0)
-    hipLaunchKernelGGL(PairKeyPackKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, cb, (uint64_t*)m2l);
+    hipLaunchKernelGGL(PairSplitKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (uint64_t*)m2l,
+                       (uint32_t*)v1);
   if (n_p2p > 0)
-    hipLaunchKernelGGL(PairKeyPackKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, cb, (uint64_t*)p2p);
+    hipLaunchKernelGGL(PairSplitKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (uint64_t*)p2p,
+                       (uint32_t*)v2);
   AM_HIP(hipGetLastError());
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, 2 * cb))) return rc;
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, 2 * cb))) return rc;
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, cb))) return rc;
+  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, cb))) return rc;
   if (n_m2l > 0)
-    hipLaunchKernelGGL(PairKeyUnpackKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, cb, (uint64_t*)m2l);
+    hipLaunchKernelGGL(PairJoinKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (uint64_t*)m2l,
+                       (const uint32_t*)v1);
   if (n_p2p > 0)
-    hipLaunchKernelGGL(PairKeyUnpackKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, cb, (uint64_t*)p2p);
+    hipLaunchKernelGGL(PairJoinKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (uint64_t*)p2p,
+                       (const uint32_t*)v2);
   AM_HIP(hipGetLastError());
   if ((rc = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return rc;
   if ((rc = Scratch(ctx, "fmm_off_p2p", (size_t)(nn + 1) * 8, &off_p2p))) return rc;
