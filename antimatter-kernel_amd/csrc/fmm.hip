@@ -728,17 +728,22 @@ __global__ void TensorToF32Kernel(int64_t n, const double* __restrict__ Dc, floa
 // For displacement class c the M2L is one fixed matrix: L_gamma += sum_alpha A_c[gamma][alpha] M_alpha
 // with A_c[gamma][alpha] = (-1)^|alpha| D_{alpha+gamma} / gamma!.  Pairs sorted by class run as dense
 // products A_c [M_S1 .. M_S16] on v_mfma_f32_16x16x4f32, 16 pairs per product; A_c stays in
-// registers while consecutive groups share the class.  Each pair's result goes to its slot of the
-// target-sorted list and M2LSumKernel adds a target's slots in list order, so the sum is
-// deterministic.  FP32 rounding (~1e-7 relative) is far below the order-4/5 truncation error.
+// registers while consecutive groups share the class.  Each pair's result row goes to its slot of
+// the target-grouped list straight from the product registers, and M2LSumKernel adds a target's
+// rows in list order, so the sum is deterministic.  FP32 rounding (~1e-7 relative) is far below the order-4/5 truncation error.
 // Operand layout (16x16x4): lane l = (j, h) with j = l%16, h = l/16 holds A[j][h] and B[h][j];
 // the result D[4h + v][j], v = 0..3.
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 constexpr int kM2LGroupsPerWave = 16;  // groups of 16 class-sorted pairs per wave
 
-__global__ void MultipoleToF32Kernel(int64_t n, const double* __restrict__ M, float* __restrict__ M32) {
+// FP32 multipoles in the B-operand order of the products: cell S's row of 4 nks floats holds
+// M_al at h * nks + ks for al = 4 ks + h (0 past nm), so lane (j, h) reads nks consecutive floats.
+__global__ void MultipoleToF32Kernel(int nn, int nm, int nks, const double* __restrict__ M, float* __restrict__ M32) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) M32[k] = (float)M[k];
+  if (k >= (int64_t)nn * 4 * nks) return;
+  const int64_t S = k / (4 * nks);
+  const int e = (int)(k - S * 4 * nks), h = e / nks, ks = e - h * nks, al = 4 * ks + h;
+  M32[k] = al < nm ? (float)M[S * nm + al] : 0.f;
 }
 
 // Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is the sort key
@@ -768,7 +773,7 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   extern __shared__ double lds[];
   const int nm = t.nm, nd = t.nd;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slice = nd + (nd + 1) / 2 + 8 * nm + 8;  // Dd, Df, the 16 x nm result rows, their pairs
+  const int slice = nd + (nd + 1) / 2;  // Dd, Df
   int* s_m2l = (int*)(lds + 4 * slice);
   int* s_rec = s_m2l + nm * nm;
   for (int k = threadIdx.x; k < nm * nm; k += 256) s_m2l[k] = t.m2l[k];
@@ -776,8 +781,6 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   __syncthreads();  // the only workgroup barrier
   double* Dd = lds + wave * slice;
   float* Df = reinterpret_cast<float*>(Dd + nd);
-  float* stage = Df + 2 * ((nd + 1) / 2);          // [16][nm] results of the current product
-  int* stage_q = reinterpret_cast<int*>(stage + 16 * nm);  // [16] pair of each row, -1: not written
   const int j = lane & 15, h = lane >> 4;
   const int64_t n_groups = (np + 15) / 16;
   const int64_t g0 = ((int64_t)blockIdx.x * 4 + wave) * kM2LGroupsPerWave;
@@ -796,13 +799,15 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
       }
     WaveSync();
   };
-  auto product = [&](bool sel, int S, uint32_t q) {  // A_c [M_S] for the selected columns
+  // A_c [M_S] for the selected columns.  Lane (j, h) ends with D[4h + v][j] of each row block: the
+  // four consecutive coefficients 16 rb + 4h .. + 3 of pair j, stored as one 16-byte write into
+  // the pair's row of nm4 = 4 ceil(nm / 4) floats.
+  const int nm4 = (nm + 3) & ~3;
+  auto product = [&](bool sel, int S, uint32_t q) {
     float bv[NKS];
+    const float* mrow = M32 + (size_t)S * (4 * NKS) + h * NKS;
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-      const int al = ks * 4 + h;
-      bv[ks] = (sel && al < nm) ? M32[(size_t)S * nm + al] : 0.f;
-    }
+    for (int ks = 0; ks < NKS; ++ks) bv[ks] = sel ? mrow[ks] : 0.f;
     f32x4v acc[NRB];
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
@@ -810,24 +815,14 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv[ks], acc[rb], 0, 0, 0);
-    // through LDS, so that each pair's row of nm results is written with consecutive addresses
     if (sel) {
+      float* orow = out + (size_t)q * nm4;
 #pragma unroll
-      for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int g = rb * 16 + 4 * h + v;
-          if (g < nm) stage[j * nm + g] = acc[rb][v];
-        }
+      for (int rb = 0; rb < NRB; ++rb) {
+        const int g = rb * 16 + 4 * h;
+        if (g < nm4) *reinterpret_cast<f32x4v*>(orow + g) = acc[rb];
+      }
     }
-    if (h == 0) stage_q[j] = sel ? (int)q : -1;
-    WaveSync();
-    for (int e = lane; e < 16 * nm; e += 64) {
-      const int row = e / nm, g = e - row * nm;
-      const int qr = stage_q[row];
-      if (qr >= 0) out[(size_t)qr * nm + g] = stage[e];
-    }
-    WaveSync();
   };
   // the group's column: sorted position r -> (class, pair q, source cell S); prefetched one ahead
   auto column = [&](int64_t grp, uint32_t& cls, uint32_t& q, int& S, bool& valid) {
@@ -885,7 +880,8 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
 }
 
 // L of each target: its pairs' results in list order, summed in FP64.
-__global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm,
+// out holds a row of nm4 floats per pair.
+__global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm, int nm4,
                                                    const float* __restrict__ out, double* __restrict__ L) {
   const int p = blockIdx.x;
   const int64_t b = off[p], e = off[p + 1];
@@ -894,8 +890,8 @@ __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __rest
     int64_t q = b;
     for (; q + 8 <= e; q += 8)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] += (double)out[(size_t)(q + u) * nm + g];
-    for (; q < e; ++q) a[0] += (double)out[(size_t)q * nm + g];
+      for (int u = 0; u < 8; ++u) a[u] += (double)out[(size_t)(q + u) * nm4 + g];
+    for (; q < e; ++q) a[0] += (double)out[(size_t)q * nm4 + g];
     L[(size_t)p * nm + g] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
 }
@@ -1502,23 +1498,25 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if (precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0) {
       void *Dc32, *M32, *ckeys, *cvals, *outp;
       if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
-      if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * tab.nm * 4, &M32))) return rc;
+      const int nks = tab.nm <= 36 ? 9 : 14;  // MFMA k-steps of the instance below
+      const int nm4 = (tab.nm + 3) & ~3;
+      if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * 4 * nks * 4, &M32))) return rc;
       if ((rc = Scratch(ctx, "fmm_cls_sortkeys", (size_t)n_m2l * 8, &ckeys))) return rc;
       if ((rc = Scratch(ctx, "fmm_cls_sortvals", (size_t)n_m2l * 4, &cvals))) return rc;
-      if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * tab.nm * 4, &outp))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * nm4 * 4, &outp))) return rc;
       const int64_t nt = (int64_t)n_cls * tab.nd;
       if (nt > 0)
         hipLaunchKernelGGL(TensorToF32Kernel, dim3(Blocks(nt, 256)), dim3(256), 0, st, nt, (const double*)Dc,
                            (float*)Dc32);
-      hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * tab.nm, 256)), dim3(256), 0, st,
-                         (int64_t)nn * tab.nm, (const double*)M, (float*)M32);
+      hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nks, 256)), dim3(256), 0, st, nn,
+                         tab.nm, nks, (const double*)M, (float*)M32);
       hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
                          (const uint32_t*)pslot, n_cls, (uint64_t*)ckeys, (uint32_t*)cvals);
       AM_HIP(hipGetLastError());
       int cls_bits = 1;
       while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
       if ((rc = RadixSortPairs(ctx, (uint64_t*)ckeys, (uint32_t*)cvals, n_m2l, cls_bits))) return rc;
-      const size_t lds_mf = sizeof(double) * ((size_t)4 * (tab.nd + (tab.nd + 1) / 2 + 8 * tab.nm + 8)) +
+      const size_t lds_mf = sizeof(double) * ((size_t)4 * (tab.nd + (tab.nd + 1) / 2)) +
                             sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10);
       const int64_t n_groups = (n_m2l + 15) / 16;
       const int64_t waves = (n_groups + kM2LGroupsPerWave - 1) / kM2LGroupsPerWave;
@@ -1532,7 +1530,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       if (tab.nm <= 36) mf(M2LMfmaKernel<3, 9>);
       else mf(M2LMfmaKernel<4, 14>);
       TimerEnd(ctx, mslot);
-      hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm,
+      hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm, nm4,
                          (const float*)outp, (double*)L);
       AM_HIP(hipGetLastError());
     } else if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
