@@ -246,9 +246,19 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
 }
 
 // ---- walk -------------------------------------------------------------------------------------
+// Minimum over the wave (every lane active), returned wave-uniform.  DPP inclusive min-scan:
+// row_shr 1, 2, 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 across rows, so lane 63
+// ends with the minimum; lanes whose source is outside the row keep the identity.  Six dependent
+// VALU steps instead of a chain of LDS permutes.
 __device__ inline int WaveMin(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-  return v;
+  constexpr int kId = 0x7fffffff;
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x142, 0xa, 0xf, false));  // row_bcast:15, rows 1 and 3
+  v = min(v, __builtin_amdgcn_update_dpp(kId, v, 0x143, 0xc, 0xf, false));  // row_bcast:31, rows 2 and 3
+  return __builtin_amdgcn_readlane(v, 63);
 }
 
 // AddPointCharge (barnes-hut.cc:19-27) without contraction.
