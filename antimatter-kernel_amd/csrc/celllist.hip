@@ -225,6 +225,26 @@ int StandardEnergyTerms(am_store* s, double rc, double* out) {
 }
 
 // ---- annihilation ------------------------------------------------------------------------------
+// The contact cell list has edge h >= 2 reach, reach = r_a (1 + 1e-6): along each axis the points
+// within reach of p lie in the cells from CellCoord(p - reach) to CellCoord(p + reach), at most
+// two (CellCoord is monotone and 1-Lipschitz in cells), so at most 8 cells are looked up instead
+// of 27.  The margin r_a 1e-6 is far above the rounding of p -+ reach.
+constexpr double kContactReach = 1.0 + 1e-6;
+
+template <typename F>
+__device__ inline void ForEachNearCell(const CellTable& tab, const uint32_t* __restrict__ idx, double px, double py,
+                                       double pz, double inv_h, double reach, F&& f) {
+  const int64_t x0 = CellCoord(px - reach, inv_h), x1 = CellCoord(px + reach, inv_h);
+  const int64_t y0 = CellCoord(py - reach, inv_h), y1 = CellCoord(py + reach, inv_h);
+  const int64_t z0 = CellCoord(pz - reach, inv_h), z1 = CellCoord(pz + reach, inv_h);
+  for (int64_t cz = z0; cz <= z1; ++cz)
+    for (int64_t cy = y0; cy <= y1; ++cy)
+      for (int64_t cx = x0; cx <= x1; ++cx) {
+        const uint2 r = CellLookup(tab, CellKey(cx, cy, cz));
+        for (uint32_t k = r.x; k < r.x + r.y; ++k) f((int64_t)idx[k]);
+      }
+}
+
 // Candidates of i: alive j != i, IsAntiparticlePair(species_i, species_j), d2 < r_a^2.
 template <typename F>
 __device__ inline void ForEachCandidate(int64_t i, int64_t n, const double* __restrict__ x,
@@ -234,7 +254,7 @@ __device__ inline void ForEachCandidate(int64_t i, int64_t n, const double* __re
                                         double ra2, F&& f) {
   const double px = x[i], py = y[i], pz = z[i];
   const int si = species[i];
-  ForEachStencil(tab, idx, px, py, pz, inv_h, [&](int64_t j) {
+  ForEachNearCell(tab, idx, px, py, pz, inv_h, sqrt(ra2) * kContactReach, [&](int64_t j) {
     if (j == i || (flags[j] & kFlagAnnihilated)) return;
     if (!IsPair(si, species[j])) return;
     const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
@@ -248,8 +268,9 @@ __global__ void ContactCountKernel(int64_t n, const double* __restrict__ x, cons
                                    const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    uint32_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
+  if (k >= n) return;
+  const int64_t i = idx[k];  // a wave's particles share cells and table lines
   uint32_t c = 0;
   if (!(flags[i] & kFlagAnnihilated))
     ForEachCandidate(i, n, x, y, z, species, flags, tab, idx, inv_h, ra2, [&](int64_t) { ++c; });
@@ -263,8 +284,9 @@ __global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, cons
                                    const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    const uint32_t* __restrict__ off, uint2* __restrict__ pairs) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
+  if (k >= n) return;
+  const int64_t i = idx[k];
   const uint32_t b = off[i], e = off[i + 1];
   if (b == e) return;
   uint32_t w = b;
@@ -388,7 +410,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
   am_ctx* ctx = s->ctx;
   const int64_t n = s->n;
   CellList cl;
-  int rc = BuildCellList(s, contact_radius, "cl_contact", &cl);
+  int rc = BuildCellList(s, 2.0 * contact_radius * kContactReach, "cl_contact", &cl);
   if (rc) return rc;
   void *cnt, *off, *marked;
   if ((rc = Scratch(ctx, "contact_count", (size_t)n * 4, &cnt))) return rc;
