@@ -55,15 +55,18 @@ void TimerBegin(am_ctx* ctx, const char* name, int* slot) {
   p.name = name;
   p.start = TakeEvent(ctx);
   p.stop = TakeEvent(ctx);
-  if (!p.start || !p.stop) return;
-  hipEventRecord(p.start, ctx->stream);
+  if (!p.start || !p.stop || hipEventRecord(p.start, ctx->stream) != hipSuccess) {  // timing is best effort
+    if (p.start) ctx->event_pool.push_back(p.start);
+    if (p.stop) ctx->event_pool.push_back(p.stop);
+    return;
+  }
   ctx->pending.push_back(p);
   *slot = (int)ctx->pending.size() - 1;
 }
 
 void TimerEnd(am_ctx* ctx, int slot) {
   if (slot < 0 || slot >= (int)ctx->pending.size()) return;
-  hipEventRecord(ctx->pending[slot].stop, ctx->stream);
+  (void)hipEventRecord(ctx->pending[slot].stop, ctx->stream);  // timing is best effort
 }
 
 static int Drain(am_ctx* ctx) {
