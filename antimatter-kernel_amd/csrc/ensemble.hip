@@ -34,7 +34,7 @@ namespace am {
 struct TrapRec {
   int64_t begin, end;
   double b0, u0, d, dt, tau_scale, bath_temp, contact_radius, probability;
-  double inv_h;  // contact cell scale
+  double inv_h;  // 1 / contact cell edge (edge >= 2 x contact radius)
   int annihilation;
   int chunks;         // source chunks of the force work items
   int64_t part_base;  // the trap's partial sums: [chunk][3][ntile * kTile]
@@ -248,6 +248,7 @@ __global__ __launch_bounds__(256) void EnsemblePushKernel(int64_t n, double* __r
 // ---- contact detection within each trap ---------------------------------------------------------
 constexpr int kECellBits = 18;
 constexpr uint64_t kEDeadKey = kCellEmpty;
+constexpr double kEnsContactReach = 1.0 + 1e-6;  // search reach / contact radius; cell edge = 2 reach
 
 __device__ inline int64_t ECoord(double v, double inv_h) {
   const double c = floor(v * inv_h);
@@ -287,13 +288,15 @@ __device__ inline void EnsForEachCandidate(int64_t i, int64_t n, const double* _
   const double px = x[i], py = y[i], pz = z[i];
   const double ra2 = tr.contact_radius * tr.contact_radius;
   const int si = species[i];
-  const int64_t cx = ECoord(px, tr.inv_h), cy = ECoord(py, tr.inv_h), cz = ECoord(pz, tr.inv_h);
-  const int64_t lim = (int64_t)1 << kECellBits;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int64_t ax = cx + dx, ay = cy + dy, az = cz + dz;
-        if (ax < 0 || ay < 0 || az < 0 || ax >= lim || ay >= lim || az >= lim) continue;
+  // cell edge >= 2 reach: along each axis at most the two cells from ECoord(p - reach) to
+  // ECoord(p + reach) can hold a partner (ECoord is monotone and 1-Lipschitz in cells)
+  const double reach = tr.contact_radius * kEnsContactReach;
+  const int64_t x0 = ECoord(px - reach, tr.inv_h), x1 = ECoord(px + reach, tr.inv_h);
+  const int64_t y0 = ECoord(py - reach, tr.inv_h), y1 = ECoord(py + reach, tr.inv_h);
+  const int64_t z0 = ECoord(pz - reach, tr.inv_h), z1 = ECoord(pz + reach, tr.inv_h);
+  for (int64_t az = z0; az <= z1; ++az)
+    for (int64_t ay = y0; ay <= y1; ++ay)
+      for (int64_t ax = x0; ax <= x1; ++ax) {
         const uint2 r = CellLookup(tab, EKey(t, ax, ay, az));
         for (uint32_t k = r.x; k < r.x + r.y; ++k) {
           const int64_t j = idx[k];
@@ -311,8 +314,9 @@ __global__ void EnsContactCountKernel(int64_t n, const double* __restrict__ x, c
                                       const uint8_t* __restrict__ flags, const uint32_t* __restrict__ trap,
                                       const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, uint32_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
+  if (k >= n) return;
+  const int64_t i = idx[k];  // a wave's particles share cells and table lines
   uint32_t c = 0;
   const uint32_t t = trap[i];
   if (t < (uint32_t)kMaxTraps && !(flags[i] & kFlagAnnihilated) && traps[t].annihilation)
@@ -326,8 +330,9 @@ __global__ void EnsContactWriteKernel(int64_t n, const double* __restrict__ x, c
                                       const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
                                       int64_t cap, uint2* __restrict__ pairs, uint32_t* __restrict__ overflow) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
+  if (k >= n) return;
+  const int64_t i = idx[k];
   const uint32_t b = off[i], e = off[i + 1];
   if (b == e) return;
   if ((int64_t)e > cap) {
@@ -633,7 +638,7 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
     r.contact_radius = d.contact_radius;
     r.probability = d.probability;
     r.annihilation = d.annihilation ? 1 : 0;
-    r.inv_h = d.annihilation ? 1.0 / (d.contact_radius * (1.0 + 1e-9)) : 0.0;
+    r.inv_h = d.annihilation ? 1.0 / (2.0 * d.contact_radius * kEnsContactReach * (1.0 + 1e-9)) : 0.0;
     const int64_t N = d.end - d.begin;
     r.tile0 = e->n_tiles;
     r.ntile = (int)(2 * ((N + 2 * kEnsTile - 1) / (2 * kEnsTile)));
