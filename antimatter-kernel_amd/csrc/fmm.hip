@@ -46,6 +46,7 @@ struct FmmTab {
   const int* l2l_off;
   const int* l2l_bc;
   const int* grad;       // [nm][3]
+  int n_m2m, n_l2l;      // entries of m2m_bc and l2l_bc (pairs)
 };
 
 struct HostTab {
@@ -163,7 +164,8 @@ static int DeviceTables(am_ctx* ctx, int p, FmmTab* out) {
   *out = FmmTab{t.P, t.nm, t.nd, (const int*)(d + o_multi), (const double*)(d + o_fact), (const double*)(d + o_ifact),
                 (const double*)(d + o_sign), (const int*)(d + o_rec), (const int*)(d + o_deg),
                 (const int*)(d + o_m2l), (const int*)(d + o_m2mo), (const int*)(d + o_m2mb),
-                (const int*)(d + o_l2lo), (const int*)(d + o_l2lb), (const int*)(d + o_grad)};
+                (const int*)(d + o_l2lo), (const int*)(d + o_l2lb), (const int*)(d + o_grad),
+                (int)(t.m2m_bc.size() / 2), (int)(t.l2l_bc.size() / 2)};
   return AM_OK;
 }
 
@@ -285,9 +287,21 @@ __device__ inline double Monomial(const FmmTab& t, int k, double x, double y, do
   return px * py * pz;
 }
 
+// Monomial x^a y^b z^c of multi-index k from a copy of the multi table (e.g. in LDS), with the
+// same products as Monomial.
+__device__ inline double MonomialAt(const int* __restrict__ multi, int k, double x, double y, double z) {
+  const int a = multi[3 * k], b = multi[3 * k + 1], c = multi[3 * k + 2];
+  double px = 1.0, py = 1.0, pz = 1.0;
+  for (int i = 0; i < a; ++i) px = px * x;
+  for (int i = 0; i < b; ++i) py = py * y;
+  for (int i = 0; i < c; ++i) pz = pz * z;
+  return px * py * pz;
+}
+
 // P2M (fast-multipole.cc:443-461): one wave per cell, lane = coefficient.
 __global__ __launch_bounds__(64) void P2MKernel(int nn, const Cell* __restrict__ cells, const double4* __restrict__ sp,
                                                 FmmTab t, double* __restrict__ M) {
+  __shared__ int s_multi[3 * kFmmMaxNm];
   const int p = blockIdx.x;
   const Cell c = cells[p];
   const int a = threadIdx.x;
@@ -295,15 +309,49 @@ __global__ __launch_bounds__(64) void P2MKernel(int nn, const Cell* __restrict__
     for (int k = a; k < t.nm; k += 64) M[(size_t)p * t.nm + k] = 0.0;
     return;
   }
+  for (int k = a; k < 3 * t.nm; k += 64) s_multi[k] = t.multi[k];
+  __syncthreads();
   for (int k = a; k < t.nm; k += 64) {
+    const double ifk = t.inv_fact[k];
     double acc = 0.0;
     for (int j = c.b; j < c.e; ++j) {
       const double4 s = sp[j];
       if (s.w == 0.0) continue;
-      acc = acc + s.w * Monomial(t, k, s.x - c.cx, s.y - c.cy, s.z - c.cz) * t.inv_fact[k];
+      acc = acc + s.w * MonomialAt(s_multi, k, s.x - c.cx, s.y - c.cy, s.z - c.cz) * ifk;
     }
     M[(size_t)p * t.nm + k] = acc;
   }
+}
+
+// Dynamic LDS of M2MKernel and L2LKernel: the tables their inner loops read, staged once per
+// workgroup (multi, fact, inv_fact of the first nm indices, the CSR offsets and the (b, c) pairs).
+inline size_t TransferLds(const FmmTab& t, bool l2l) {
+  return sizeof(double) * 2 * (size_t)t.nm +
+         sizeof(int) * (3 * (size_t)t.nm + (size_t)t.nm + 1 + 2 * (size_t)(l2l ? t.n_l2l : t.n_m2m));
+}
+struct TransferTabs {
+  double *fact, *ifact;
+  int *multi, *off, *bc;
+};
+__device__ inline TransferTabs StageTransferTabs(const FmmTab& t, bool l2l) {
+  extern __shared__ double tlds[];
+  TransferTabs r;
+  r.fact = tlds;
+  r.ifact = tlds + t.nm;
+  r.multi = reinterpret_cast<int*>(tlds + 2 * t.nm);
+  r.off = r.multi + 3 * t.nm;
+  r.bc = r.off + t.nm + 1;
+  const int* off = l2l ? t.l2l_off : t.m2m_off;
+  const int* bc = l2l ? t.l2l_bc : t.m2m_bc;
+  const int nbc = 2 * (l2l ? t.n_l2l : t.n_m2m);
+  for (int k = threadIdx.x; k < t.nm; k += blockDim.x) {
+    r.fact[k] = t.fact[k];
+    r.ifact[k] = t.inv_fact[k];
+  }
+  for (int k = threadIdx.x; k < 3 * t.nm; k += blockDim.x) r.multi[k] = t.multi[k];
+  for (int k = threadIdx.x; k <= t.nm; k += blockDim.x) r.off[k] = off[k];
+  for (int k = threadIdx.x; k < nbc; k += blockDim.x) r.bc[k] = bc[k];
+  return r;
 }
 
 // M2M (fast-multipole.cc:463-482) for the parents of one BFS level.
@@ -312,6 +360,7 @@ __global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uin
   __shared__ double mono[kFmmMaxNm];
   const int node = n0 + blockIdx.x;
   if (nd.leaf[node]) return;
+  const TransferTabs tt = StageTransferTabs(t, false);
   const int p = (int)bfs2pre[node];
   const int c0 = nd.first_child[node];
   double acc[3] = {0.0, 0.0, 0.0};  // up to 3 coefficients per lane (nm <= 165 < 192)
@@ -322,16 +371,16 @@ __global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uin
   for (int c = c0; c < c0 + nd.nchild[node]; ++c) {
     const double dx = nd.cx[c] - nd.cx[node], dy = nd.cy[c] - nd.cy[node], dz = nd.cz[c] - nd.cz[node];
     __syncthreads();
-    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = Monomial(t, k, dx, dy, dz);
+    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = MonomialAt(tt.multi, k, dx, dy, dz);
     __syncthreads();
     const double* mc = M + (size_t)bfs2pre[c] * t.nm;
     for (int k = 0; k < 3; ++k) {
       const int a = threadIdx.x + 64 * k;
       if (a >= t.nm) continue;
       double v = acc[k];
-      for (int q = t.m2m_off[a]; q < t.m2m_off[a + 1]; ++q) {
-        const int b = t.m2m_bc[2 * q], cc = t.m2m_bc[2 * q + 1];
-        v = v + mc[b] * mono[cc] * t.inv_fact[cc];
+      for (int q = tt.off[a]; q < tt.off[a + 1]; ++q) {
+        const int b = tt.bc[2 * q], cc = tt.bc[2 * q + 1];
+        v = v + mc[b] * mono[cc] * tt.ifact[cc];
       }
       acc[k] = v;
     }
@@ -903,20 +952,21 @@ __global__ __launch_bounds__(64) void L2LKernel(int n0, NodeArrays nd, const uin
   __shared__ double lp[kFmmMaxNm];
   const int node = n0 + blockIdx.x;
   if (nd.leaf[node]) return;
+  const TransferTabs tt = StageTransferTabs(t, true);
   const int p = (int)bfs2pre[node];
   for (int k = threadIdx.x; k < t.nm; k += 64) lp[k] = L[(size_t)p * t.nm + k];
   const int c0 = nd.first_child[node];
   for (int c = c0; c < c0 + nd.nchild[node]; ++c) {
     const double dx = nd.cx[c] - nd.cx[node], dy = nd.cy[c] - nd.cy[node], dz = nd.cz[c] - nd.cz[node];
     __syncthreads();
-    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = Monomial(t, k, dx, dy, dz);
+    for (int k = threadIdx.x; k < t.nm; k += 64) mono[k] = MonomialAt(tt.multi, k, dx, dy, dz);
     __syncthreads();
     double* lc = L + (size_t)bfs2pre[c] * t.nm;
     for (int a = threadIdx.x; a < t.nm; a += 64) {
       double v = lc[a];
-      for (int q = t.l2l_off[a]; q < t.l2l_off[a + 1]; ++q) {
-        const int bb = t.l2l_bc[2 * q], cc = t.l2l_bc[2 * q + 1];
-        v += lp[bb] * t.fact[bb] * t.inv_fact[a] * t.inv_fact[cc] * mono[cc];
+      for (int q = tt.off[a]; q < tt.off[a + 1]; ++q) {
+        const int bb = tt.bc[2 * q], cc = tt.bc[2 * q + 1];
+        v += lp[bb] * tt.fact[bb] * tt.ifact[a] * tt.ifact[cc] * mono[cc];
       }
       lc[a] = v;
     }
@@ -1349,7 +1399,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   hipLaunchKernelGGL(P2MKernel, dim3(nn), dim3(64), 0, st, nn, (const Cell*)cells, (const double4*)sp, tab, (double*)M);
   for (int d = nlevels - 2; d >= 0; --d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
-    hipLaunchKernelGGL(M2MKernel, dim3(n1 - n0), dim3(64), 0, st, n0, o.nd, (const uint32_t*)o.bfs2pre, tab,
+    hipLaunchKernelGGL(M2MKernel, dim3(n1 - n0), dim3(64), TransferLds(tab, false), st, n0, o.nd,
+                       (const uint32_t*)o.bfs2pre, tab,
                        (double*)M);
   }
   TimerEnd(ctx, slot);
@@ -1545,7 +1596,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   TimerBegin(ctx, "fmm_downward", &slot);
   for (int d = 0; d < nlevels - 1; ++d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
-    hipLaunchKernelGGL(L2LKernel, dim3(n1 - n0), dim3(64), 0, st, n0, o.nd, (const uint32_t*)o.bfs2pre, tab,
+    hipLaunchKernelGGL(L2LKernel, dim3(n1 - n0), dim3(64), TransferLds(tab, true), st, n0, o.nd,
+                       (const uint32_t*)o.bfs2pre, tab,
                        (double*)L);
   }
   TimerEnd(ctx, slot);
