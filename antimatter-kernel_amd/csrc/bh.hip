@@ -228,16 +228,22 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
   }
   w.skip = lo;
   out[p] = w;
+  // FP32 monopoles; an empty charge class sits at the cell centre with charge 0, so the walk can
+  // sum both monopoles of an accepted node without testing (a charge-0 pair adds exactly 0).
   WalkNodeF32 f;
   const double qs = 1.0 / kElementaryCharge;
   {
-    const double xs = w.pqx * kUnitScale, ys = w.pqy * kUnitScale, zs = w.pqz * kUnitScale;
+    const bool has = w.qp > 0.0;
+    const double xs = (has ? w.pqx : w.cx) * kUnitScale, ys = (has ? w.pqy : w.cy) * kUnitScale,
+                 zs = (has ? w.pqz : w.cz) * kUnitScale;
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.p_hi = make_float4(xh, yh, zh, (float)(w.qp * qs));
     f.p_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
   }
   {
-    const double xs = w.nqx * kUnitScale, ys = w.nqy * kUnitScale, zs = w.nqz * kUnitScale;
+    const bool has = w.qn > 0.0;
+    const double xs = (has ? w.nqx : w.cx) * kUnitScale, ys = (has ? w.nqy : w.cy) * kUnitScale,
+                 zs = (has ? w.nqz : w.cz) * kUnitScale;
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.n_hi = make_float4(xh, yh, zh, (float)(-w.qn * qs));
     f.n_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
@@ -246,6 +252,8 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
 }
 
 // ---- walk -------------------------------------------------------------------------------------
+constexpr int kTreeSrcPad = 8;  // zero entries after the tree-ordered sources
+
 // Minimum over the wave (every lane active), returned wave-uniform.  DPP inclusive min-scan:
 // row_shr 1, 2, 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 across rows, so lane 63
 // ends with the minimum; lanes whose source is outside the row keep the identity.  Six dependent
@@ -312,14 +320,14 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         next = nd.skip;
       } else if (nd.leaf) {
         if (MODE == 0) {
-          // the leaf's sources are contiguous in the tree-ordered copy of the FP32 mirror
+          // The leaf's sources are contiguous in the tree-ordered copy of the FP32 mirror.  The target
+          // itself and charge-0 sources add exactly 0 (softened pair, q = 0), so they are not
+          // tested; the interaction count leaves them out.
           float fx = 0.f, fy = 0.f, fz = 0.f;
           for (int kk = nd.b; kk < nd.e; ++kk) {
-            if (kk == ks) continue;
             const float4 sh = ssrc[2 * kk];
-            if (sh.w == 0.f) continue;
             PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
-            ++cnt;
+            if (counts) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
           }
           ax += fx;
           ay += fy;
@@ -347,10 +355,11 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         const double w = __dmul_rn(2.0, nd.half);
         if (__dmul_rn(w, w) < __dmul_rn(theta2, d2)) {  // accepted (barnes-hut.cc:201-215)
           if (MODE == 0) {
-            const WalkNodeF32 f = nodes32[c];
+            const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            if (nd.qp > 0.0) { PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz); ++cnt; }
-            if (nd.qn > 0.0) { PairF32(th, tl, f.n_hi, f.n_lo, fx, fy, fz); ++cnt; }
+            PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz);
+            PairF32(th, tl, f.n_hi, f.n_lo, fx, fy, fz);
+            cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
             ax += fx;
             ay += fy;
             az += fz;
@@ -402,14 +411,16 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
-// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position).
+// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position), followed by
+// kTreeSrcPad zero entries that the walk's 8-source leaf reads may run into.
 __global__ void GatherTreeSrcKernel(int m, const uint32_t* __restrict__ order, const float4* __restrict__ src,
                                     float4* __restrict__ ssrc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const uint32_t i = order[k];
-  ssrc[2 * k] = src[2 * i];
-  ssrc[2 * k + 1] = src[2 * i + 1];
+  if (k >= m + kTreeSrcPad) return;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t i = k < m ? order[k] : 0u;
+  ssrc[2 * k] = k < m ? src[2 * i] : zero;
+  ssrc[2 * k + 1] = k < m ? src[2 * i + 1] : zero;
 }
 
 int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
@@ -473,9 +484,9 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
                      (FeatNode*)featp);
   AM_HIP(hipGetLastError());
   void* ssrc;
-  if ((rc = Scratch(ctx, "bh_ssrc", (size_t)m * 2 * sizeof(float4), &ssrc))) return rc;
-  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
-                     (const float4*)s->src, (float4*)ssrc);
+  if ((rc = Scratch(ctx, "bh_ssrc", ((size_t)m + kTreeSrcPad) * 2 * sizeof(float4), &ssrc))) return rc;
+  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m + kTreeSrcPad, 256)), dim3(256), 0, st, m,
+                     (const uint32_t*)o.order, (const float4*)s->src, (float4*)ssrc);
   AM_HIP(hipGetLastError());
   t->ssrc = (float4*)ssrc;
   t->feat = (FeatNode*)featp;
