@@ -252,7 +252,6 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
 }
 
 // ---- walk -------------------------------------------------------------------------------------
-constexpr int kTreeSrcPad = 8;  // zero entries after the tree-ordered sources
 
 // Minimum over the wave (every lane active), returned wave-uniform.  DPP inclusive min-scan:
 // row_shr 1, 2, 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 across rows, so lane 63
@@ -411,16 +410,14 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
-// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position), followed by
-// kTreeSrcPad zero entries that the walk's 8-source leaf reads may run into.
+// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position).
 __global__ void GatherTreeSrcKernel(int m, const uint32_t* __restrict__ order, const float4* __restrict__ src,
                                     float4* __restrict__ ssrc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m + kTreeSrcPad) return;
-  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t i = k < m ? order[k] : 0u;
-  ssrc[2 * k] = k < m ? src[2 * i] : zero;
-  ssrc[2 * k + 1] = k < m ? src[2 * i + 1] : zero;
+  if (k >= m) return;
+  const uint32_t i = order[k];
+  ssrc[2 * k] = src[2 * i];
+  ssrc[2 * k + 1] = src[2 * i + 1];
 }
 
 int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
@@ -484,9 +481,9 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
                      (FeatNode*)featp);
   AM_HIP(hipGetLastError());
   void* ssrc;
-  if ((rc = Scratch(ctx, "bh_ssrc", ((size_t)m + kTreeSrcPad) * 2 * sizeof(float4), &ssrc))) return rc;
-  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m + kTreeSrcPad, 256)), dim3(256), 0, st, m,
-                     (const uint32_t*)o.order, (const float4*)s->src, (float4*)ssrc);
+  if ((rc = Scratch(ctx, "bh_ssrc", (size_t)m * 2 * sizeof(float4), &ssrc))) return rc;
+  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
+                     (const float4*)s->src, (float4*)ssrc);
   AM_HIP(hipGetLastError());
   t->ssrc = (float4*)ssrc;
   t->feat = (FeatNode*)featp;
