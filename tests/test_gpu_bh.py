@@ -99,3 +99,23 @@ def test_degenerate_inputs(store):
     assert np.all(store.forces() == 0)  # coincident pairs are skipped (r^2 < 1e-30)
     store.upload(pos, np.zeros_like(pos), sp, annihilated=np.ones(20, np.uint8))
     store.compute_barnes_hut_coulomb_forces(0.5, am.FP32)
+
+
+@pytest.mark.parametrize("species", ["mixed", "positrons"])
+def test_fp32_neutral_dead_and_one_sign(store, species):
+    """FP32 walk with neutral and annihilated particles (charge-0 leaf sources, nodes whose
+    negative class is empty), against the FP64 oracle; the interaction counts of the FP32 walk
+    equal the FP64 walk's (same MAC decisions, charge-0 sources and the target itself excluded)."""
+    pos, sp, ann = cloud(6000, seed=25)
+    if species == "positrons":
+        sp = np.where(sp == 2, 2, 1).astype(np.uint8)  # positrons and neutrals only
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    m, q = oracle.mass_charge(sp)
+    cnt32 = store.barnes_hut_forces_counted(0.6, am.FP32)
+    f32 = store.forces()
+    cnt64 = store.barnes_hut_forces_counted(0.6, am.FP64)
+    ref = oracle.bh_forces(pos, q, ann, 0.6, 8)
+    live = ann == 0
+    assert rel_l2(f32[live], ref[live]) < 1e-5
+    assert np.all(f32[(sp == 2) & live] == 0)
+    np.testing.assert_array_equal(cnt32[live], cnt64[live])
