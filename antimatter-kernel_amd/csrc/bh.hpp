@@ -15,10 +15,12 @@ struct FeatNode {
 };
 
 struct WalkNode {
+  // The first 64 bytes are what the walk reads at every visit (one scalar fetch).
   double cx, cy, cz, half;     // geometric cube
-  double pqx, pqy, pqz, qp;    // positive charge centre and charge
-  double nqx, nqy, nqz, qn;    // negative charge centre and |charge|
+  double qp, qn;               // positive charge and |negative charge|
   int b, e, skip, leaf;        // particle range in the sorted order, subtree skip pointer
+  double pqx, pqy, pqz;        // positive charge centre
+  double nqx, nqy, nqz;        // negative charge centre
 };
 struct WalkNodeF32 {  // FP32 hi/lo mirror of the monopoles in units (charge in e)
   float4 p_hi, p_lo, n_hi, n_lo;
