@@ -795,13 +795,16 @@ __global__ void MultipoleToF32Kernel(int nn, int nm, int nks, const double* __re
   M32[k] = al < nm ? (float)M[S * nm + al] : 0.f;
 }
 
-// Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is the sort key
-// (a stable sort of a few bits), the pair index the value.
+// Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is sorted on (a
+// stable sort of its few low bits); the source cell rides in the key's upper half, so the product
+// kernel reads it with the class instead of gathering it from the list; the pair index is the value.
 __global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair_cls, uint32_t n_cls,
-                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                   const uint64_t* __restrict__ list, uint64_t* __restrict__ keys,
+                                   uint32_t* __restrict__ vals) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
-  keys[q] = pair_cls[q] == kNoClass ? n_cls : pair_cls[q];
+  const uint32_t c = pair_cls[q] == kNoClass ? n_cls : pair_cls[q];
+  keys[q] = ((uint64_t)(uint32_t)list[q] << 32) | c;
   vals[q] = (uint32_t)q;
 }
 
@@ -877,9 +880,10 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   auto column = [&](int64_t grp, uint32_t& cls, uint32_t& q, int& S, bool& valid) {
     const int64_t r = grp * 16 + j;
     valid = grp < g1 && r < np;
-    cls = valid ? (uint32_t)skeys[r] : 0xffffffffu;
+    const uint64_t kv = valid ? skeys[r] : ~0ull;
+    cls = (uint32_t)kv;
     q = valid ? svals[r] : 0u;
-    S = valid ? (int)(uint32_t)list[q] : 0;
+    S = valid ? (int)(kv >> 32) : 0;
   };
   uint32_t n_cls_, n_q;
   int n_S;
@@ -1562,7 +1566,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nks, 256)), dim3(256), 0, st, nn,
                          tab.nm, nks, (const double*)M, (float*)M32);
       hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                         (const uint32_t*)pslot, n_cls, (uint64_t*)ckeys, (uint32_t*)cvals);
+                         (const uint32_t*)pslot, n_cls, (const uint64_t*)m2l, (uint64_t*)ckeys, (uint32_t*)cvals);
       AM_HIP(hipGetLastError());
       int cls_bits = 1;
       while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
