@@ -775,24 +775,29 @@ __global__ void TensorToF32Kernel(int64_t n, const double* __restrict__ Dc, floa
 
 // ---- M2L on MFMA (the FP32-precision path, orders <= 5) -------------------------------------------
 // For displacement class c the M2L is one fixed matrix: L_gamma += sum_alpha A_c[gamma][alpha] M_alpha
-// with A_c[gamma][alpha] = (-1)^|alpha| D_{alpha+gamma} / gamma!.  Pairs sorted by class run as dense
-// products A_c [M_S1 .. M_S16] on v_mfma_f32_16x16x4f32, 16 pairs per product; A_c stays in
+// with A_c[gamma][alpha] = (-1)^|alpha| D_{alpha+gamma} / gamma!.  The sign is folded into the FP32
+// multipoles and 1/gamma! is applied in FP64 to each target's sum, so the product matrix is the plain
+// gather D_{alpha+gamma} from the class tensor.  Pairs sorted by class run as dense products
+// D_c [M'_S1 .. M'_S16] on v_mfma_f32_16x16x4f32, 16 pairs per product; the fragments of D_c stay in
 // registers while consecutive groups share the class.  Each pair's result row goes to its slot of
 // the target-grouped list straight from the product registers, and M2LSumKernel adds a target's
-// rows in list order, so the sum is deterministic.  FP32 rounding (~1e-7 relative) is far below the order-4/5 truncation error.
+// rows in list order, so the sum is deterministic.  FP32 rounding (~1e-7 relative) is far below the
+// order-4/5 truncation error.
 // Operand layout (16x16x4): lane l = (j, h) with j = l%16, h = l/16 holds A[j][h] and B[h][j];
-// the result D[4h + v][j], v = 0..3.
+// the result D[4h + v][j], v = 0..3.  Column j of the result depends on column j of B alone.
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 constexpr int kM2LGroupsPerWave = 16;  // groups of 16 class-sorted pairs per wave
 
-// FP32 multipoles in the B-operand order of the products: cell S's row of 4 nks floats holds
-// M_al at h * nks + ks for al = 4 ks + h (0 past nm), so lane (j, h) reads nks consecutive floats.
-__global__ void MultipoleToF32Kernel(int nn, int nm, int nks, const double* __restrict__ M, float* __restrict__ M32) {
+// FP32 signed multipoles in the B-operand order of the products: cell S's row of 4 nkp floats holds
+// (-1)^|al| M_al at h * nkp + ks for al = 4 ks + h (0 past nm); nkp is the k-step count rounded up
+// to 4, so lane (j, h) reads its values as aligned 16-byte loads.
+__global__ void MultipoleToF32Kernel(int nn, int nm, int nkp, const double* __restrict__ M,
+                                     const double* __restrict__ sign, float* __restrict__ M32) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (int64_t)nn * 4 * nks) return;
-  const int64_t S = k / (4 * nks);
-  const int e = (int)(k - S * 4 * nks), h = e / nks, ks = e - h * nks, al = 4 * ks + h;
-  M32[k] = al < nm ? (float)M[S * nm + al] : 0.f;
+  if (k >= (int64_t)nn * 4 * nkp) return;
+  const int64_t S = k / (4 * nkp);
+  const int e = (int)(k - S * 4 * nkp), h = e / nkp, ks = e - h * nkp, al = 4 * ks + h;
+  M32[k] = al < nm ? (float)(sign[al] * M[S * nm + al]) : 0.f;
 }
 
 // Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is sorted on (a
@@ -808,9 +813,10 @@ __global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair
   vals[q] = (uint32_t)q;
 }
 
-__device__ inline uint32_t WaveMinU32(uint32_t v) {
-  for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
-  return v;
+// LDS bytes of M2LMfmaKernel<NRB, NKS>: per wave Dd[nd] (doubles) and Df[nd + 1] (floats); the
+// per-lane fragment index table [NRB NKS][64]; the recurrence table [nd][10].
+static size_t M2LMfmaLds(int nd, int nrb, int nks) {
+  return sizeof(double) * (size_t)4 * (nd + (nd + 2) / 2) + sizeof(int) * ((size_t)nrb * nks * 64 + (size_t)nd * 10);
 }
 
 template <int NRB, int NKS>
@@ -820,122 +826,126 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
                                                      const float* __restrict__ Dc32, const Cell* __restrict__ cells,
                                                      FmmTab t, const float* __restrict__ M32,
                                                      float* __restrict__ out) {
-  // Dynamic LDS: per wave Dd[nd] (doubles, unclassed pairs) and Df[nd] (floats); then the m2l
-  // table [nm][nm] and the recurrence table [nd][10] (ints).
+  constexpr int NKP = (NKS + 3) & ~3;
   extern __shared__ double lds[];
   const int nm = t.nm, nd = t.nd;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slice = nd + (nd + 1) / 2;  // Dd, Df
-  int* s_m2l = (int*)(lds + 4 * slice);
-  int* s_rec = s_m2l + nm * nm;
-  for (int k = threadIdx.x; k < nm * nm; k += 256) s_m2l[k] = t.m2l[k];
+  const int slice = nd + (nd + 2) / 2;
+  int* s_aidx = (int*)(lds + 4 * slice);
+  int* s_rec = s_aidx + NRB * NKS * 64;
+  // Fragment (rb, ks) of lane (j, h) is D[m2l[16 rb + j][4 ks + h]]; outside the nm x nm matrix it
+  // is Df[nd], which holds 0.
+  for (int e = threadIdx.x; e < NRB * NKS * 64; e += 256) {
+    const int rk = e >> 6, l = e & 63, rb = rk / NKS, ks = rk - rb * NKS;
+    const int g = rb * 16 + (l & 15), al = ks * 4 + (l >> 4);
+    s_aidx[e] = (g < nm && al < nm) ? t.m2l[g * nm + al] : nd;
+  }
   for (int k = threadIdx.x; k < nd * 10; k += 256) s_rec[k] = t.rec[k];
-  __syncthreads();  // the only workgroup barrier
   double* Dd = lds + wave * slice;
   float* Df = reinterpret_cast<float*>(Dd + nd);
+  if (lane == 0) Df[nd] = 0.f;
+  __syncthreads();  // the only workgroup barrier
   const int j = lane & 15, h = lane >> 4;
   const int64_t n_groups = (np + 15) / 16;
   const int64_t g0 = ((int64_t)blockIdx.x * 4 + wave) * kM2LGroupsPerWave;
   const int64_t g1 = min(n_groups, g0 + kM2LGroupsPerWave);
-  // A_c fragments and the coefficients that scale them: A[g][al] = sign[al] / g! * D[m2l[g][al]]
+  const int nm4 = (nm + 3) & ~3;  // floats per result row
   float afrag[NRB][NKS];
   uint32_t cur = 0xffffffffu;
-  auto build_a = [&]() {  // A_c fragments from the tensor in Df
-    WaveSync();
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-      for (int ks = 0; ks < NKS; ++ks) {
-        const int g = rb * 16 + j, al = ks * 4 + h;
-        afrag[rb][ks] = (g < nm && al < nm) ? (float)(t.sign[al] * t.inv_fact[g]) * Df[s_m2l[g * nm + al]] : 0.f;
-      }
-    WaveSync();
-  };
-  // A_c [M_S] for the selected columns.  Lane (j, h) ends with D[4h + v][j] of each row block: the
-  // four consecutive coefficients 16 rb + 4h .. + 3 of pair j, stored as one 16-byte write into
-  // the pair's row of nm4 = 4 ceil(nm / 4) floats.
-  const int nm4 = (nm + 3) & ~3;
-  auto product = [&](bool sel, int S, uint32_t q) {
-    float bv[NKS];
-    const float* mrow = M32 + (size_t)S * (4 * NKS) + h * NKS;
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) bv[ks] = sel ? mrow[ks] : 0.f;
-    f32x4v acc[NRB];
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks)
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv[ks], acc[rb], 0, 0, 0);
-    if (sel) {
-      float* orow = out + (size_t)q * nm4;
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) {
-        const int g = rb * 16 + 4 * h;
-        if (g < nm4) *reinterpret_cast<f32x4v*>(orow + g) = acc[rb];
-      }
-    }
-  };
-  // the group's column: sorted position r -> (class, pair q, source cell S); prefetched one ahead
-  auto column = [&](int64_t grp, uint32_t& cls, uint32_t& q, int& S, bool& valid) {
+  // the group's column j: sorted position r -> key (source cell << 32 | class) and pair q; an
+  // empty column has key ~0
+  auto column = [&](int64_t grp, uint64_t& kv, uint32_t& q) {
     const int64_t r = grp * 16 + j;
-    valid = grp < g1 && r < np;
-    const uint64_t kv = valid ? skeys[r] : ~0ull;
-    cls = (uint32_t)kv;
+    const bool valid = grp < g1 && r < np;
+    kv = valid ? skeys[r] : ~0ull;
     q = valid ? svals[r] : 0u;
-    S = valid ? (int)(kv >> 32) : 0;
   };
-  uint32_t n_cls_, n_q;
-  int n_S;
-  bool n_valid;
-  column(g0, n_cls_, n_q, n_S, n_valid);
+  // the lane's multipole values of the column's source (cell 0 for an empty column: its result
+  // column is never stored)
+  auto row = [&](uint64_t kv, f32x4v (&bv)[NKP / 4]) {
+    const uint32_t S = kv == ~0ull ? 0u : (uint32_t)(kv >> 32);
+    const f32x4v* mrow = reinterpret_cast<const f32x4v*>(M32 + (size_t)S * (4 * NKP) + h * NKP);
+#pragma unroll
+    for (int i = 0; i < NKP / 4; ++i) bv[i] = mrow[i];
+  };
+  uint64_t kv0, kv1;
+  uint32_t q0, q1;
+  f32x4v bv0[NKP / 4];
+  column(g0, kv0, q0);
+  column(g0 + 1, kv1, q1);
+  row(kv0, bv0);
   for (int64_t grp = g0; grp < g1; ++grp) {
-    const uint32_t cls = n_cls_, q = n_q;
-    const int S = n_S;
-    const bool valid = n_valid;
-    column(grp + 1, n_cls_, n_q, n_S, n_valid);  // in flight during this group
-    const uint32_t c0 = __builtin_amdgcn_readfirstlane(cls);  // column 0 is always valid here
-    if (c0 != n_cls && __ballot(valid && cls != c0) == 0) {  // one class: the common case
-      if (c0 != cur) {
-        for (int k = lane; k < nd; k += 64) Df[k] = Dc32[(size_t)c0 * nd + k];
-        cur = c0;
-        build_a();
-      }
-      product(valid, S, q);
-      continue;
-    }
-    bool done = !valid;
-    for (;;) {  // several classes (or unclassed pairs) in the group: one class at a time
-      const uint32_t c = __builtin_amdgcn_readfirstlane(WaveMinU32(done ? 0xffffffffu : cls));
-      if (c == 0xffffffffu) break;
-      bool sel = !done && cls == c;
-      if (c == n_cls) {  // unclassed: one pair at a time with its own tensor
-        const int jj = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(sel)) - 1) & 15;
-        sel = sel && j == jj;
-        const int Sj = __shfl(S, jj);
-        const uint32_t qj = (uint32_t)__shfl((int)q, jj);
+    f32x4v bv1[NKP / 4];
+    row(kv1, bv1);  // the next group's multipoles and the column after it are in flight meanwhile
+    uint64_t kv2;
+    uint32_t q2;
+    column(grp + 2, kv2, q2);
+    const uint32_t cls = (uint32_t)kv0;
+    uint64_t pending = __ballot(kv0 != ~0ull);  // uniform
+    while (pending) {  // one class at a time (one pair at a time for unclassed pairs)
+      const int l0 = __builtin_ctzll(pending);
+      const uint32_t c = __builtin_amdgcn_readlane(cls, l0);
+      bool sel = ((pending >> lane) & 1ull) && cls == c;
+      bool rebuild = true;
+      if (c == n_cls) {  // unclassed: the pair's own tensor
+        sel = sel && j == (l0 & 15);
+        const uint32_t Sj = __builtin_amdgcn_readlane((uint32_t)(kv0 >> 32), l0);
+        const uint32_t qj = __builtin_amdgcn_readlane(q0, l0);
         const Cell& T = cells[(int)(list[qj] >> 32)];
         const Cell& Sc = cells[Sj];
         const double xs[3] = {T.cx - Sc.cx, T.cy - Sc.cy, T.cz - Sc.cz};
         DerivTensorWave(xs, t, s_rec, Dd);
         for (int k = lane; k < nd; k += 64) Df[k] = (float)Dd[k];
         cur = 0xffffffffu;
-        build_a();
       } else if (c != cur) {
+        WaveSync();
         for (int k = lane; k < nd; k += 64) Df[k] = Dc32[(size_t)c * nd + k];
         cur = c;
-        build_a();
+      } else {
+        rebuild = false;
       }
-      product(sel, S, q);
-      done = done || sel;
+      if (rebuild) {
+        WaveSync();
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int ks = 0; ks < NKS; ++ks) afrag[rb][ks] = Df[s_aidx[(rb * NKS + ks) * 64 + lane]];
+        WaveSync();
+      }
+      f32x4v acc[NRB];
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv0[ks >> 2][ks & 3], acc[rb], 0, 0, 0);
+      // Lane (j, h) holds the four consecutive coefficients 16 rb + 4h .. + 3 of pair j: one
+      // 16-byte write into the pair's row.
+      if (sel) {
+        float* orow = out + (size_t)q0 * nm4;
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+          const int g = rb * 16 + 4 * h;
+          if (g < nm4) *reinterpret_cast<f32x4v*>(orow + g) = acc[rb];
+        }
+      }
+      pending &= ~__ballot(sel);
     }
+    kv0 = kv1;
+    q0 = q1;
+#pragma unroll
+    for (int i = 0; i < NKP / 4; ++i) bv0[i] = bv1[i];
+    kv1 = kv2;
+    q1 = q2;
   }
 }
 
-// L of each target: its pairs' results in list order, summed in FP64.
+// L of each target: its pairs' results in list order, summed in FP64, times 1/gamma!.
 // out holds a row of nm4 floats per pair.
 __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm, int nm4,
-                                                   const float* __restrict__ out, double* __restrict__ L) {
+                                                   const float* __restrict__ out, const double* __restrict__ inv_fact,
+                                                   double* __restrict__ L) {
   const int p = blockIdx.x;
   const int64_t b = off[p], e = off[p + 1];
   for (int g = threadIdx.x; g < nm; g += 64) {
@@ -945,7 +955,7 @@ __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __rest
 #pragma unroll
       for (int u = 0; u < 8; ++u) a[u] += (double)out[(size_t)(q + u) * nm4 + g];
     for (; q < e; ++q) a[0] += (double)out[(size_t)q * nm4 + g];
-    L[(size_t)p * nm + g] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    L[(size_t)p * nm + g] = (((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) * inv_fact[g];
   }
 }
 
@@ -1553,9 +1563,10 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if (precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0) {
       void *Dc32, *M32, *ckeys, *cvals, *outp;
       if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
-      const int nks = tab.nm <= 36 ? 9 : 14;  // MFMA k-steps of the instance below
+      const int nrb = tab.nm <= 36 ? 3 : 4, nks = tab.nm <= 36 ? 9 : 14;  // the instance below
+      const int nkp = (nks + 3) & ~3;
       const int nm4 = (tab.nm + 3) & ~3;
-      if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * 4 * nks * 4, &M32))) return rc;
+      if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * 4 * nkp * 4, &M32))) return rc;
       if ((rc = Scratch(ctx, "fmm_cls_sortkeys", (size_t)n_m2l * 8, &ckeys))) return rc;
       if ((rc = Scratch(ctx, "fmm_cls_sortvals", (size_t)n_m2l * 4, &cvals))) return rc;
       if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * nm4 * 4, &outp))) return rc;
@@ -1563,16 +1574,15 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       if (nt > 0)
         hipLaunchKernelGGL(TensorToF32Kernel, dim3(Blocks(nt, 256)), dim3(256), 0, st, nt, (const double*)Dc,
                            (float*)Dc32);
-      hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nks, 256)), dim3(256), 0, st, nn,
-                         tab.nm, nks, (const double*)M, (float*)M32);
+      hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nkp, 256)), dim3(256), 0, st, nn,
+                         tab.nm, nkp, (const double*)M, tab.sign, (float*)M32);
       hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
                          (const uint32_t*)pslot, n_cls, (const uint64_t*)m2l, (uint64_t*)ckeys, (uint32_t*)cvals);
       AM_HIP(hipGetLastError());
       int cls_bits = 1;
       while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
       if ((rc = RadixSortPairs(ctx, (uint64_t*)ckeys, (uint32_t*)cvals, n_m2l, cls_bits))) return rc;
-      const size_t lds_mf = sizeof(double) * ((size_t)4 * (tab.nd + (tab.nd + 1) / 2)) +
-                            sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10);
+      const size_t lds_mf = M2LMfmaLds(tab.nd, nrb, nks);
       const int64_t n_groups = (n_m2l + 15) / 16;
       const int64_t waves = (n_groups + kM2LGroupsPerWave - 1) / kM2LGroupsPerWave;
       auto mf = [&](auto kernel) {
@@ -1586,7 +1596,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       else mf(M2LMfmaKernel<4, 14>);
       TimerEnd(ctx, mslot);
       hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm, nm4,
-                         (const float*)outp, (double*)L);
+                         (const float*)outp, tab.inv_fact, (double*)L);
       AM_HIP(hipGetLastError());
     } else if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
     else if (dpl <= 3) launch(M2LKernel<3, false>);
