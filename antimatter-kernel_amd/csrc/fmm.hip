@@ -1456,6 +1456,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     // grow outputs (preserving contents)
     auto grow = [&](const char* name, void** buf, size_t& cap, size_t need, size_t used, size_t elem) -> int {
       if (need <= cap) return AM_OK;
+      const size_t held = ctx->scratch[name].bytes / elem;  // a previous call's buffer may be larger
+      if (need <= held) {
+        cap = held;
+        return AM_OK;
+      }
       const size_t ncap = std::max(need, cap * 2);
       void* tmp = nullptr;
       if (used) {

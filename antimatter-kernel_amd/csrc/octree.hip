@@ -285,6 +285,11 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
     AM_HIP(hipMemcpyAsync(&kids, (uint32_t*)coff + cnt_nodes, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
     if (kids == 0) break;
+    if ((size_t)total + kids > cap) {  // a previous call's buffers may already be large enough
+      size_t held = SIZE_MAX;
+      for (int a = 0; a < 10; ++a) held = std::min(held, ctx->scratch[Name(tag, names[a])].bytes / sz[a]);
+      if ((size_t)total + kids <= held) cap = held;
+    }
     if ((size_t)total + kids > cap) {  // grow, preserving the nodes built so far
       const size_t ncap = std::max(cap * 2, (size_t)total + kids + 1024);
       auto dst = ptrs();
