@@ -1,4 +1,6 @@
 // Stable LSD radix sort of (uint64 key, uint32 value) pairs and an exclusive prefix scan.
+// (Below 2^30 keys the single-pass-per-digit sort near the end of this file is used; the tiled
+// passes described here remain for larger inputs.)
 //
 // Sort: 8-bit digits, one pass per digit of the key's significant bits.  Each pass:
 //   1. RadixHistKernel: per-tile (2048 keys) digit histogram in LDS -> hist[digit][tile].
@@ -179,6 +181,231 @@ __global__ __launch_bounds__(kScanThreads) void ScanTileApplyKernel(const uint32
   if (blockIdx.x == 0 && t == 0) out[n] = tile_off[num_tiles];
 }
 
+// ---- Single-pass-per-digit LSD radix sort ----------------------------------------------------------
+// One read of the keys gives every pass's global digit histogram (OsHistKernel) and one small kernel
+// their exclusive scans (OsDigitBaseKernel).  Each pass is then one kernel over tiles of 4096 keys
+// (OsScatterKernel):
+//   * a workgroup takes the next tile id from a counter, so every tile it may wait for belongs to a
+//     workgroup that is already running;
+//   * wave w ranks the tile's w-th quarter (1024 consecutive keys, 16 rounds of 64) by digit with 8
+//     ballots per round and wave-private digit counters, so ranking needs no workgroup barrier;
+//     quarters in wave order keep the sort stable;
+//   * the tile publishes its digit counts, then walks back over the earlier tiles' published counts
+//     (and stops at the first published prefix) to find how many keys of each digit come before it,
+//     and publishes its own prefix;
+//   * the keys are staged in LDS in digit order and written out in runs of consecutive positions.
+// The look-back waits at most kOsSpinLimit polls per tile and digit; a wait that long sets the error
+// word instead of hanging, and no write ever goes past n.
+constexpr int kOsThreads = 256;
+constexpr int kOsWaves = kOsThreads / 64;
+constexpr int kOsItems = 16;
+constexpr int kOsTile = kOsThreads * kOsItems;  // 4096 keys
+constexpr uint32_t kOsAgg = 1u << 30;           // status word: the tile's own count
+constexpr uint32_t kOsPre = 2u << 30;           // status word: the inclusive prefix through the tile
+constexpr uint32_t kOsVal = kOsAgg - 1;
+constexpr int kOsSpinLimit = 1 << 22;
+constexpr int kOsMaxPasses = 8;
+
+// Exclusive scan of one value per thread of a 256-thread workgroup (wave shuffles, one barrier);
+// wsum is [kOsWaves] LDS words.
+__device__ inline uint32_t OsBlockExclusiveScan(uint32_t v, uint32_t* wsum) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int i = 0; i < w; ++i) pre += wsum[i];
+  return pre + x - v;
+}
+
+__global__ void OsClearKernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// One tile of kOsTile keys per workgroup; every pass's digit counts added into ghist[pass][256].
+// Keys often arrive grouped (lists in traversal order, narrow key ranges), and same-address LDS
+// atomics serialise, so each run of equal digits over consecutive lanes of a load is added once,
+// by its first lane.
+__global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __restrict__ keys, int64_t n, int passes,
+                                                           uint32_t* __restrict__ ghist) {
+  __shared__ uint32_t h[kOsMaxPasses][256];
+  const int t = threadIdx.x, lane = t & 63;
+  for (int p = 0; p < passes; ++p) h[p][t] = 0u;
+  __syncthreads();
+  const int64_t tile0 = (int64_t)blockIdx.x * kOsTile;
+  uint64_t key[kOsItems];
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const int64_t idx = tile0 + r * kOsThreads + t;
+    key[r] = idx < n ? keys[idx] : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const bool valid = tile0 + r * kOsThreads + t < n;
+    const int nv = __popcll(__ballot(valid));  // the valid lanes are lanes 0 .. nv - 1
+    const uint32_t plo = (uint32_t)__shfl_up((int)(uint32_t)key[r], 1);
+    const uint32_t phi = (uint32_t)__shfl_up((int)(uint32_t)(key[r] >> 32), 1);
+    const uint64_t prev = ((uint64_t)phi << 32) | plo;
+    for (int p = 0; p < passes; ++p) {
+      const uint32_t d = (uint32_t)(key[r] >> (8 * p)) & 255u;
+      const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> (8 * p)) & 255u));
+      const uint64_t heads = __ballot(head);
+      if (head) {
+        const uint64_t above = lane == 63 ? 0ull : heads >> (lane + 1);
+        const uint32_t len = above ? (uint32_t)__builtin_ctzll(above) + 1u : (uint32_t)(nv - lane);
+        atomicAdd(&h[p][d], len);
+      }
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p)
+    if (h[p][t]) atomicAdd(&ghist[p * 256 + t], h[p][t]);
+}
+
+// gbase[pass][d] = keys of a smaller digit in that pass (one workgroup per pass).
+__global__ __launch_bounds__(kOsThreads) void OsDigitBaseKernel(const uint32_t* __restrict__ ghist,
+                                                                uint32_t* __restrict__ gbase) {
+  __shared__ uint32_t wsum[kOsWaves];
+  const int p = blockIdx.x, t = threadIdx.x;
+  gbase[p * 256 + t] = OsBlockExclusiveScan(ghist[p * 256 + t], wsum);
+}
+
+__global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ gbase,
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ status, uint32_t* __restrict__ err) {
+  __shared__ uint64_t sk[kOsTile];           // keys, then values, in digit order
+  __shared__ uint32_t wcnt[kOsWaves][256];  // per-wave digit counts, then per-wave digit offsets
+  __shared__ uint32_t loc[256], glob[256], wsum[kOsWaves];
+  __shared__ uint32_t s_tile;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (t == 0) s_tile = atomicAdd(counter, 1u);
+  for (int i = 0; i < kOsWaves; ++i) wcnt[i][t] = 0u;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t tile0 = (int64_t)tile * kOsTile;
+  const int64_t q0 = tile0 + (int64_t)w * (kOsTile / kOsWaves);  // this wave's quarter
+  uint64_t key[kOsItems];
+  uint32_t val[kOsItems], rk[kOsItems];
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const int64_t idx = q0 + r * 64 + lane;
+    key[r] = idx < n ? kin[idx] : 0ull;
+    val[r] = idx < n ? vin[idx] : 0u;
+  }
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t* mycnt = wcnt[w];
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const bool valid = q0 + r * 64 + lane < n;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+    uint64_t mask = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bits = __ballot(bit);
+      mask &= bit ? bits : ~bits;
+    }
+    const uint32_t before = mycnt[d];
+    rk[r] = before + (uint32_t)__popcll(mask & lt);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid && (mask & lt) == 0) mycnt[d] = before + (uint32_t)__popcll(mask);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __syncthreads();
+  // digit t: the tile's count, and each wave's offset among the tile's keys of digit t
+  uint32_t c = 0;
+  for (int i = 0; i < kOsWaves; ++i) {
+    const uint32_t ci = wcnt[i][t];
+    wcnt[i][t] = c;
+    c += ci;
+  }
+  // keys of digit t in the earlier tiles: thread t looks back over their status words
+  uint32_t excl = 0;
+  uint32_t* mine = status + (size_t)tile * 256 + t;
+  if (tile == 0) {
+    __hip_atomic_store(mine, kOsPre | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(mine, kOsAgg | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int64_t tt = (int64_t)tile - 1;
+    int spins = 0;
+    while (tt >= 0) {
+      const uint32_t sw =
+          __hip_atomic_load(status + (size_t)tt * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (sw & kOsPre) {
+        excl += sw & kOsVal;
+        break;
+      }
+      if (sw & kOsAgg) {
+        excl += sw & kOsVal;
+        --tt;
+        continue;
+      }
+      if (++spins > kOsSpinLimit) {
+        atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(mine, kOsPre | (excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const uint32_t lbase = OsBlockExclusiveScan(c, wsum);
+  loc[t] = lbase;
+  glob[t] = gbase[t] + excl;
+  __syncthreads();
+  // stage the keys in digit order and write runs of consecutive positions, then the values the same
+  // way (through the same LDS), so that the tile needs 32 KB of LDS
+  uint32_t sp[kOsItems];
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+    sp[r] = loc[d] + wcnt[w][d] + rk[r];
+    if (q0 + r * 64 + lane < n) sk[sp[r]] = key[r];
+  }
+  __syncthreads();
+  const int tile_n = (int)std::min<int64_t>(kOsTile, n - tile0);
+  uint32_t dst[kOsItems];
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const int i = r * kOsThreads + t;
+    dst[r] = 0xffffffffu;
+    if (i < tile_n) {
+      const uint64_t k = sk[i];
+      const uint32_t d = (uint32_t)(k >> shift) & 255u;
+      dst[r] = glob[d] + (uint32_t)i - loc[d];
+      if (dst[r] < (uint64_t)n) kout[dst[r]] = k;
+    }
+  }
+  __syncthreads();
+  uint32_t* sv = reinterpret_cast<uint32_t*>(sk);
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r)
+    if (q0 + r * 64 + lane < n) sv[sp[r]] = val[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kOsItems; ++r) {
+    const int i = r * kOsThreads + t;
+    if (i < tile_n && dst[r] < (uint64_t)n) vout[dst[r]] = sv[i];
+  }
+}
+
+// Words of the single-pass sort's scratch: histograms [passes][256], digit bases [passes][256], tile
+// counters [passes], the error word, then the status words [passes][tiles][256].
+static int64_t OsMetaWords(int passes) { return (int64_t)passes * 512 + passes + 1; }
+static int64_t OsWords(int64_t n, int passes) {
+  return OsMetaWords(passes) + (int64_t)passes * ((n + kOsTile - 1) / kOsTile) * 256;
+}
+
 int ExclusiveScan(am_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n) {
   if (n <= 0) {
     hipLaunchKernelGGL(ScanSumsKernel, dim3(1), dim3(kScanSumsThreads), 0, ctx->stream, in, out, (int64_t)0);
@@ -217,35 +444,62 @@ int ReserveSortScratch(am_ctx* ctx, int64_t n) {
   if ((rc = Scratch(ctx, "sort_hist", (size_t)hist * 4, &p))) return rc;
   if ((rc = Scratch(ctx, "sort_offs", (size_t)(hist + 1) * 4, &p))) return rc;
   if ((rc = Scratch(ctx, "scan_tiles", (size_t)(2 * scan_tiles + 1) * 4, &p))) return rc;
+  if ((rc = Scratch(ctx, "sort_onesweep", (size_t)OsWords(n, kOsMaxPasses) * 4, &p))) return rc;
   return AM_OK;
 }
 
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
   if (n <= 1) return AM_OK;
   if (n > 0x7fffffffLL) return Fail(AM_ERR_CAPACITY, "radix sort: n too large");
-  const int num_tiles = (int)((n + kSortTile - 1) / kSortTile);
-  void *kbuf, *vbuf, *hist, *offs;
+  void *kbuf, *vbuf;
   int rc;
   if ((rc = Scratch(ctx, "sort_keys", (size_t)n * 8, &kbuf))) return rc;
   if ((rc = Scratch(ctx, "sort_vals", (size_t)n * 4, &vbuf))) return rc;
-  if ((rc = Scratch(ctx, "sort_hist", (size_t)256 * num_tiles * 4, &hist))) return rc;
-  if ((rc = Scratch(ctx, "sort_offs", ((size_t)256 * num_tiles + 1) * 4, &offs))) return rc;
   uint64_t* ka = keys;
   uint32_t* va = vals;
   uint64_t* kb = (uint64_t*)kbuf;
   uint32_t* vb = (uint32_t*)vbuf;
   const int passes = (std::max(1, std::min(64, key_bits)) + 7) / 8;
-  for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(RadixHistKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, n, shift,
-                       (uint32_t*)hist, num_tiles);
+  if (n < (int64_t)kOsVal) {  // single-pass-per-digit sort (status words count up to kOsVal)
+    const int64_t tiles = (n + kOsTile - 1) / kOsTile;
+    const int64_t words = OsWords(n, passes);
+    void* os;
+    if ((rc = Scratch(ctx, "sort_onesweep", (size_t)words * 4, &os))) return rc;
+    uint32_t* ghist = (uint32_t*)os;
+    uint32_t* gbase = ghist + passes * 256;
+    uint32_t* counters = gbase + passes * 256;
+    uint32_t* err = counters + passes;
+    uint32_t* status = ghist + OsMetaWords(passes);
+    hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint32_t*)os, words);
+    hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, passes, ghist);
+    hipLaunchKernelGGL(OsDigitBaseKernel, dim3(passes), dim3(kOsThreads), 0, ctx->stream, (const uint32_t*)ghist,
+                       gbase);
     AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)hist, (uint32_t*)offs, (int64_t)256 * num_tiles))) return rc;
-    hipLaunchKernelGGL(RadixScatterKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, va, kb, vb, n,
-                       shift, (const uint32_t*)offs, num_tiles);
-    AM_HIP(hipGetLastError());
-    std::swap(ka, kb);
-    std::swap(va, vb);
+    for (int p = 0; p < passes; ++p) {
+      hipLaunchKernelGGL(OsScatterKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka, va, kb, vb, n,
+                         8 * p, gbase + 256 * p, counters + p, status + (size_t)p * tiles * 256, err);
+      AM_HIP(hipGetLastError());
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
+  } else {  // per-tile histograms and a scan per pass
+    const int num_tiles = (int)((n + kSortTile - 1) / kSortTile);
+    void *hist, *offs;
+    if ((rc = Scratch(ctx, "sort_hist", (size_t)256 * num_tiles * 4, &hist))) return rc;
+    if ((rc = Scratch(ctx, "sort_offs", ((size_t)256 * num_tiles + 1) * 4, &offs))) return rc;
+    for (int p = 0; p < passes; ++p) {
+      const int shift = 8 * p;
+      hipLaunchKernelGGL(RadixHistKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, n, shift,
+                         (uint32_t*)hist, num_tiles);
+      AM_HIP(hipGetLastError());
+      if ((rc = ExclusiveScan(ctx, (const uint32_t*)hist, (uint32_t*)offs, (int64_t)256 * num_tiles))) return rc;
+      hipLaunchKernelGGL(RadixScatterKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, va, kb, vb, n,
+                         shift, (const uint32_t*)offs, num_tiles);
+      AM_HIP(hipGetLastError());
+      std::swap(ka, kb);
+      std::swap(va, vb);
+    }
   }
   if (ka != keys) {
     AM_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));

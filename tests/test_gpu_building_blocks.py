@@ -23,3 +23,50 @@ def test_radix_sort_is_stable(ctx, n, bits):
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(k, keys[order])
     np.testing.assert_array_equal(v, vals[order])
+
+
+@pytest.mark.parametrize("n,bits", [(4095, 8), (4096, 16), (4097, 17), (8193, 24), (65537, 9)])
+def test_radix_sort_pass_counts_and_ragged_tiles(ctx, n, bits):
+    rng = np.random.default_rng(11 + n)
+    keys = rng.integers(0, 1 << bits, n, dtype=np.uint64)
+    vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    k, v = ctx.radix_sort(keys, vals, bits)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
+
+
+def test_radix_sort_heavy_duplicates(ctx):
+    n = 250_000
+    rng = np.random.default_rng(3)
+    keys = rng.integers(0, 4, n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.radix_sort(keys, vals, 16)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
+
+
+def test_radix_sort_full_width_random_keys(ctx):
+    n = 300_001
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.radix_sort(keys, vals, 64)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
+
+
+def test_radix_sort_low_bits_only_with_payload_above(ctx):
+    # Keys carry a payload in their upper half; only the low 17 bits are sorted on (stably).
+    n = 200_003
+    rng = np.random.default_rng(9)
+    low = rng.integers(0, 1 << 17, n, dtype=np.uint64)
+    high = rng.integers(0, 1 << 32, n, dtype=np.uint64)
+    keys = (high << np.uint64(32)) | low
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.radix_sort(keys, vals, 17)
+    order = np.argsort(low, kind="stable")
+    np.testing.assert_array_equal(k, keys[order])
+    np.testing.assert_array_equal(v, vals[order])
