@@ -399,9 +399,10 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
   }
 }
 
-// Words of the single-pass sort's scratch: histograms [passes][256], digit bases [passes][256], tile
-// counters [passes], the error word, then the status words [passes][tiles][256].
-static int64_t OsMetaWords(int passes) { return (int64_t)passes * 512 + passes + 1; }
+// Words of the single-pass sort's scratch: the error word (first, so that its place does not depend on
+// the pass count), histograms [passes][256], digit bases [passes][256], tile counters [passes], then
+// the status words [passes][tiles][256].
+static int64_t OsMetaWords(int passes) { return 1 + (int64_t)passes * 512 + passes; }
 static int64_t OsWords(int64_t n, int passes) {
   return OsMetaWords(passes) + (int64_t)passes * ((n + kOsTile - 1) / kOsTile) * 256;
 }
@@ -465,11 +466,11 @@ int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int k
     const int64_t words = OsWords(n, passes);
     void* os;
     if ((rc = Scratch(ctx, "sort_onesweep", (size_t)words * 4, &os))) return rc;
-    uint32_t* ghist = (uint32_t*)os;
+    uint32_t* err = (uint32_t*)os;
+    uint32_t* ghist = err + 1;
     uint32_t* gbase = ghist + passes * 256;
     uint32_t* counters = gbase + passes * 256;
-    uint32_t* err = counters + passes;
-    uint32_t* status = ghist + OsMetaWords(passes);
+    uint32_t* status = err + OsMetaWords(passes);
     hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
                        ctx->stream, (uint32_t*)os, words);
     hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, passes, ghist);
@@ -540,7 +541,11 @@ int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
   if ((rc = RadixSortPairs(ctx, (uint64_t*)dk, (uint32_t*)dv, n, key_bits))) return rc;
   AM_HIP(hipMemcpyAsync(keys, dk, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipMemcpyAsync(vals, dv, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t err = 0;
+  if (n > 1 && n < (int64_t)kOsVal)  // the single-pass sort's look-back error word
+    AM_HIP(hipMemcpyAsync(&err, ctx->scratch["sort_onesweep"].ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
+  if (err) return Fail(AM_ERR_HIP, "am_debug_radix_sort: a look-back wait on the device gave up");
   return AM_OK;
 }
 
