@@ -132,7 +132,8 @@ __device__ inline int Digit(uint64_t key, int level) {
   return (int)((key >> (3 * (kOctreeLevels - 1 - level))) & 7u);
 }
 
-__global__ void RootNodeKernel(NodeArrays nd, const RootBox* __restrict__ root, int m) {
+// The root, and the first two level starts (depth 0 is node 0, depth 1 starts at node 1).
+__global__ void RootNodeKernel(NodeArrays nd, const RootBox* __restrict__ root, int m, int* __restrict__ lvl) {
   nd.b[0] = 0;
   nd.e[0] = m;
   nd.depth[0] = 0;
@@ -140,62 +141,144 @@ __global__ void RootNodeKernel(NodeArrays nd, const RootBox* __restrict__ root, 
   nd.cy[0] = root->cy;
   nd.cz[0] = root->cz;
   nd.half[0] = root->half;
+  lvl[0] = 0;
+  lvl[1] = 1;
 }
 
-__global__ void SplitCountKernel(int n0, int n1, NodeArrays nd, const uint64_t* __restrict__ keys, int leaf_cap,
-                                 int* __restrict__ bounds, uint32_t* __restrict__ cnt) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int node = n0 + t;
-  if (node >= n1) return;
-  const int b = nd.b[node], e = nd.e[node], d = nd.depth[node];
-  int* bd = bounds + 9 * t;
-  if (e - b <= leaf_cap || d >= kOctreeLevels) {
-    nd.leaf[node] = 1;
-    nd.nchild[node] = 0;
-    nd.first_child[node] = -1;
-    cnt[t] = 0;
+// One level of the split, depth d (nodes [lvl[d], lvl[d + 1])), in one kernel.  A node holding more
+// than leaf_cap particles above the deepest level splits into its non-empty octants (their key
+// ranges found by binary search on the sorted keys); the level's children are numbered from
+// lvl[d + 1] on, in node order and octant order within a node.  The child offsets come from a scan
+// over tiles of 256 nodes: a tile takes the next tile id from a counter (so every tile it waits for
+// is already running), publishes its child count, walks back over the earlier tiles' published
+// counts to the first published prefix, and publishes its own prefix.  The last tile sets
+// lvl[d + 2].  Children past cap are not written: flags[0] is set and the next levels are empty.
+// A look-back that waits kLvSpinLimit polls gives up and sets flags[1] (as does a grid smaller than
+// the level's tile count, which the launch bound rules out).
+constexpr int kLvThreads = 256;
+constexpr uint32_t kLvAgg = 1u << 30;
+constexpr uint32_t kLvPre = 2u << 30;
+constexpr uint32_t kLvVal = kLvAgg - 1;
+constexpr int kLvSpinLimit = 1 << 22;
+
+__global__ __launch_bounds__(kLvThreads) void OctreeLevelKernel(int d, int* __restrict__ lvl, NodeArrays nd,
+                                                                const uint64_t* __restrict__ keys, int leaf_cap,
+                                                                int64_t cap, uint32_t* __restrict__ counter,
+                                                                uint32_t* __restrict__ status,
+                                                                uint32_t* __restrict__ flags) {
+  __shared__ uint32_t wsum[kLvThreads / 64];
+  __shared__ uint32_t s_tile, s_excl;
+  const int n0 = lvl[d], n1 = lvl[d + 1];
+  const int count = n1 - n0;
+  const int tiles = (count + kLvThreads - 1) / kLvThreads;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (count == 0) {
+    if (blockIdx.x == 0 && t == 0) lvl[d + 2] = n1;
     return;
   }
-  nd.leaf[node] = 0;
-  bd[0] = b;
-  for (int o = 1; o < 8; ++o) {  // first position whose digit at level d is >= o
-    int lo = bd[o - 1], hi = e;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (Digit(keys[mid], d) < o) lo = mid + 1;
-      else hi = mid;
-    }
-    bd[o] = lo;
-  }
-  bd[8] = e;
+  if (blockIdx.x == 0 && t == 0 && tiles > (int)gridDim.x) atomicOr(&flags[1], 2u);  // grid below the count
+  if ((int)blockIdx.x >= tiles) return;
+  if (t == 0) s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  const int tile = (int)s_tile;
+  const int node = n0 + tile * kLvThreads + t;
+  int bd[9];
   uint32_t c = 0;
-  for (int o = 0; o < 8; ++o) c += bd[o + 1] > bd[o] ? 1u : 0u;
-  cnt[t] = c;
-}
-
-__global__ void WriteChildrenKernel(int n0, int n1, int next_base, NodeArrays nd, const int* __restrict__ bounds,
-                                    const uint32_t* __restrict__ off) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int node = n0 + t;
-  if (node >= n1 || nd.leaf[node]) return;
-  const int* bd = bounds + 9 * t;
-  const int c = next_base + (int)off[t];
-  nd.first_child[node] = c;
-  int k = 0;
-  const double ch = 0.5 * nd.half[node];
-  const double px = nd.cx[node], py = nd.cy[node], pz = nd.cz[node];
-  for (int o = 0; o < 8; ++o) {
-    if (bd[o + 1] == bd[o]) continue;
-    const int ci = c + k++;
-    nd.b[ci] = bd[o];
-    nd.e[ci] = bd[o + 1];
-    nd.depth[ci] = nd.depth[node] + 1;
-    nd.cx[ci] = px + ((o & 1) ? ch : -ch);
-    nd.cy[ci] = py + ((o & 2) ? ch : -ch);
-    nd.cz[ci] = pz + ((o & 4) ? ch : -ch);
-    nd.half[ci] = ch;
+  bool split = false;
+  if (node < n1) {
+    const int b = nd.b[node], e = nd.e[node];
+    split = e - b > leaf_cap && nd.depth[node] < kOctreeLevels;
+    if (split) {
+      bd[0] = b;
+      for (int o = 1; o < 8; ++o) {  // first position whose digit at depth d is >= o
+        int lo = bd[o - 1], hi = e;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (Digit(keys[mid], d) < o) lo = mid + 1;
+          else hi = mid;
+        }
+        bd[o] = lo;
+      }
+      bd[8] = e;
+      for (int o = 0; o < 8; ++o) c += bd[o + 1] > bd[o] ? 1u : 0u;
+    }
   }
-  nd.nchild[node] = k;
+  // exclusive scan of the children over the tile (wave shuffles, one barrier)
+  uint32_t x = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t local = x - c, total = 0;
+  for (int i = 0; i < kLvThreads / 64; ++i) {
+    if (i < w) local += wsum[i];
+    total += wsum[i];
+  }
+  if (t == 0) {
+    uint32_t excl = 0;
+    uint32_t* mine = status + tile;
+    if (tile == 0) {
+      __hip_atomic_store(mine, kLvPre | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(mine, kLvAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int tt = tile - 1;
+      int spins = 0;
+      while (tt >= 0) {
+        const uint32_t sw = __hip_atomic_load(status + tt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sw & kLvPre) {
+          excl += sw & kLvVal;
+          break;
+        }
+        if (sw & kLvAgg) {
+          excl += sw & kLvVal;
+          --tt;
+          continue;
+        }
+        if (++spins > kLvSpinLimit) {
+          atomicOr(&flags[1], 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __hip_atomic_store(mine, kLvPre | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_excl = excl;
+  }
+  __syncthreads();
+  const int64_t first = (int64_t)n1 + s_excl;  // the tile's first child id
+  const bool over = first + total > cap;
+  if (over && t == 0) atomicOr(&flags[0], 1u);
+  if (node < n1) {
+    if (!split) {
+      nd.leaf[node] = 1;
+      nd.nchild[node] = 0;
+      nd.first_child[node] = -1;
+    } else if (!over) {
+      const int ci0 = (int)(first + local);
+      nd.leaf[node] = 0;
+      nd.first_child[node] = ci0;
+      const double ch = 0.5 * nd.half[node];
+      const double px = nd.cx[node], py = nd.cy[node], pz = nd.cz[node];
+      const int dep = nd.depth[node] + 1;
+      int k = 0;
+      for (int o = 0; o < 8; ++o) {
+        if (bd[o + 1] == bd[o]) continue;
+        const int ci = ci0 + k++;
+        nd.b[ci] = bd[o];
+        nd.e[ci] = bd[o + 1];
+        nd.depth[ci] = dep;
+        nd.cx[ci] = px + ((o & 1) ? ch : -ch);
+        nd.cy[ci] = py + ((o & 2) ? ch : -ch);
+        nd.cz[ci] = pz + ((o & 4) ? ch : -ch);
+        nd.half[ci] = ch;
+      }
+      nd.nchild[node] = k;
+    }
+  }
+  if (tile == tiles - 1 && t == 0) lvl[d + 2] = over ? n1 : (int)(first + total);
 }
 
 // Leaves keep particle-index order (the reference's stable partition stops at the leaf).
@@ -218,6 +301,11 @@ __global__ void LeafSortKernel(int nn, NodeArrays nd, uint32_t* __restrict__ val
 __global__ void ZeroU32Kernel(uint32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+__global__ void ClearU32Kernel(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
 }
 
 __global__ void PreorderKeyKernel(int nn, NodeArrays nd, uint64_t* __restrict__ keys, uint32_t* __restrict__ ids) {
@@ -252,6 +340,11 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   NodeArrays& nd = t->nd;
   const char* names[] = {"nb", "ne", "nd", "nfc", "nnc", "nleaf", "ncx", "ncy", "ncz", "nhalf"};
   const size_t sz[] = {4, 4, 4, 4, 4, 1, 8, 8, 8, 8};
+  {  // a previous call's node arrays may be larger
+    size_t held = SIZE_MAX;
+    for (int a = 0; a < 10; ++a) held = std::min(held, ctx->scratch[Name(tag, names[a])].bytes / sz[a]);
+    if (held != SIZE_MAX && held > cap) cap = std::min<size_t>(held, (size_t)kLvVal);
+  }
   auto ptrs = [&]() {
     return std::vector<void**>{(void**)&nd.b, (void**)&nd.e, (void**)&nd.depth, (void**)&nd.first_child,
                                (void**)&nd.nchild, (void**)&nd.leaf, (void**)&nd.cx, (void**)&nd.cy,
@@ -265,54 +358,50 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
       *dst[a] = p;
     }
   }
-  hipLaunchKernelGGL(RootNodeKernel, dim3(1), dim3(1), 0, st, nd, root, m);
-  AM_HIP(hipGetLastError());
-  t->level_start = {0, 1};
-  int total = 1;
-  for (int d = 0; d <= kOctreeLevels; ++d) {
-    const int n0 = t->level_start[d], n1 = t->level_start[d + 1];
-    const int cnt_nodes = n1 - n0;
-    if (cnt_nodes == 0) break;
-    void *bounds, *cnt, *coff;
-    if ((rc = Scratch(ctx, Name(tag, "bounds").c_str(), (size_t)cnt_nodes * 9 * 4, &bounds))) return rc;
-    if ((rc = Scratch(ctx, Name(tag, "cnt").c_str(), (size_t)cnt_nodes * 4, &cnt))) return rc;
-    if ((rc = Scratch(ctx, Name(tag, "coff").c_str(), (size_t)(cnt_nodes + 1) * 4, &coff))) return rc;
-    hipLaunchKernelGGL(SplitCountKernel, dim3(Blocks(cnt_nodes, 256)), dim3(256), 0, st, n0, n1, nd,
-                       (const uint64_t*)keys, leaf_cap, (int*)bounds, (uint32_t*)cnt);
+  // The levels run back to back with no host round trip: level d's grid covers an upper bound of its
+  // node count (at most 8 per parent, at most one per particle); its real count is read on the
+  // device.  Level starts, the overflow and the look-back error come back once at the end; an
+  // overflow rebuilds with larger node arrays.
+  const int max_tiles = (m + kLvThreads - 1) / kLvThreads;
+  const int64_t lv_words = 2 + (int64_t)(kOctreeLevels + 1) * (1 + max_tiles);  // flags, counters, status
+  void *lvl, *lvw;
+  if ((rc = Scratch(ctx, Name(tag, "lvl").c_str(), (kOctreeLevels + 3) * sizeof(int), &lvl))) return rc;
+  if ((rc = Scratch(ctx, Name(tag, "lvwords").c_str(), (size_t)lv_words * 4, &lvw))) return rc;
+  uint32_t* flags = (uint32_t*)lvw;
+  uint32_t* counters = flags + 2;
+  uint32_t* status = counters + kOctreeLevels + 1;
+  int lv[kOctreeLevels + 3];
+  for (;;) {
+    hipLaunchKernelGGL(ClearU32Kernel, dim3((unsigned)std::min<int64_t>(1024, (lv_words + 255) / 256)), dim3(256), 0,
+                       st, (uint32_t*)lvw, lv_words);
+    hipLaunchKernelGGL(RootNodeKernel, dim3(1), dim3(1), 0, st, nd, root, m, (int*)lvl);
     AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)coff, cnt_nodes))) return rc;
-    uint32_t kids = 0;
-    AM_HIP(hipMemcpyAsync(&kids, (uint32_t*)coff + cnt_nodes, 4, hipMemcpyDeviceToHost, st));
+    int64_t ub = 1;
+    for (int d = 0; d <= kOctreeLevels; ++d) {
+      hipLaunchKernelGGL(OctreeLevelKernel, dim3((unsigned)((ub + kLvThreads - 1) / kLvThreads)), dim3(kLvThreads), 0,
+                         st, d, (int*)lvl, nd, (const uint64_t*)keys, leaf_cap, (int64_t)cap, counters + d,
+                         status + (size_t)d * max_tiles, flags);
+      ub = std::min<int64_t>(ub * 8, m);
+    }
+    AM_HIP(hipGetLastError());
+    uint32_t fl[2];
+    AM_HIP(hipMemcpyAsync(lv, lvl, sizeof(lv), hipMemcpyDeviceToHost, st));
+    AM_HIP(hipMemcpyAsync(fl, flags, sizeof(fl), hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
-    if (kids == 0) break;
-    if ((size_t)total + kids > cap) {  // a previous call's buffers may already be large enough
-      size_t held = SIZE_MAX;
-      for (int a = 0; a < 10; ++a) held = std::min(held, ctx->scratch[Name(tag, names[a])].bytes / sz[a]);
-      if ((size_t)total + kids <= held) cap = held;
+    if (fl[1]) return Fail(AM_ERR_HIP, "octree build: a look-back wait on the device gave up");
+    if (!fl[0]) break;
+    if (cap >= (size_t)kLvVal) return Fail(AM_ERR_CAPACITY, "octree build: too many nodes");
+    cap = std::min<size_t>(cap * 2, (size_t)kLvVal);  // rebuild with larger node arrays
+    auto dst = ptrs();
+    for (int a = 0; a < 10; ++a) {
+      void* p;
+      if ((rc = Scratch(ctx, Name(tag, names[a]).c_str(), cap * sz[a], &p))) return rc;
+      *dst[a] = p;
     }
-    if ((size_t)total + kids > cap) {  // grow, preserving the nodes built so far
-      const size_t ncap = std::max(cap * 2, (size_t)total + kids + 1024);
-      auto dst = ptrs();
-      for (int a = 0; a < 10; ++a) {
-        void* tmp;
-        AM_HIP(hipMalloc(&tmp, (size_t)total * sz[a]));
-        AM_HIP(hipMemcpyAsync(tmp, *dst[a], (size_t)total * sz[a], hipMemcpyDeviceToDevice, st));
-        void* p;
-        AM_HIP(hipStreamSynchronize(st));
-        if ((rc = Scratch(ctx, Name(tag, names[a]).c_str(), ncap * sz[a], &p))) return rc;
-        AM_HIP(hipMemcpyAsync(p, tmp, (size_t)total * sz[a], hipMemcpyDeviceToDevice, st));
-        AM_HIP(hipStreamSynchronize(st));
-        AM_HIP(hipFree(tmp));
-        *dst[a] = p;
-      }
-      cap = ncap;
-    }
-    hipLaunchKernelGGL(WriteChildrenKernel, dim3(Blocks(cnt_nodes, 256)), dim3(256), 0, st, n0, n1, total, nd,
-                       (const int*)bounds, (const uint32_t*)coff);
-    AM_HIP(hipGetLastError());
-    total += (int)kids;
-    t->level_start.push_back(total);
   }
+  t->level_start = {0};
+  for (int d = 0; d <= kOctreeLevels + 1 && lv[d + 1] > lv[d]; ++d) t->level_start.push_back(lv[d + 1]);
+  const int total = t->level_start.back();
   const int nn = total;
   void* over;
   if ((rc = Scratch(ctx, Name(tag, "over").c_str(), 4, &over))) return rc;
