@@ -941,21 +941,52 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   }
 }
 
-// L of each target: its pairs' results in list order, summed in FP64, times 1/gamma!.
-// out holds a row of nm4 floats per pair.
+// L of each target: its pairs' result rows summed in FP64, times 1/gamma!.  out holds a row of nm4
+// floats per pair (nm4 a multiple of 4, at most 64).  Lane l reads the 16-byte chunk l % nch
+// (nch = nm4 / 4) of rows b + l / nch, + rpi, + 2 rpi, ... (rpi = 64 / nch rows per load
+// instruction), and the rpi partial sums of a chunk are then added in row-group order: the order is
+// fixed, so the sum is deterministic.
 __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm, int nm4,
                                                    const float* __restrict__ out, const double* __restrict__ inv_fact,
                                                    double* __restrict__ L) {
-  const int p = blockIdx.x;
+  __shared__ double part[64][4];
+  const int p = blockIdx.x, lane = threadIdx.x;
   const int64_t b = off[p], e = off[p + 1];
-  for (int g = threadIdx.x; g < nm; g += 64) {
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight independent chains, fixed order
-    int64_t q = b;
-    for (; q + 8 <= e; q += 8)
+  const int nch = nm4 / 4, rpi = 64 / nch;
+  const int r0 = lane / nch, ch = lane - r0 * nch;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (r0 < rpi) {
+    int64_t q = b + r0;
+    for (; q + 3 * (int64_t)rpi < e; q += 4 * (int64_t)rpi) {
+      f32x4v v[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] += (double)out[(size_t)(q + u) * nm4 + g];
-    for (; q < e; ++q) a[0] += (double)out[(size_t)q * nm4 + g];
-    L[(size_t)p * nm + g] = (((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) * inv_fact[g];
+      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const f32x4v*>(out + (size_t)(q + u * rpi) * nm4 + 4 * ch);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0 += (double)v[u][0];
+        a1 += (double)v[u][1];
+        a2 += (double)v[u][2];
+        a3 += (double)v[u][3];
+      }
+    }
+    for (; q < e; q += rpi) {
+      const f32x4v v = *reinterpret_cast<const f32x4v*>(out + (size_t)q * nm4 + 4 * ch);
+      a0 += (double)v[0];
+      a1 += (double)v[1];
+      a2 += (double)v[2];
+      a3 += (double)v[3];
+    }
+  }
+  part[lane][0] = a0;
+  part[lane][1] = a1;
+  part[lane][2] = a2;
+  part[lane][3] = a3;
+  __syncthreads();
+  if (lane < nm4) {
+    const int c = lane >> 2, j = lane & 3, g = lane;
+    double sum = 0.0;
+    for (int k = 0; k < rpi; ++k) sum += part[c + k * nch][j];
+    if (g < nm) L[(size_t)p * nm + g] = sum * inv_fact[g];
   }
 }
 
