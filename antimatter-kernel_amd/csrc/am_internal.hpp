@@ -181,6 +181,8 @@ int64_t CountActive(am_store* s, int* status);
 // Radix sort (stable, LSD) of 64-bit keys with 32-bit values.  keys/vals are sorted in place
 // (ping-pong through scratch).  key_bits: number of low bits that may be non-zero.
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
+// Stable radix sort of 64-bit keys alone on bits [bit0, bit0 + key_bits), in place; n < 2^30.
+int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);
 // Deterministic FP64 sum of n device doubles into *host_out (synchronises the stream).
 int ReduceSum(am_ctx* ctx, const double* in, int64_t n, double* host_out);
 // Pre-sizes the scratch of RadixSortPairs / ExclusiveScan for n elements (no allocation later).

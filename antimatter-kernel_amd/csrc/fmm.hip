@@ -459,21 +459,6 @@ __global__ void TraverseTotalsKernel(int np, const uint32_t* __restrict__ off, u
   }
 }
 
-// (T << 32 | S) -> key T, value S, and back: the lists are grouped by target with a stable sort on
-// the cb bits of T, so within a target the pairs keep the traversal's (deterministic) order.
-__global__ void PairSplitKernel(int64_t n, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = keys[i];
-  keys[i] = k >> 32;
-  vals[i] = (uint32_t)k;
-}
-
-__global__ void PairJoinKernel(int64_t n, uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) keys[i] = (keys[i] << 32) | vals[i];
-}
-
 // CSR offsets of target cells in a pair list (T << 32 | S) grouped by ascending T.
 __global__ void PairOffsetsKernel(int nn, const uint64_t* __restrict__ keys, int64_t np, int64_t* __restrict__ off) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1524,28 +1509,12 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     // keep the scratch names pointing at the right buffers
     std::swap(ctx->scratch["fmm_front_a"], ctx->scratch["fmm_front_b"]);
   }
-  void *v1, *v2, *off_m2l, *off_p2p;
-  if ((rc = Scratch(ctx, "fmm_v1", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &v1))) return rc;
-  if ((rc = Scratch(ctx, "fmm_v2", (size_t)std::max<int64_t>(n_p2p, 1) * 4, &v2))) return rc;
-  // group both lists by target cell: stable sort on the cb bits of T, S carried as the value
+  void *off_m2l, *off_p2p;
+  // group both lists by target cell: a stable sort of the (T << 32 | S) pairs on the cb bits of T
   int cb = 1;
   while ((1 << cb) < nn) ++cb;
-  if (n_m2l > 0)
-    hipLaunchKernelGGL(PairSplitKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (uint64_t*)m2l,
-                       (uint32_t*)v1);
-  if (n_p2p > 0)
-    hipLaunchKernelGGL(PairSplitKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (uint64_t*)p2p,
-                       (uint32_t*)v2);
-  AM_HIP(hipGetLastError());
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)m2l, (uint32_t*)v1, n_m2l, cb))) return rc;
-  if ((rc = RadixSortPairs(ctx, (uint64_t*)p2p, (uint32_t*)v2, n_p2p, cb))) return rc;
-  if (n_m2l > 0)
-    hipLaunchKernelGGL(PairJoinKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (uint64_t*)m2l,
-                       (const uint32_t*)v1);
-  if (n_p2p > 0)
-    hipLaunchKernelGGL(PairJoinKernel, dim3(Blocks(n_p2p, 256)), dim3(256), 0, st, n_p2p, (uint64_t*)p2p,
-                       (const uint32_t*)v2);
-  AM_HIP(hipGetLastError());
+  if ((rc = RadixSortKeys(ctx, (uint64_t*)m2l, n_m2l, 32, cb))) return rc;
+  if ((rc = RadixSortKeys(ctx, (uint64_t*)p2p, n_p2p, 32, cb))) return rc;
   if ((rc = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return rc;
   if ((rc = Scratch(ctx, "fmm_off_p2p", (size_t)(nn + 1) * 8, &off_p2p))) return rc;
   hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)m2l, n_m2l,

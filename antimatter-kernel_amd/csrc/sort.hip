@@ -232,8 +232,8 @@ __global__ void OsClearKernel(uint32_t* __restrict__ p, int64_t n) {
 // Keys often arrive grouped (lists in traversal order, narrow key ranges), and same-address LDS
 // atomics serialise, so each run of equal digits over consecutive lanes of a load is added once,
 // by its first lane.
-__global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __restrict__ keys, int64_t n, int passes,
-                                                           uint32_t* __restrict__ ghist) {
+__global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __restrict__ keys, int64_t n, int shift0,
+                                                           int passes, uint32_t* __restrict__ ghist) {
   __shared__ uint32_t h[kOsMaxPasses][256];
   const int t = threadIdx.x, lane = t & 63;
   for (int p = 0; p < passes; ++p) h[p][t] = 0u;
@@ -253,8 +253,9 @@ __global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __res
     const uint32_t phi = (uint32_t)__shfl_up((int)(uint32_t)(key[r] >> 32), 1);
     const uint64_t prev = ((uint64_t)phi << 32) | plo;
     for (int p = 0; p < passes; ++p) {
-      const uint32_t d = (uint32_t)(key[r] >> (8 * p)) & 255u;
-      const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> (8 * p)) & 255u));
+      const int sh = shift0 + 8 * p;
+      const uint32_t d = (uint32_t)(key[r] >> sh) & 255u;
+      const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> sh) & 255u));
       const uint64_t heads = __ballot(head);
       if (head) {
         const uint64_t above = lane == 63 ? 0ull : heads >> (lane + 1);
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(kOsThreads) void OsDigitBaseKernel(const uint32_t* 
   gbase[p * 256 + t] = OsBlockExclusiveScan(ghist[p * 256 + t], wsum);
 }
 
+template <bool VALS>  // values travel with the keys
 __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ gbase,
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
   for (int r = 0; r < kOsItems; ++r) {
     const int64_t idx = q0 + r * 64 + lane;
     key[r] = idx < n ? kin[idx] : 0ull;
-    val[r] = idx < n ? vin[idx] : 0u;
+    val[r] = (VALS && idx < n) ? vin[idx] : 0u;
   }
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   uint32_t* mycnt = wcnt[w];
@@ -386,16 +388,18 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
       if (dst[r] < (uint64_t)n) kout[dst[r]] = k;
     }
   }
-  __syncthreads();
-  uint32_t* sv = reinterpret_cast<uint32_t*>(sk);
+  if (VALS) {
+    __syncthreads();
+    uint32_t* sv = reinterpret_cast<uint32_t*>(sk);
 #pragma unroll
-  for (int r = 0; r < kOsItems; ++r)
-    if (q0 + r * 64 + lane < n) sv[sp[r]] = val[r];
-  __syncthreads();
+    for (int r = 0; r < kOsItems; ++r)
+      if (q0 + r * 64 + lane < n) sv[sp[r]] = val[r];
+    __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kOsItems; ++r) {
-    const int i = r * kOsThreads + t;
-    if (i < tile_n && dst[r] < (uint64_t)n) vout[dst[r]] = sv[i];
+    for (int r = 0; r < kOsItems; ++r) {
+      const int i = r * kOsThreads + t;
+      if (i < tile_n && dst[r] < (uint64_t)n) vout[dst[r]] = sv[i];
+    }
   }
 }
 
@@ -449,64 +453,88 @@ int ReserveSortScratch(am_ctx* ctx, int64_t n) {
   return AM_OK;
 }
 
+// The single-pass-per-digit sort of keys (and vals, when given) on bits [bit0, bit0 + 8 passes);
+// n < kOsVal.  The result ends in keys / vals.
+static int OneSweepSort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bit0, int passes) {
+  void *kbuf, *vbuf = nullptr, *os;
+  int rc;
+  if ((rc = Scratch(ctx, "sort_keys", (size_t)n * 8, &kbuf))) return rc;
+  if (vals && (rc = Scratch(ctx, "sort_vals", (size_t)n * 4, &vbuf))) return rc;
+  const int64_t tiles = (n + kOsTile - 1) / kOsTile;
+  const int64_t words = OsWords(n, passes);
+  if ((rc = Scratch(ctx, "sort_onesweep", (size_t)words * 4, &os))) return rc;
+  uint32_t* err = (uint32_t*)os;
+  uint32_t* ghist = err + 1;
+  uint32_t* gbase = ghist + passes * 256;
+  uint32_t* counters = gbase + passes * 256;
+  uint32_t* status = err + OsMetaWords(passes);
+  hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
+                     ctx->stream, (uint32_t*)os, words);
+  hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, bit0, passes,
+                     ghist);
+  hipLaunchKernelGGL(OsDigitBaseKernel, dim3(passes), dim3(kOsThreads), 0, ctx->stream, (const uint32_t*)ghist, gbase);
+  AM_HIP(hipGetLastError());
+  uint64_t *ka = keys, *kb = (uint64_t*)kbuf;
+  uint32_t *va = vals, *vb = (uint32_t*)vbuf;
+  for (int p = 0; p < passes; ++p) {
+    if (vals)
+      hipLaunchKernelGGL(OsScatterKernel<true>, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka, va, kb,
+                         vb, n, bit0 + 8 * p, gbase + 256 * p, counters + p, status + (size_t)p * tiles * 256, err);
+    else
+      hipLaunchKernelGGL(OsScatterKernel<false>, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka,
+                         (const uint32_t*)nullptr, kb, (uint32_t*)nullptr, n, bit0 + 8 * p, gbase + 256 * p,
+                         counters + p, status + (size_t)p * tiles * 256, err);
+    AM_HIP(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    AM_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if (vals) AM_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return AM_OK;
+}
+
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits) {
   if (n <= 1) return AM_OK;
   if (n > 0x7fffffffLL) return Fail(AM_ERR_CAPACITY, "radix sort: n too large");
-  void *kbuf, *vbuf;
+  const int passes = (std::max(1, std::min(64, key_bits)) + 7) / 8;
+  if (n < (int64_t)kOsVal) return OneSweepSort(ctx, keys, vals, n, 0, passes);
+  // per-tile histograms and a scan per pass (the status words of the single-pass sort count up to
+  // kOsVal)
+  void *kbuf, *vbuf, *hist, *offs;
   int rc;
   if ((rc = Scratch(ctx, "sort_keys", (size_t)n * 8, &kbuf))) return rc;
   if ((rc = Scratch(ctx, "sort_vals", (size_t)n * 4, &vbuf))) return rc;
-  uint64_t* ka = keys;
-  uint32_t* va = vals;
-  uint64_t* kb = (uint64_t*)kbuf;
-  uint32_t* vb = (uint32_t*)vbuf;
-  const int passes = (std::max(1, std::min(64, key_bits)) + 7) / 8;
-  if (n < (int64_t)kOsVal) {  // single-pass-per-digit sort (status words count up to kOsVal)
-    const int64_t tiles = (n + kOsTile - 1) / kOsTile;
-    const int64_t words = OsWords(n, passes);
-    void* os;
-    if ((rc = Scratch(ctx, "sort_onesweep", (size_t)words * 4, &os))) return rc;
-    uint32_t* err = (uint32_t*)os;
-    uint32_t* ghist = err + 1;
-    uint32_t* gbase = ghist + passes * 256;
-    uint32_t* counters = gbase + passes * 256;
-    uint32_t* status = err + OsMetaWords(passes);
-    hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (uint32_t*)os, words);
-    hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, passes, ghist);
-    hipLaunchKernelGGL(OsDigitBaseKernel, dim3(passes), dim3(kOsThreads), 0, ctx->stream, (const uint32_t*)ghist,
-                       gbase);
+  const int num_tiles = (int)((n + kSortTile - 1) / kSortTile);
+  if ((rc = Scratch(ctx, "sort_hist", (size_t)256 * num_tiles * 4, &hist))) return rc;
+  if ((rc = Scratch(ctx, "sort_offs", ((size_t)256 * num_tiles + 1) * 4, &offs))) return rc;
+  uint64_t *ka = keys, *kb = (uint64_t*)kbuf;
+  uint32_t *va = vals, *vb = (uint32_t*)vbuf;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(RadixHistKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, n, shift,
+                       (uint32_t*)hist, num_tiles);
     AM_HIP(hipGetLastError());
-    for (int p = 0; p < passes; ++p) {
-      hipLaunchKernelGGL(OsScatterKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka, va, kb, vb, n,
-                         8 * p, gbase + 256 * p, counters + p, status + (size_t)p * tiles * 256, err);
-      AM_HIP(hipGetLastError());
-      std::swap(ka, kb);
-      std::swap(va, vb);
-    }
-  } else {  // per-tile histograms and a scan per pass
-    const int num_tiles = (int)((n + kSortTile - 1) / kSortTile);
-    void *hist, *offs;
-    if ((rc = Scratch(ctx, "sort_hist", (size_t)256 * num_tiles * 4, &hist))) return rc;
-    if ((rc = Scratch(ctx, "sort_offs", ((size_t)256 * num_tiles + 1) * 4, &offs))) return rc;
-    for (int p = 0; p < passes; ++p) {
-      const int shift = 8 * p;
-      hipLaunchKernelGGL(RadixHistKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, n, shift,
-                         (uint32_t*)hist, num_tiles);
-      AM_HIP(hipGetLastError());
-      if ((rc = ExclusiveScan(ctx, (const uint32_t*)hist, (uint32_t*)offs, (int64_t)256 * num_tiles))) return rc;
-      hipLaunchKernelGGL(RadixScatterKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, va, kb, vb, n,
-                         shift, (const uint32_t*)offs, num_tiles);
-      AM_HIP(hipGetLastError());
-      std::swap(ka, kb);
-      std::swap(va, vb);
-    }
+    if ((rc = ExclusiveScan(ctx, (const uint32_t*)hist, (uint32_t*)offs, (int64_t)256 * num_tiles))) return rc;
+    hipLaunchKernelGGL(RadixScatterKernel, dim3(num_tiles), dim3(kSortThreads), 0, ctx->stream, ka, va, kb, vb, n,
+                       shift, (const uint32_t*)offs, num_tiles);
+    AM_HIP(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
   }
   if (ka != keys) {
     AM_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
     AM_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream));
   }
   return AM_OK;
+}
+
+int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits) {
+  if (n <= 1) return AM_OK;
+  if (n >= (int64_t)kOsVal) return Fail(AM_ERR_CAPACITY, "radix sort of keys: n too large");
+  if (bit0 < 0 || bit0 + key_bits > 64 || key_bits < 1) return Fail(AM_ERR_INVALID, "radix sort of keys: bad bit range");
+  return OneSweepSort(ctx, keys, nullptr, n, bit0, (key_bits + 7) / 8);
 }
 
 }  // namespace am
