@@ -788,12 +788,15 @@ __global__ void MultipoleToF32Kernel(int nn, int nm, int nkp, const double* __re
 // Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is sorted on (a
 // stable sort of its few low bits); the source cell rides in the key's upper half, so the product
 // kernel reads it with the class instead of gathering it from the list; the pair index is the value.
-__global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair_cls, uint32_t n_cls,
+// pair_slot holds table slots (kNoClass for unclassed pairs); ids maps a slot to its class id.
+__global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair_slot,
+                                   const uint32_t* __restrict__ ids, uint32_t n_cls,
                                    const uint64_t* __restrict__ list, uint64_t* __restrict__ keys,
                                    uint32_t* __restrict__ vals) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
-  const uint32_t c = pair_cls[q] == kNoClass ? n_cls : pair_cls[q];
+  const uint32_t slot = pair_slot[q];
+  const uint32_t c = slot == kNoClass ? n_cls : ids[slot];
   keys[q] = ((uint64_t)(uint32_t)list[q] << 32) | c;
   vals[q] = (uint32_t)q;
 }
@@ -1556,7 +1559,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     const size_t m2l_lds =
         sizeof(double) * ((size_t)kM2LWaves * (tab.nd + tab.nm) + tab.nm) +
         (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
-    if (n_m2l > 0)
+    const bool mfma = precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0;
+    if (n_m2l > 0 && !mfma)  // the FP64 contraction reads class ids; the MFMA path reads slots
       hipLaunchKernelGGL(PairClassKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (const uint32_t*)pslot,
                          (const uint32_t*)ids, (uint32_t*)pslot);  // in place: slot -> class id
     const int dpl = (tab.nd + 63) / 64;
@@ -1565,7 +1569,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                          (const Cell*)cells, (const uint64_t*)m2l, (const int64_t*)off_m2l, (const uint32_t*)pslot,
                          (const double*)Dc, tab, (const double*)M, (double*)L);
     };
-    if (precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0) {
+    if (mfma) {
       void *Dc32, *M32, *ckeys, *cvals, *outp;
       if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
       const int nrb = tab.nm <= 36 ? 3 : 4, nks = tab.nm <= 36 ? 9 : 14;  // the instance below
@@ -1582,7 +1586,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nkp, 256)), dim3(256), 0, st, nn,
                          tab.nm, nkp, (const double*)M, tab.sign, (float*)M32);
       hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                         (const uint32_t*)pslot, n_cls, (const uint64_t*)m2l, (uint64_t*)ckeys, (uint32_t*)cvals);
+                         (const uint32_t*)pslot, (const uint32_t*)ids, n_cls, (const uint64_t*)m2l, (uint64_t*)ckeys,
+                         (uint32_t*)cvals);
       AM_HIP(hipGetLastError());
       int cls_bits = 1;
       while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
