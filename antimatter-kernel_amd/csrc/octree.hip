@@ -45,6 +45,7 @@ int GatherParticles(am_ctx* ctx, int64_t n, const uint8_t* flags, const double* 
 
 constexpr int kBoxThreads = 256;
 constexpr int kBoxBlocks = 256;
+static_assert(kBoxBlocks <= kBoxThreads, "BBoxFinalKernel takes one partial per thread");
 
 __global__ __launch_bounds__(kBoxThreads) void BBoxPartialKernel(int m, const uint32_t* __restrict__ idx,
                                                                  const double* __restrict__ x,
@@ -77,17 +78,26 @@ __global__ __launch_bounds__(kBoxThreads) void BBoxPartialKernel(int m, const ui
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void BBoxFinalKernel(int blocks, const double* __restrict__ part, double* __restrict__ box) {
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < blocks; ++b)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], part[b * 6 + d]);
-      hi[d] = fmax(hi[d], part[b * 6 + 3 + d]);
-    }
+// One workgroup of kBoxThreads: thread b takes block b's partial box, then a tree in LDS (min and
+// max do not depend on the order).
+__global__ __launch_bounds__(kBoxThreads) void BBoxFinalKernel(int blocks, const double* __restrict__ part,
+                                                               double* __restrict__ box) {
+  __shared__ double sh[6][kBoxThreads];
+  const int t = threadIdx.x;
   for (int d = 0; d < 3; ++d) {
-    box[d] = lo[d];
-    box[3 + d] = hi[d];
+    sh[d][t] = t < blocks ? part[t * 6 + d] : INFINITY;
+    sh[3 + d][t] = t < blocks ? part[t * 6 + 3 + d] : -INFINITY;
   }
+  __syncthreads();
+  for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int d = 0; d < 3; ++d) {
+        sh[d][t] = fmin(sh[d][t], sh[d][t + w]);
+        sh[3 + d][t] = fmax(sh[3 + d][t], sh[3 + d][t + w]);
+      }
+    __syncthreads();
+  }
+  if (t < 6) box[t] = sh[t][0];
 }
 
 int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const double* y, const double* z,
@@ -98,7 +108,7 @@ int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const
   const int bb = std::min(kBoxBlocks, Blocks(m, kBoxThreads));
   hipLaunchKernelGGL(BBoxPartialKernel, dim3(bb), dim3(kBoxThreads), 0, ctx->stream, m, idx, x, y, z, (double*)part);
   AM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(BBoxFinalKernel, dim3(1), dim3(1), 0, ctx->stream, bb, (const double*)part, box_dev);
+  hipLaunchKernelGGL(BBoxFinalKernel, dim3(1), dim3(kBoxThreads), 0, ctx->stream, bb, (const double*)part, box_dev);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -130,6 +140,29 @@ __global__ void MortonKernel(int m, const uint32_t* __restrict__ idx, const doub
 
 __device__ inline int Digit(uint64_t key, int level) {
   return (int)((key >> (3 * (kOctreeLevels - 1 - level))) & 7u);
+}
+
+// First position in [lo, hi) whose digit at depth d is >= o (hi if none); the digits are
+// non-decreasing over the range.  While the range is long, each step probes 15 evenly spaced
+// positions with independent loads and keeps the sixteenth in which the digit crosses o; binary
+// steps finish.
+__device__ inline int FirstDigitAtLeast(const uint64_t* __restrict__ keys, int lo, int hi, int d, int o) {
+  while (hi - lo > 64) {
+    const int step = (hi - lo) / 16;
+    int c = 0;  // probes lo + k step (k = 1..15) with a smaller digit: a prefix of them
+#pragma unroll
+    for (int k = 1; k < 16; ++k) c += Digit(keys[lo + k * step], d) < o ? 1 : 0;
+    const int nlo = c > 0 ? lo + c * step + 1 : lo;
+    const int nhi = c < 15 ? lo + (c + 1) * step : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (Digit(keys[mid], d) < o) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
 // The root, and the first two level starts (depth 0 is node 0, depth 1 starts at node 1).
@@ -190,15 +223,7 @@ __global__ __launch_bounds__(kLvThreads) void OctreeLevelKernel(int d, int* __re
     split = e - b > leaf_cap && nd.depth[node] < kOctreeLevels;
     if (split) {
       bd[0] = b;
-      for (int o = 1; o < 8; ++o) {  // first position whose digit at depth d is >= o
-        int lo = bd[o - 1], hi = e;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (Digit(keys[mid], d) < o) lo = mid + 1;
-          else hi = mid;
-        }
-        bd[o] = lo;
-      }
+      for (int o = 1; o < 8; ++o) bd[o] = FirstDigitAtLeast(keys, bd[o - 1], e, d, o);
       bd[8] = e;
       for (int o = 0; o < 8; ++o) c += bd[o + 1] > bd[o] ? 1u : 0u;
     }
