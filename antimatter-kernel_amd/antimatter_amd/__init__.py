@@ -123,6 +123,7 @@ SIGNATURES = {
     "am_compute_barnes_hut_learned_forces": (_int, [_vp, _d, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp]),
     "am_debug_exclusive_scan": (_int, [_vp, _vp, _i64, _vp]),
     "am_debug_radix_sort": (_int, [_vp, _vp, _vp, _i64, _int]),
+    "am_debug_radix_sort_keys": (_int, [_vp, _vp, _i64, _int, _int]),
 }
 
 _lib = None
@@ -218,6 +219,12 @@ class Context:
         v = np.array(vals, dtype=np.uint32)
         _check(lib().am_debug_radix_sort(self.h, _p(k), _p(v), len(k), int(key_bits)))
         return k, v
+
+    def radix_sort_keys(self, keys, bit0, key_bits):
+        """Stable sort of 64-bit keys on bits [bit0, bit0 + key_bits) alone (device)."""
+        k = np.array(keys, dtype=np.uint64)
+        _check(lib().am_debug_radix_sort_keys(self.h, _p(k), len(k), int(bit0), int(key_bits)))
+        return k
 
     def enable_timing(self, on: bool = True):
         _check(lib().am_ctx_enable_timing(self.h, int(on)))

@@ -178,8 +178,9 @@ __device__ __forceinline__ void PairF32(const float4 th, const float4 tl, const 
 int RefreshSource(am_store* s);  // rebuild store->src from the FP64 state
 int64_t CountActive(am_store* s, int* status);
 
-// Radix sort (stable, LSD) of 64-bit keys with 32-bit values.  keys/vals are sorted in place
-// (ping-pong through scratch).  key_bits: number of low bits that may be non-zero.
+// Radix sort (stable, LSD) of 64-bit keys with 32-bit values on the keys' low key_bits bits (higher
+// bits are carried, not compared; from 2^30 keys on they must be zero up to the next multiple of 8).
+// keys/vals are sorted in place (ping-pong through scratch).
 int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
 // Stable radix sort of 64-bit keys alone on bits [bit0, bit0 + key_bits), in place; n < 2^30.
 int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);

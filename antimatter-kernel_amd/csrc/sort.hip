@@ -232,8 +232,15 @@ __global__ void OsClearKernel(uint32_t* __restrict__ p, int64_t n) {
 // Keys often arrive grouped (lists in traversal order, narrow key ranges), and same-address LDS
 // atomics serialise, so each run of equal digits over consecutive lanes of a load is added once,
 // by its first lane.
+// Digit mask of pass p of a sort on key_bits bits: 8 bits, fewer in the last pass.
+__device__ __host__ inline uint32_t OsDigitMask(int p, int key_bits) {
+  const int b = key_bits - 8 * p;
+  return b >= 8 ? 255u : (1u << b) - 1u;
+}
+
 __global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __restrict__ keys, int64_t n, int shift0,
-                                                           int passes, uint32_t* __restrict__ ghist) {
+                                                           int key_bits, uint32_t* __restrict__ ghist) {
+  const int passes = (key_bits + 7) / 8;
   __shared__ uint32_t h[kOsMaxPasses][256];
   const int t = threadIdx.x, lane = t & 63;
   for (int p = 0; p < passes; ++p) h[p][t] = 0u;
@@ -254,8 +261,9 @@ __global__ __launch_bounds__(kOsThreads) void OsHistKernel(const uint64_t* __res
     const uint64_t prev = ((uint64_t)phi << 32) | plo;
     for (int p = 0; p < passes; ++p) {
       const int sh = shift0 + 8 * p;
-      const uint32_t d = (uint32_t)(key[r] >> sh) & 255u;
-      const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> sh) & 255u));
+      const uint32_t dm = OsDigitMask(p, key_bits);
+      const uint32_t d = (uint32_t)(key[r] >> sh) & dm;
+      const bool head = valid && (lane == 0 || d != ((uint32_t)(prev >> sh) & dm));
       const uint64_t heads = __ballot(head);
       if (head) {
         const uint64_t above = lane == 63 ? 0ull : heads >> (lane + 1);
@@ -280,7 +288,7 @@ __global__ __launch_bounds__(kOsThreads) void OsDigitBaseKernel(const uint32_t* 
 template <bool VALS>  // values travel with the keys
 __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ gbase,
+    uint32_t* __restrict__ vout, int64_t n, int shift, uint32_t dmask, const uint32_t* __restrict__ gbase,
     uint32_t* __restrict__ counter, uint32_t* __restrict__ status, uint32_t* __restrict__ err) {
   __shared__ uint64_t sk[kOsTile];           // keys, then values, in digit order
   __shared__ uint32_t wcnt[kOsWaves][256];  // per-wave digit counts, then per-wave digit offsets
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
 #pragma unroll
   for (int r = 0; r < kOsItems; ++r) {
     const bool valid = q0 + r * 64 + lane < n;
-    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & dmask;
     uint64_t mask = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -370,7 +378,7 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
   uint32_t sp[kOsItems];
 #pragma unroll
   for (int r = 0; r < kOsItems; ++r) {
-    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & dmask;
     sp[r] = loc[d] + wcnt[w][d] + rk[r];
     if (q0 + r * 64 + lane < n) sk[sp[r]] = key[r];
   }
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
     dst[r] = 0xffffffffu;
     if (i < tile_n) {
       const uint64_t k = sk[i];
-      const uint32_t d = (uint32_t)(k >> shift) & 255u;
+      const uint32_t d = (uint32_t)(k >> shift) & dmask;
       dst[r] = glob[d] + (uint32_t)i - loc[d];
       if (dst[r] < (uint64_t)n) kout[dst[r]] = k;
     }
@@ -455,7 +463,8 @@ int ReserveSortScratch(am_ctx* ctx, int64_t n) {
 
 // The single-pass-per-digit sort of keys (and vals, when given) on bits [bit0, bit0 + 8 passes);
 // n < kOsVal.  The result ends in keys / vals.
-static int OneSweepSort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bit0, int passes) {
+static int OneSweepSort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bit0, int key_bits) {
+  const int passes = (key_bits + 7) / 8;
   void *kbuf, *vbuf = nullptr, *os;
   int rc;
   if ((rc = Scratch(ctx, "sort_keys", (size_t)n * 8, &kbuf))) return rc;
@@ -470,19 +479,20 @@ static int OneSweepSort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
   uint32_t* status = err + OsMetaWords(passes);
   hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
                      ctx->stream, (uint32_t*)os, words);
-  hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, bit0, passes,
+  hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, bit0, key_bits,
                      ghist);
   hipLaunchKernelGGL(OsDigitBaseKernel, dim3(passes), dim3(kOsThreads), 0, ctx->stream, (const uint32_t*)ghist, gbase);
   AM_HIP(hipGetLastError());
   uint64_t *ka = keys, *kb = (uint64_t*)kbuf;
   uint32_t *va = vals, *vb = (uint32_t*)vbuf;
   for (int p = 0; p < passes; ++p) {
+    const uint32_t dm = OsDigitMask(p, key_bits);
     if (vals)
       hipLaunchKernelGGL(OsScatterKernel<true>, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka, va, kb,
-                         vb, n, bit0 + 8 * p, gbase + 256 * p, counters + p, status + (size_t)p * tiles * 256, err);
+                         vb, n, bit0 + 8 * p, dm, gbase + 256 * p, counters + p, status + (size_t)p * tiles * 256, err);
     else
       hipLaunchKernelGGL(OsScatterKernel<false>, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, ka,
-                         (const uint32_t*)nullptr, kb, (uint32_t*)nullptr, n, bit0 + 8 * p, gbase + 256 * p,
+                         (const uint32_t*)nullptr, kb, (uint32_t*)nullptr, n, bit0 + 8 * p, dm, gbase + 256 * p,
                          counters + p, status + (size_t)p * tiles * 256, err);
     AM_HIP(hipGetLastError());
     std::swap(ka, kb);
@@ -499,7 +509,7 @@ int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int k
   if (n <= 1) return AM_OK;
   if (n > 0x7fffffffLL) return Fail(AM_ERR_CAPACITY, "radix sort: n too large");
   const int passes = (std::max(1, std::min(64, key_bits)) + 7) / 8;
-  if (n < (int64_t)kOsVal) return OneSweepSort(ctx, keys, vals, n, 0, passes);
+  if (n < (int64_t)kOsVal) return OneSweepSort(ctx, keys, vals, n, 0, std::max(1, std::min(64, key_bits)));
   // per-tile histograms and a scan per pass (the status words of the single-pass sort count up to
   // kOsVal)
   void *kbuf, *vbuf, *hist, *offs;
@@ -534,7 +544,7 @@ int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits
   if (n <= 1) return AM_OK;
   if (n >= (int64_t)kOsVal) return Fail(AM_ERR_CAPACITY, "radix sort of keys: n too large");
   if (bit0 < 0 || bit0 + key_bits > 64 || key_bits < 1) return Fail(AM_ERR_INVALID, "radix sort of keys: bad bit range");
-  return OneSweepSort(ctx, keys, nullptr, n, bit0, (key_bits + 7) / 8);
+  return OneSweepSort(ctx, keys, nullptr, n, bit0, key_bits);
 }
 
 }  // namespace am
@@ -574,6 +584,22 @@ int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
     AM_HIP(hipMemcpyAsync(&err, ctx->scratch["sort_onesweep"].ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
   if (err) return Fail(AM_ERR_HIP, "am_debug_radix_sort: a look-back wait on the device gave up");
+  return AM_OK;
+}
+
+int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits) {
+  if (!ctx || n < 0 || (n > 0 && !keys)) return Fail(AM_ERR_INVALID, "am_debug_radix_sort_keys: bad args");
+  if (n == 0) return AM_OK;
+  void* dk;
+  int rc;
+  if ((rc = Scratch(ctx, "dbg_keys", (size_t)n * 8, &dk))) return rc;
+  AM_HIP(hipMemcpyAsync(dk, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = RadixSortKeys(ctx, (uint64_t*)dk, n, bit0, key_bits))) return rc;
+  AM_HIP(hipMemcpyAsync(keys, dk, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t err = 0;
+  if (n > 1) AM_HIP(hipMemcpyAsync(&err, ctx->scratch["sort_onesweep"].ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipStreamSynchronize(ctx->stream));
+  if (err) return Fail(AM_ERR_HIP, "am_debug_radix_sort_keys: a look-back wait on the device gave up");
   return AM_OK;
 }
 

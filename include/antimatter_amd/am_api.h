@@ -281,6 +281,9 @@ int am_compute_barnes_hut_learned_forces(am_store* store, double theta, am_mlp* 
 int am_debug_exclusive_scan(am_ctx* ctx, const uint32_t* in, int64_t n, uint32_t* out);
 /* Stable LSD radix sort of (key, value) pairs on the device, in place on the host arrays. */
 int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int key_bits);
+/* Stable radix sort of keys alone on bits [bit0, bit0 + key_bits) on the device, in place on the host
+ * array; n < 2^30. */
+int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);
 
 #ifdef __cplusplus
 }

@@ -70,3 +70,23 @@ def test_radix_sort_low_bits_only_with_payload_above(ctx):
     order = np.argsort(low, kind="stable")
     np.testing.assert_array_equal(k, keys[order])
     np.testing.assert_array_equal(v, vals[order])
+
+
+@pytest.mark.parametrize("n,bit0,bits", [(2, 32, 17), (4096, 32, 8), (300_007, 32, 17), (100_001, 5, 23), (65537, 56, 8)])
+def test_radix_sort_keys_on_a_bit_range(ctx, n, bit0, bits):
+    # Keys sorted stably on bits [bit0, bit0 + bits) alone: pairs (T << 32 | S) grouped by T keep
+    # their input order within a group.
+    rng = np.random.default_rng(13 + n)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    mask = np.uint64((1 << bits) - 1)
+    field = (keys >> np.uint64(bit0)) & mask
+    k = ctx.radix_sort_keys(keys, bit0, bits)
+    np.testing.assert_array_equal(k, keys[np.argsort(field, kind="stable")])
+
+
+def test_radix_sort_keys_rejects_bad_ranges(ctx):
+    import antimatter_amd as am
+    keys = np.arange(10, dtype=np.uint64)
+    for bit0, bits in ((60, 8), (-1, 4), (0, 0)):
+        with pytest.raises(am.AmError):
+            ctx.radix_sort_keys(keys, bit0, bits)
