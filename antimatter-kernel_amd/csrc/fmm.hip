@@ -260,7 +260,8 @@ __global__ void RadiusKernel(int n0, int n1, NodeArrays nd, const double4* __res
 
 __global__ void GatherCellsKernel(int nn, NodeArrays nd, const uint32_t* __restrict__ pre,
                                   const uint32_t* __restrict__ bfs2pre, const double* __restrict__ rad,
-                                  Cell* __restrict__ cells) {
+                                  Cell* __restrict__ cells, double4* __restrict__ geo, double4* __restrict__ trv,
+                                  uint32_t* __restrict__ tinfo) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nn) return;
   const int node = (int)pre[p];
@@ -276,6 +277,9 @@ __global__ void GatherCellsKernel(int nn, NodeArrays nd, const uint32_t* __restr
   c.nchild = nd.nchild[node];
   for (int k = 0; k < 8; ++k) c.child[k] = k < c.nchild ? (int)bfs2pre[nd.first_child[node] + k] : -1;
   cells[p] = c;
+  geo[p] = make_double4(c.cx, c.cy, c.cz, c.half);
+  trv[p] = make_double4(c.cx, c.cy, c.cz, c.radius);
+  tinfo[p] = (c.leaf ? 1u : 0u) | ((uint32_t)c.nchild << 1);
 }
 
 __device__ inline double Monomial(const FmmTab& t, int k, double x, double y, double z) {
@@ -394,33 +398,37 @@ __global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uin
 // One expansion step of the dual-tree traversal (fast-multipole.cc:494-525).  Pair i of the
 // frontier is accepted for M2L, kept for P2P, or split; cnt[i], cnt[np + i] and cnt[2 np + i] count
 // its M2L pair, its P2P pair and its child pairs, so one scan over cnt places all three outputs.
-__global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
-                                    double theta2, uint32_t* __restrict__ cnt) {
+// The decision reads the traversal records trv[c] = (centre, radius) and tinfo[c] = leaf | nchild << 1
+// (72 bytes per pair instead of two whole cell records); children come from the cell record only
+// for split pairs.
+__device__ inline int TraverseKind(const double4 T, const double4 S, uint32_t ti, uint32_t si, double theta2) {
+  const double dx = T.x - S.x, dy = T.y - S.y, dz = T.z - S.z;
+  const double dist2 = dx * dx + dy * dy + dz * dz;
+  const double rs = T.w + S.w;
+  if (rs * rs < theta2 * dist2) return 0;  // M2L
+  const bool t_leaf = ti & 1u, s_leaf = si & 1u;
+  if (t_leaf && s_leaf) return 1;  // P2P
+  return (s_leaf || (!t_leaf && T.w >= S.w)) ? 2 : 3;  // split the target / the source
+}
+
+__global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, const double4* __restrict__ trv,
+                                    const uint32_t* __restrict__ tinfo, double theta2, uint32_t* __restrict__ cnt) {
   uint32_t* n_m2l = cnt;
   uint32_t* n_p2p = cnt + np;
   uint32_t* n_new = cnt + 2 * (size_t)np;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint2 pr = pairs[i];
-  const Cell& T = cells[pr.x];
-  const Cell& S = cells[pr.y];
-  const double dx = T.cx - S.cx, dy = T.cy - S.cy, dz = T.cz - S.cz;
-  const double dist2 = dx * dx + dy * dy + dz * dz;
-  const double rs = T.radius + S.radius;
-  uint32_t a = 0, b = 0, c = 0;
-  if (rs * rs < theta2 * dist2) a = 1;
-  else if (T.leaf && S.leaf) b = 1;
-  else {
-    const bool split_t = S.leaf || (!T.leaf && T.radius >= S.radius);
-    c = split_t ? (uint32_t)T.nchild : (uint32_t)S.nchild;
-  }
-  n_m2l[i] = a;
-  n_p2p[i] = b;
-  n_new[i] = c;
+  const uint32_t ti = tinfo[pr.x], si = tinfo[pr.y];
+  const int kind = TraverseKind(trv[pr.x], trv[pr.y], ti, si, theta2);
+  n_m2l[i] = kind == 0 ? 1u : 0u;
+  n_p2p[i] = kind == 1 ? 1u : 0u;
+  n_new[i] = kind == 2 ? (ti >> 1) : (kind == 3 ? (si >> 1) : 0u);
 }
 
 // off = exclusive scan of TraverseCountKernel's cnt (3 np + 1 entries).
-__global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, const Cell* __restrict__ cells,
+__global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, const double4* __restrict__ trv,
+                                    const uint32_t* __restrict__ tinfo, const Cell* __restrict__ cells,
                                     double theta2, const uint32_t* __restrict__ off, uint64_t* __restrict__ m2l_out,
                                     uint64_t* __restrict__ p2p_out, uint2* __restrict__ next) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -430,23 +438,21 @@ __global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, con
   const uint32_t* o_p2p = off + np;
   const uint32_t* o_new = off + 2 * (size_t)np;
   const uint2 pr = pairs[i];
-  const Cell& T = cells[pr.x];
-  const Cell& S = cells[pr.y];
-  const double dx = T.cx - S.cx, dy = T.cy - S.cy, dz = T.cz - S.cz;
-  const double dist2 = dx * dx + dy * dy + dz * dz;
-  const double rs = T.radius + S.radius;
+  const int kind = TraverseKind(trv[pr.x], trv[pr.y], tinfo[pr.x], tinfo[pr.y], theta2);
   const uint64_t key = ((uint64_t)pr.x << 32) | pr.y;
-  if (rs * rs < theta2 * dist2) {
+  if (kind == 0) {
     m2l_out[o_m2l[i]] = key;
-  } else if (T.leaf && S.leaf) {
+  } else if (kind == 1) {
     p2p_out[o_p2p[i] - b2] = key;
   } else {
-    const bool split_t = S.leaf || (!T.leaf && T.radius >= S.radius);
     uint2* out = next + (o_new[i] - b3);
-    if (split_t)
+    if (kind == 2) {
+      const Cell& T = cells[pr.x];
       for (int k = 0; k < T.nchild; ++k) out[k] = make_uint2((uint32_t)T.child[k], pr.y);
-    else
+    } else {
+      const Cell& S = cells[pr.y];
       for (int k = 0; k < S.nchild; ++k) out[k] = make_uint2(pr.x, (uint32_t)S.child[k]);
+    }
   }
 }
 
@@ -495,12 +501,13 @@ __device__ inline void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Key (d: 5 bits, q: 3 x 19 bits) of the displacement T - S; false if it does not fit.
-__device__ inline bool M2LClassKey(const Cell& T, const Cell& S, int root_exp, uint64_t* key) {
-  const double hf = fmin(T.half, S.half);
+// Key (d: 5 bits, q: 3 x 19 bits) of the displacement T - S; false if it does not fit.  T and S are
+// (centre, half width) records.
+__device__ inline bool M2LClassKey(const double4 T, const double4 S, int root_exp, uint64_t* key) {
+  const double hf = fmin(T.w, S.w);
   const int d = root_exp - ilogb(hf);
   if (d < 0 || d > 31) return false;
-  const double qd[3] = {(T.cx - S.cx) / hf, (T.cy - S.cy) / hf, (T.cz - S.cz) / hf};
+  const double qd[3] = {(T.x - S.x) / hf, (T.y - S.y) / hf, (T.z - S.z) / hf};
   uint64_t k = (uint64_t)d;
   for (int a = 0; a < 3; ++a) {
     const double r = rint(qd[a]);
@@ -525,16 +532,17 @@ __global__ void ClassClearKernel(uint64_t size, uint64_t* __restrict__ hkeys) {
   if (h < size) hkeys[h] = kClassEmpty;
 }
 
-// Inserts every pair's key; pair_slot[q] = its table slot, or kNoClass.
-__global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pairs, const Cell* __restrict__ cells,
+// Inserts every pair's key; pair_slot[q] = its table slot, or kNoClass.  geo holds each cell's
+// (centre, half width): 32 bytes instead of the whole cell record per lookup.
+__global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pairs, const double4* __restrict__ geo,
                                      uint64_t* __restrict__ hkeys, uint64_t mask, uint32_t* __restrict__ pair_slot) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
   const uint64_t pk = pairs[q];
-  const int root_exp = ilogb(cells[0].half);
+  const int root_exp = ilogb(geo[0].w);
   uint64_t key;
   uint32_t slot = kNoClass;
-  if (M2LClassKey(cells[pk >> 32], cells[(uint32_t)pk], root_exp, &key)) {
+  if (M2LClassKey(geo[pk >> 32], geo[(uint32_t)pk], root_exp, &key)) {
     uint64_t h = CellHashMix(key) & mask;
     for (int probe = 0; probe < kClassProbes; ++probe) {
       const uint64_t cur = hkeys[h];
@@ -1415,6 +1423,10 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   if ((rc = Scratch(ctx, "fmm_sp", (size_t)m * sizeof(double4), &sp))) return rc;
   if ((rc = Scratch(ctx, "fmm_rad", (size_t)nn * 8, &rad))) return rc;
   if ((rc = Scratch(ctx, "fmm_cells", (size_t)nn * sizeof(Cell), &cells))) return rc;
+  void *geo, *trv, *tinfo;
+  if ((rc = Scratch(ctx, "fmm_cellgeo", (size_t)nn * sizeof(double4), &geo))) return rc;
+  if ((rc = Scratch(ctx, "fmm_celltrv", (size_t)nn * sizeof(double4), &trv))) return rc;
+  if ((rc = Scratch(ctx, "fmm_cellinfo", (size_t)nn * 4, &tinfo))) return rc;
   if ((rc = Scratch(ctx, "fmm_M", (size_t)nn * tab.nm * 8, &M))) return rc;
   if ((rc = Scratch(ctx, "fmm_L", (size_t)nn * tab.nm * 8, &L))) return rc;
   hipLaunchKernelGGL(SortedCopyKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
@@ -1425,7 +1437,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                        (const double4*)sp, (double*)rad);
   }
   hipLaunchKernelGGL(GatherCellsKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, o.nd, (const uint32_t*)o.pre,
-                     (const uint32_t*)o.bfs2pre, (const double*)rad, (Cell*)cells);
+                     (const uint32_t*)o.bfs2pre, (const double*)rad, (Cell*)cells, (double4*)geo, (double4*)trv,
+                     (uint32_t*)tinfo);
   AM_HIP(hipGetLastError());
   int slot;
   TimerBegin(ctx, "fmm_upward", &slot);
@@ -1463,7 +1476,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = Scratch(ctx, "fmm_trav_off", ((size_t)np * 3 + 1) * 4, &off3))) return rc;
     if ((rc = Scratch(ctx, "fmm_trav_tot", 16, &tot_dev))) return rc;
     hipLaunchKernelGGL(TraverseCountKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const Cell*)cells, theta2, (uint32_t*)cnt3);
+                       (const double4*)trv, (const uint32_t*)tinfo, theta2, (uint32_t*)cnt3);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt3, (uint32_t*)off3, 3 * np))) return rc;
     hipLaunchKernelGGL(TraverseTotalsKernel, dim3(1), dim3(64), 0, st, (int)np, (const uint32_t*)off3,
@@ -1501,8 +1514,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = grow("fmm_p2p", &p2p, cap_p2p, (size_t)n_p2p + tot[1], (size_t)n_p2p, 8))) return rc;
     if ((rc = grow("fmm_front_b", &fb, cap_fb, tot[2], 0, 8))) return rc;
     hipLaunchKernelGGL(TraverseWriteKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const Cell*)cells, theta2, (const uint32_t*)off3, (uint64_t*)m2l + n_m2l,
-                       (uint64_t*)p2p + n_p2p, (uint2*)fb);
+                       (const double4*)trv, (const uint32_t*)tinfo, (const Cell*)cells, theta2,
+                       (const uint32_t*)off3, (uint64_t*)m2l + n_m2l, (uint64_t*)p2p + n_p2p, (uint2*)fb);
     AM_HIP(hipGetLastError());
     n_m2l += tot[0];
     n_p2p += tot[1];
@@ -1540,7 +1553,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     hipLaunchKernelGGL(ClassClearKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize, (uint64_t*)hk);
     if (n_m2l > 0)
       hipLaunchKernelGGL(M2LClassInsertKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                         (const uint64_t*)m2l, (const Cell*)cells, (uint64_t*)hk, tsize - 1, (uint32_t*)pslot);
+                         (const uint64_t*)m2l, (const double4*)geo, (uint64_t*)hk, tsize - 1, (uint32_t*)pslot);
     hipLaunchKernelGGL(ClassOccupiedKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize,
                        (const uint64_t*)hk, (uint32_t*)occ);
     AM_HIP(hipGetLastError());
