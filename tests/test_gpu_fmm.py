@@ -88,10 +88,13 @@ def test_fmm_fp32_near_field(store):
     assert rel_l2(store.forces(), oracle.allpairs_forces(pos, q)) < 1e-5
 
 
-@pytest.mark.parametrize("order,theta", [(1, 0.5), (3, 0.6), (6, 0.4), (8, 0.5)])
+@pytest.mark.parametrize("order,theta", [(1, 0.5), (2, 0.5), (3, 0.6), (4, 0.5), (5, 0.5), (6, 0.4), (7, 0.5),
+                                         (8, 0.5)])
 def test_fmm_fp32_orders_with_dead_and_neutral(store, order, theta):
-    """The FP32 near field for every order its kernel is built for, with annihilated and neutral
-    particles (sources of charge 0 in the near lists) and leaves of every occupancy class."""
+    """The FP32 path for every order: the near field, and the M2L on matrix cores for orders <= 5
+    (36-coefficient rows up to order 4, 56 at order 5; the FP64 contraction above), with annihilated
+    and neutral particles (sources of charge 0 in the near lists) and leaves of every occupancy
+    class."""
     pos, sp, ann = cloud(7000, seed=40 + order)
     store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
     store.compute_fast_multipole_coulomb_forces(order, theta, am.FP32)
