@@ -390,7 +390,12 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         }
       }
     }
-    c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : WaveMin(next));
+    // The wave's next node is the smallest lane next.  A lane reaches a node inside c's subtree
+    // only by opening c, and the wave visits c once (its node only increases), so every lane
+    // that was not at c already points at skip(c) or beyond, and every lane at c now points at
+    // c + 1 (opened) or skip(c).  The minimum is therefore c + 1 if any lane opened c, else
+    // skip(c): no reduction over the lanes is needed.
+    c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : nd.skip);
   }
   if (!valid) return;
   if (rec_cnt) rec_cnt[k] = nrec;
