@@ -186,6 +186,11 @@ int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int k
 int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);
 // Deterministic FP64 sum of n device doubles into *host_out (synchronises the stream).
 int ReduceSum(am_ctx* ctx, const double* in, int64_t n, double* host_out);
+// The radix sort's look-back error word on the device (sticky: set by a wait that gave up, never
+// cleared); callers read it with a synchronisation they make anyway and pass it to
+// CheckSortErrorAfterSync.
+uint32_t* SortErrorWord(am_ctx* ctx);
+int CheckSortErrorAfterSync(uint32_t word);
 // Pre-sizes the scratch of RadixSortPairs / ExclusiveScan for n elements (no allocation later).
 int ReserveSortScratch(am_ctx* ctx, int64_t n);
 // Exclusive prefix sum of n uint32 counts into out (n+1 entries; out[n] = total).

@@ -423,9 +423,12 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
                      s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (uint32_t*)cnt);
   AM_HIP(hipGetLastError());
   if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)off, n))) return rc;
-  uint32_t total = 0;
+  uint32_t total = 0, sort_err = 0;  // (the cell list's sort is checked here)
   AM_HIP(hipMemcpyAsync(&total, (uint32_t*)off + n, 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (const uint32_t* w = SortErrorWord(ctx))
+    AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
+  if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
   if (total > 0) {
     void *pairs, *gslot, *gpass, *gmark;
     if ((rc = Scratch(ctx, "contact_pairs", (size_t)total * 8, &pairs))) return rc;

@@ -1558,9 +1558,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                        (const uint64_t*)hk, (uint32_t*)occ);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, (const uint32_t*)occ, (uint32_t*)ids, (int64_t)tsize))) return rc;
-    uint32_t n_cls = 0;
+    uint32_t n_cls = 0, sort_err = 0;  // (the traversal's list sorts are checked here too)
     AM_HIP(hipMemcpyAsync(&n_cls, (uint32_t*)ids + tsize, 4, hipMemcpyDeviceToHost, st));
+    if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
+    if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
     void *ckey, *Dc;
     if ((rc = Scratch(ctx, "fmm_cls_key", (size_t)std::max<uint32_t>(n_cls, 1) * 8, &ckey))) return rc;
     if ((rc = Scratch(ctx, "fmm_cls_tensor", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 8, &Dc))) return rc;
@@ -1659,9 +1661,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                        (const int64_t*)off_p2p, (uint32_t*)icnt);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, (const uint32_t*)icnt, (uint32_t*)ibase, nn))) return rc;
-    uint32_t n_items = 0;
+    uint32_t n_items = 0, sort_err = 0;  // (the M2L class sort is checked here)
     AM_HIP(hipMemcpyAsync(&n_items, (uint32_t*)ibase + nn, 4, hipMemcpyDeviceToHost, st));
+    if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
+    if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
     if ((rc = Scratch(ctx, "fmm_p2p_items", (size_t)std::max<uint32_t>(n_items, 1) * sizeof(P2PItem), &items)))
       return rc;
     if ((rc = Scratch(ctx, "fmm_p2p_part", (size_t)std::max<uint32_t>(n_items, 1) * 32 * 3 * 8, &part))) return rc;

@@ -445,9 +445,11 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   hipLaunchKernelGGL(InversePermKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, (const uint32_t*)pids,
                      (uint32_t*)inv);
   AM_HIP(hipGetLastError());
-  uint32_t ov = 0;
+  uint32_t ov = 0, sort_err = 0;
   AM_HIP(hipMemcpyAsync(&ov, over, 4, hipMemcpyDeviceToHost, st));
+  if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
   AM_HIP(hipStreamSynchronize(st));
+  if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
   t->nn = nn;
   t->order = (uint32_t*)vals;
   t->pre = (uint32_t*)pids;

@@ -194,8 +194,8 @@ __global__ __launch_bounds__(kScanThreads) void ScanTileApplyKernel(const uint32
 //     (and stops at the first published prefix) to find how many keys of each digit come before it,
 //     and publishes its own prefix;
 //   * the keys are staged in LDS in digit order and written out in runs of consecutive positions.
-// The look-back waits at most kOsSpinLimit polls per tile and digit; a wait that long sets the error
-// word instead of hanging, and no write ever goes past n.
+// The look-back waits at most kOsSpinLimit polls per tile and digit; a wait that long sets the sticky
+// error word (SortErrorWord) instead of hanging, and no write ever goes past n.
 constexpr int kOsThreads = 256;
 constexpr int kOsWaves = kOsThreads / 64;
 constexpr int kOsItems = 16;
@@ -411,10 +411,26 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
   }
 }
 
-// Words of the single-pass sort's scratch: the error word (first, so that its place does not depend on
-// the pass count), histograms [passes][256], digit bases [passes][256], tile counters [passes], then
-// the status words [passes][tiles][256].
-static int64_t OsMetaWords(int passes) { return 1 + (int64_t)passes * 512 + passes; }
+// Words of the single-pass sort's scratch (cleared by every sort): histograms [passes][256], digit
+// bases [passes][256], tile counters [passes], then the status words [passes][tiles][256].  The
+// look-back error word is kept apart (SortErrorWord) and is never cleared by a sort.
+static int64_t OsMetaWords(int passes) { return (int64_t)passes * 512 + passes; }
+
+__global__ void OsZeroWordKernel(uint32_t* w) { *w = 0u; }
+
+uint32_t* SortErrorWord(am_ctx* ctx) {
+  Buffer& b = ctx->scratch["sort_err"];
+  if (!b.ptr) {
+    void* p;
+    if (Scratch(ctx, "sort_err", 4, &p)) return nullptr;
+    hipLaunchKernelGGL(OsZeroWordKernel, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)p);
+  }
+  return (uint32_t*)b.ptr;
+}
+
+int CheckSortErrorAfterSync(uint32_t word) {
+  return word ? Fail(AM_ERR_HIP, "radix sort: a look-back wait on the device gave up") : AM_OK;
+}
 static int64_t OsWords(int64_t n, int passes) {
   return OsMetaWords(passes) + (int64_t)passes * ((n + kOsTile - 1) / kOsTile) * 256;
 }
@@ -458,6 +474,7 @@ int ReserveSortScratch(am_ctx* ctx, int64_t n) {
   if ((rc = Scratch(ctx, "sort_offs", (size_t)(hist + 1) * 4, &p))) return rc;
   if ((rc = Scratch(ctx, "scan_tiles", (size_t)(2 * scan_tiles + 1) * 4, &p))) return rc;
   if ((rc = Scratch(ctx, "sort_onesweep", (size_t)OsWords(n, kOsMaxPasses) * 4, &p))) return rc;
+  if (!SortErrorWord(ctx)) return Fail(AM_ERR_HIP, "radix sort: error word");
   return AM_OK;
 }
 
@@ -472,11 +489,12 @@ static int OneSweepSort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
   const int64_t tiles = (n + kOsTile - 1) / kOsTile;
   const int64_t words = OsWords(n, passes);
   if ((rc = Scratch(ctx, "sort_onesweep", (size_t)words * 4, &os))) return rc;
-  uint32_t* err = (uint32_t*)os;
-  uint32_t* ghist = err + 1;
+  uint32_t* err = SortErrorWord(ctx);
+  if (!err) return Fail(AM_ERR_HIP, "radix sort: error word");
+  uint32_t* ghist = (uint32_t*)os;
   uint32_t* gbase = ghist + passes * 256;
   uint32_t* counters = gbase + passes * 256;
-  uint32_t* status = err + OsMetaWords(passes);
+  uint32_t* status = ghist + OsMetaWords(passes);
   hipLaunchKernelGGL(OsClearKernel, dim3((unsigned)std::min<int64_t>(2048, (words + 255) / 256)), dim3(256), 0,
                      ctx->stream, (uint32_t*)os, words);
   hipLaunchKernelGGL(OsHistKernel, dim3((unsigned)tiles), dim3(kOsThreads), 0, ctx->stream, keys, n, bit0, key_bits,
@@ -580,11 +598,9 @@ int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
   AM_HIP(hipMemcpyAsync(keys, dk, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipMemcpyAsync(vals, dv, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
   uint32_t err = 0;
-  if (n > 1 && n < (int64_t)kOsVal)  // the single-pass sort's look-back error word
-    AM_HIP(hipMemcpyAsync(&err, ctx->scratch["sort_onesweep"].ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
-  if (err) return Fail(AM_ERR_HIP, "am_debug_radix_sort: a look-back wait on the device gave up");
-  return AM_OK;
+  return CheckSortErrorAfterSync(err);
 }
 
 int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits) {
@@ -597,10 +613,9 @@ int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, i
   if ((rc = RadixSortKeys(ctx, (uint64_t*)dk, n, bit0, key_bits))) return rc;
   AM_HIP(hipMemcpyAsync(keys, dk, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   uint32_t err = 0;
-  if (n > 1) AM_HIP(hipMemcpyAsync(&err, ctx->scratch["sort_onesweep"].ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
-  if (err) return Fail(AM_ERR_HIP, "am_debug_radix_sort_keys: a look-back wait on the device gave up");
-  return AM_OK;
+  return CheckSortErrorAfterSync(err);
 }
 
 }  // extern "C"
