@@ -282,8 +282,10 @@ __device__ __forceinline__ void PairF64Acc(double tx, double ty, double tz, doub
   az = __dadd_rn(az, __dmul_rn(rz, c));
 }
 
-// MODE 0: FP32 force; MODE 1: FP64 force; MODE 2: FP64 potential (q_i phi_i into out_x).
-template <int MODE>
+// MODE 0: FP32 force; MODE 1: FP64 force; MODE 2: FP64 potential (q_i phi_i into out_x).  COUNT:
+// interactions are counted into counts (a compile-time switch, so the plain walk carries no counting
+// arithmetic in its loops).
+template <int MODE, bool COUNT>
 __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes,
                                                      const WalkNodeF32* __restrict__ nodes32,
                                                      const uint32_t* __restrict__ order,
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
           for (int kk = nd.b; kk < nd.e; ++kk) {
             const float4 sh = ssrc[2 * kk];
             PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
-            if (counts) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
+            if (COUNT) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
           }
           ax += fx;
           ay += fy;
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
               const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
               if (d2 >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(qj, sqrt(d2)));
             }
-            ++cnt;
+            if (COUNT) ++cnt;
           }
         }
         next = nd.skip;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
             float fx = 0.f, fy = 0.f, fz = 0.f;
             PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz);
             PairF32(th, tl, f.n_hi, f.n_lo, fx, fy, fz);
-            cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
+            if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
             ax += fx;
             ay += fy;
             az += fz;
@@ -367,20 +369,20 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
               ++nrec;
             }
           } else if (MODE == 1) {
-            if (nd.qp > 0.0) { PairF64Acc(px, py, pz, qi, nd.pqx, nd.pqy, nd.pqz, nd.qp, ax, ay, az); ++cnt; }
-            if (nd.qn > 0.0) { PairF64Acc(px, py, pz, qi, nd.nqx, nd.nqy, nd.nqz, -nd.qn, ax, ay, az); ++cnt; }
+            if (nd.qp > 0.0) { PairF64Acc(px, py, pz, qi, nd.pqx, nd.pqy, nd.pqz, nd.qp, ax, ay, az); if (COUNT) ++cnt; }
+            if (nd.qn > 0.0) { PairF64Acc(px, py, pz, qi, nd.nqx, nd.nqy, nd.nqz, -nd.qn, ax, ay, az); if (COUNT) ++cnt; }
           } else {
             if (nd.qp > 0.0) {
               const double rx = __dsub_rn(px, nd.pqx), ry = __dsub_rn(py, nd.pqy), rz = __dsub_rn(pz, nd.pqz);
               const double d2p = Dot3(rx, ry, rz, rx, ry, rz);
               if (d2p >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(nd.qp, sqrt(d2p)));
-              ++cnt;
+              if (COUNT) ++cnt;
             }
             if (nd.qn > 0.0) {
               const double rx = __dsub_rn(px, nd.nqx), ry = __dsub_rn(py, nd.nqy), rz = __dsub_rn(pz, nd.nqz);
               const double d2n = Dot3(rx, ry, rz, rx, ry, rz);
               if (d2n >= kMinSquaredDistance) ax = __dsub_rn(ax, __ddiv_rn(nd.qn, sqrt(d2n)));
-              ++cnt;
+              if (COUNT) ++cnt;
             }
           }
           next = nd.skip;
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
   } else {
     out_x[i] = __dmul_rn(qi, ax);
   }
-  if (counts) counts[i] = cnt;
+  if (COUNT) counts[i] = cnt;
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
@@ -514,18 +516,13 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   int slot;
   TimerBegin(ctx, mode == 0 ? "bh_walk_f32" : (mode == 1 ? "bh_walk_f64" : "bh_potential"), &slot);
   const dim3 grid(Blocks(t.m, 256)), block(256);
-  if (mode == 0)
-    hipLaunchKernelGGL(BHWalkKernel<0>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
-                       nullptr, 0, nullptr);
-  else if (mode == 1)
-    hipLaunchKernelGGL(BHWalkKernel<1>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
-                       nullptr, 0, nullptr);
-  else
-    hipLaunchKernelGGL(BHWalkKernel<2>, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x,
-                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr,
-                       nullptr, 0, nullptr);
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z,
+                       s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr, nullptr, 0, nullptr);
+  };
+  if (mode == 0) counts ? launch(BHWalkKernel<0, true>) : launch(BHWalkKernel<0, false>);
+  else if (mode == 1) counts ? launch(BHWalkKernel<1, true>) : launch(BHWalkKernel<1, false>);
+  else counts ? launch(BHWalkKernel<2, true>) : launch(BHWalkKernel<2, false>);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;
@@ -537,7 +534,7 @@ int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* 
   am_ctx* ctx = s->ctx;
   int slot;
   TimerBegin(ctx, "bh_walk_f32", &slot);
-  hipLaunchKernelGGL(BHWalkKernel<0>, dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
+  hipLaunchKernelGGL((BHWalkKernel<0, false>), dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
                      t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
                      ox, oy, oz,
                      (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt);
