@@ -858,20 +858,31 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   };
   // the lane's multipole values of the column's source (cell 0 for an empty column: its result
   // column is never stored)
-  auto row = [&](uint64_t kv, f32x4v (&bv)[NKP / 4]) {
+  // Exactly the NKS values are loaded (16-byte loads, then single floats): a padded load would keep
+  // its unused destination registers pending, and the compiler would wait for the load before
+  // reusing them.
+  auto row = [&](uint64_t kv, float (&bv)[NKS]) {
     const uint32_t S = kv == ~0ull ? 0u : (uint32_t)(kv >> 32);
-    const f32x4v* mrow = reinterpret_cast<const f32x4v*>(M32 + (size_t)S * (4 * NKP) + h * NKP);
+    const float* mrow = M32 + (size_t)S * (4 * NKP) + h * NKP;
 #pragma unroll
-    for (int i = 0; i < NKP / 4; ++i) bv[i] = mrow[i];
+    for (int i = 0; i < NKS / 4; ++i) {
+      const f32x4v v = reinterpret_cast<const f32x4v*>(mrow)[i];
+      bv[4 * i] = v[0];
+      bv[4 * i + 1] = v[1];
+      bv[4 * i + 2] = v[2];
+      bv[4 * i + 3] = v[3];
+    }
+#pragma unroll
+    for (int i = (NKS / 4) * 4; i < NKS; ++i) bv[i] = mrow[i];
   };
   uint64_t kv0, kv1;
   uint32_t q0, q1;
-  f32x4v bv0[NKP / 4];
+  float bv0[NKS];
   column(g0, kv0, q0);
   column(g0 + 1, kv1, q1);
   row(kv0, bv0);
   for (int64_t grp = g0; grp < g1; ++grp) {
-    f32x4v bv1[NKP / 4];
+    float bv1[NKS];
     row(kv1, bv1);  // the next group's multipoles and the column after it are in flight meanwhile
     uint64_t kv2;
     uint32_t q2;
@@ -915,7 +926,7 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
       for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
-          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv0[ks >> 2][ks & 3], acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv0[ks], acc[rb], 0, 0, 0);
       // Lane (j, h) holds the four consecutive coefficients 16 rb + 4h .. + 3 of pair j: one
       // 16-byte write into the pair's row.
       if (sel) {
@@ -931,7 +942,7 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
     kv0 = kv1;
     q0 = q1;
 #pragma unroll
-    for (int i = 0; i < NKP / 4; ++i) bv0[i] = bv1[i];
+    for (int i = 0; i < NKS; ++i) bv0[i] = bv1[i];
     kv1 = kv2;
     q1 = q2;
   }
