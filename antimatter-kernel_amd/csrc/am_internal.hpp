@@ -69,6 +69,7 @@ struct am_ctx {
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, int64_t>> timers;
   std::map<std::string, am::Buffer> scratch;  // grow-only named scratch buffers
+  std::map<std::string, int> octree_depth;     // deepest non-empty level of the last build, per tree tag
 };
 
 struct am_store {

@@ -56,30 +56,40 @@ struct Moments {  // BFS order
   double* cnz;
 };
 
+// Tree-ordered FP64 copy of (x, y, z, q), one record per sorted position, so that the leaf sums of
+// the moment passes read their particles contiguously instead of gathering them level by level.
+__global__ void GatherTreeF64Kernel(int m, const uint32_t* __restrict__ order, const double* __restrict__ x,
+                                    const double* __restrict__ y, const double* __restrict__ z,
+                                    const double* __restrict__ q, double4* __restrict__ sxq) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t i = order[k];
+  sxq[k] = make_double4(x[i], y[i], z[i], q[i]);
+}
+
 // Dual monopoles (barnes-hut.cc:138-163): leaves sum q x over their particles in order; internal
 // nodes combine their children's (charge, centre) in octant order; centre = moment * (1/charge).
-__global__ void MomentKernel(int n0, int n1, NodeArrays nd, Moments mo, const uint32_t* __restrict__ vals,
-                             const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                             const double* __restrict__ q) {
+// sxq is the tree-ordered (x, y, z, q) of GatherTreeF64Kernel.
+__global__ void MomentKernel(int n0, int n1, NodeArrays nd, Moments mo, const double4* __restrict__ sxq) {
   const int node = n0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= n1) return;
   double qp = 0.0, qn = 0.0;
   double mpx = 0.0, mpy = 0.0, mpz = 0.0, mnx = 0.0, mny = 0.0, mnz = 0.0;
   if (nd.leaf[node]) {
     for (int k = nd.b[node]; k < nd.e[node]; ++k) {
-      const uint32_t j = vals[k];
-      const double qj = q[j];
+      const double4 pj = sxq[k];
+      const double qj = pj.w;
       if (qj > 0.0) {
         qp = qp + qj;
-        mpx = mpx + x[j] * qj;
-        mpy = mpy + y[j] * qj;
-        mpz = mpz + z[j] * qj;
+        mpx = mpx + pj.x * qj;
+        mpy = mpy + pj.y * qj;
+        mpz = mpz + pj.z * qj;
       } else if (qj < 0.0) {
         const double a = -qj;
         qn = qn + a;
-        mnx = mnx + x[j] * a;
-        mny = mny + y[j] * a;
-        mnz = mnz + z[j] * a;
+        mnx = mnx + pj.x * a;
+        mny = mny + pj.y * a;
+        mnz = mnz + pj.z * a;
       }
     }
   } else {
@@ -135,9 +145,7 @@ struct SecondMoments {  // BFS order, 6 per node per sign
 };
 
 __global__ void SecondMomentKernel(int n0, int n1, NodeArrays nd, Moments mo, SecondMoments sm,
-                                   const uint32_t* __restrict__ vals, const double* __restrict__ x,
-                                   const double* __restrict__ y, const double* __restrict__ z,
-                                   const double* __restrict__ q) {
+                                   const double4* __restrict__ sxq) {
   const int node = n0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= n1) return;
   double ap[6] = {0, 0, 0, 0, 0, 0}, an[6] = {0, 0, 0, 0, 0, 0};
@@ -153,10 +161,10 @@ __global__ void SecondMomentKernel(int n0, int n1, NodeArrays nd, Moments mo, Se
   };
   if (nd.leaf[node]) {
     for (int k = nd.b[node]; k < nd.e[node]; ++k) {
-      const uint32_t j = vals[k];
-      const double qj = q[j];
-      if (qj > 0.0) add(ap, qj, x[j] - cp[0], y[j] - cp[1], z[j] - cp[2]);
-      else if (qj < 0.0) add(an, -qj, x[j] - cn[0], y[j] - cn[1], z[j] - cn[2]);
+      const double4 pj = sxq[k];
+      const double qj = pj.w;
+      if (qj > 0.0) add(ap, qj, pj.x - cp[0], pj.y - cp[1], pj.z - cp[2]);
+      else if (qj < 0.0) add(an, -qj, pj.x - cn[0], pj.y - cn[1], pj.z - cn[2]);
     }
   } else {
     const int c0 = nd.first_child[node], c1 = c0 + nd.nchild[node];
@@ -470,14 +478,18 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
     sm.sp = (double*)a;
     sm.sn = (double*)b;
   }
+  void* sxq;
+  if ((rc = Scratch(ctx, "bh_sxq", (size_t)m * sizeof(double4), &sxq))) return rc;
+  hipLaunchKernelGGL(GatherTreeF64Kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order, s->x,
+                     s->y, s->z, s->q, (double4*)sxq);
   for (int d = nlevels - 1; d >= 0; --d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
     if (n1 <= n0) continue;
     hipLaunchKernelGGL(MomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd, mo,
-                       (const uint32_t*)o.order, s->x, s->y, s->z, s->q);
+                       (const double4*)sxq);
     if (with_features)
       hipLaunchKernelGGL(SecondMomentKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd, mo, sm,
-                         (const uint32_t*)o.order, s->x, s->y, s->z, s->q);
+                         (const double4*)sxq);
     AM_HIP(hipGetLastError());
   }
   void *wn, *wn32;

@@ -300,7 +300,123 @@ __global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, cons
   });
 }
 
-constexpr int kResolveLds = 8192;  // candidate entries resolved from LDS
+constexpr int kResolveLds = 8192;      // candidate entries resolved in LDS by ContactResolveRoundsKernel
+constexpr int kResolveMaxRounds = 64;  // parallel rounds before the remaining runs are walked serially
+
+// The greedy resolution of ContactResolveKernel for lists of at most kResolveLds entries, held in
+// LDS and decided in parallel rounds with the same result.  A particle's run (its entries, sorted by
+// j) is decided when every particle with a smaller index among its candidates and its candidates'
+// candidates is decided: only those can set the marks the run reads (its own and its candidates')
+// before it in scan order, and two runs decided in one round read and write disjoint marks (a shared
+// candidate, or one being the other's candidate, makes the larger wait for the smaller).  The
+// smallest undecided run is always ready, so every round makes progress; after kResolveMaxRounds
+// rounds thread 0 decides whatever is left in scan order.
+__global__ __launch_bounds__(256) void ContactResolveRoundsKernel(const uint2* __restrict__ pairs,
+                                                                  const uint32_t* __restrict__ total_ptr, double p,
+                                                                  uint64_t seed, uint64_t step,
+                                                                  uint8_t* __restrict__ flags,
+                                                                  float4* __restrict__ src,
+                                                                  unsigned int* __restrict__ n_marked) {
+  __shared__ uint32_t li[kResolveLds];     // i of each entry
+  __shared__ uint16_t lslot[kResolveLds];  // first entry of j's run
+  __shared__ uint16_t lend[kResolveLds];   // one past the last entry of i's run (kept at run starts)
+  __shared__ uint8_t lpass[kResolveLds];   // U(i, j) < p
+  __shared__ uint8_t lmark[kResolveLds];   // particle marked (at run starts)
+  __shared__ uint8_t lstate[kResolveLds];  // run: 0 undecided, 2 ready this round, 1 decided
+  __shared__ unsigned int s_count;
+  const uint32_t m = *total_ptr;
+  if (m > (uint32_t)kResolveLds) return;  // the host launches the serial kernel instead
+  auto lower = [&](uint32_t v) {           // first entry whose i >= v
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (li[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) li[e] = pairs[e].x;
+  if (threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+    const uint2 pr = pairs[e];
+    lslot[e] = (uint16_t)lower(pr.y);
+    lpass[e] = AnnihilationUniform(seed, step, pr.x, pr.y) < p ? 1 : 0;
+    lmark[e] = 0;
+    lstate[e] = 0;
+    if (e == 0 || li[e - 1] != pr.x) {
+      uint32_t end = e + 1;
+      while (end < m && li[end] == pr.x) ++end;
+      lend[e] = (uint16_t)end;
+    }
+  }
+  __syncthreads();
+  // Run r (a start entry) of particle i waits for any run with a smaller particle index among
+  // its candidates and their candidates.
+  auto ready = [&](uint32_t r) {
+    const uint32_t i = li[r];
+    for (uint32_t k = r; k < lend[r]; ++k) {
+      const uint32_t s = lslot[k];
+      if (li[s] < i && lstate[s] != 1) return false;
+      for (uint32_t kk = s; kk < lend[s]; ++kk) {
+        const uint32_t t = lslot[kk];
+        if (li[t] < i && lstate[t] != 1) return false;
+      }
+    }
+    return true;
+  };
+  auto decide = [&](uint32_t r) {
+    if (lmark[r]) return;
+    for (uint32_t k = r; k < lend[r]; ++k) {
+      const uint32_t s = lslot[k];
+      if (lmark[s]) continue;
+      if (lpass[k]) {
+        lmark[r] = 1;
+        lmark[s] = 1;
+        return;
+      }
+    }
+  };
+  bool left = true;
+  for (int round = 0; round < kResolveMaxRounds && left; ++round) {
+    bool mine = false;
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+      const bool start = e == 0 || li[e - 1] != li[e];
+      if (start && lstate[e] == 0) {
+        if (ready(e)) lstate[e] = 2;
+        else mine = true;
+      }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+      if (lstate[e] == 2) {
+        decide(e);
+        lstate[e] = 1;
+      }
+    }
+    left = __syncthreads_or(mine);
+  }
+  if (left && threadIdx.x == 0) {
+    for (uint32_t e = 0; e < m; e = lend[e]) {
+      if (lstate[e] == 0) {
+        decide(e);
+        lstate[e] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  unsigned int cnt = 0;
+  for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+    if (!lmark[e]) continue;  // marks sit at run starts
+    const uint32_t i = li[e];
+    flags[i] |= kFlagAnnihilated;
+    src[2 * i].w = 0.f;
+    ++cnt;
+  }
+  if (cnt) atomicAdd(&s_count, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) *n_marked = s_count;
+}
 
 // Greedy resolution in scan order.  entries: the (i, j) list sorted by (i, j); candidates are
 // symmetric, so the distinct i values are exactly the involved particles.  All threads first
@@ -438,9 +554,13 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
     hipLaunchKernelGGL(ContactWriteKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
                        s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (const uint32_t*)off, (uint2*)pairs);
     AM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ContactResolveKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
-                       (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked,
-                       (uint32_t*)gslot, (uint8_t*)gpass, (uint8_t*)gmark);
+    if (total <= (uint32_t)kResolveLds)
+      hipLaunchKernelGGL(ContactResolveRoundsKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
+                         (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked);
+    else
+      hipLaunchKernelGGL(ContactResolveKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
+                         (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked,
+                         (uint32_t*)gslot, (uint8_t*)gpass, (uint8_t*)gmark);
     AM_HIP(hipGetLastError());
     unsigned int hm = 0;
     AM_HIP(hipMemcpyAsync(&hm, marked, 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -142,27 +142,33 @@ __device__ inline int Digit(uint64_t key, int level) {
   return (int)((key >> (3 * (kOctreeLevels - 1 - level))) & 7u);
 }
 
-// First position in [lo, hi) whose digit at depth d is >= o (hi if none); the digits are
-// non-decreasing over the range.  While the range is long, each step probes 15 evenly spaced
-// positions with independent loads and keeps the sixteenth in which the digit crosses o; binary
-// steps finish.
-__device__ inline int FirstDigitAtLeast(const uint64_t* __restrict__ keys, int lo, int hi, int d, int o) {
-  while (hi - lo > 64) {
-    const int step = (hi - lo) / 16;
-    int c = 0;  // probes lo + k step (k = 1..15) with a smaller digit: a prefix of them
+// The octant starts of a node whose sorted keys are [b, e) (e > b): bd[o] = the first position whose
+// digit at depth d is >= o (e if none), for o = 1..7; the digits are non-decreasing over the range.
+// The seven lower-bound searches advance together, so each step issues seven independent loads and
+// the chain of dependent loads is log2(e - b) long rather than seven searches long.  A finished
+// search keeps its result; its load is clamped into the range and ignored.
+__device__ inline void OctantStarts(const uint64_t* __restrict__ keys, int b, int e, int d, int (&bd)[9]) {
+  int base[8], len[8];
 #pragma unroll
-    for (int k = 1; k < 16; ++k) c += Digit(keys[lo + k * step], d) < o ? 1 : 0;
-    const int nlo = c > 0 ? lo + c * step + 1 : lo;
-    const int nhi = c < 15 ? lo + (c + 1) * step : hi;
-    lo = nlo;
-    hi = nhi;
+  for (int o = 1; o < 8; ++o) {
+    base[o] = b;
+    len[o] = e - b;
   }
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (Digit(keys[mid], d) < o) lo = mid + 1;
-    else hi = mid;
+  for (int steps = 32 - __builtin_clz((unsigned)(e - b)); steps > 0; --steps) {
+    uint64_t k[8];
+#pragma unroll
+    for (int o = 1; o < 8; ++o) k[o] = keys[min(base[o] + (len[o] >> 1), e - 1)];
+#pragma unroll
+    for (int o = 1; o < 8; ++o) {
+      const int half = len[o] >> 1;
+      const bool lt = Digit(k[o], d) < o;
+      const bool live = len[o] > 0;
+      base[o] = live && lt ? base[o] + half + 1 : base[o];
+      len[o] = live ? (lt ? len[o] - half - 1 : half) : 0;
+    }
   }
-  return lo;
+#pragma unroll
+  for (int o = 1; o < 8; ++o) bd[o] = base[o];
 }
 
 // The root, and the first two level starts (depth 0 is node 0, depth 1 starts at node 1).
@@ -223,7 +229,7 @@ __global__ __launch_bounds__(kLvThreads) void OctreeLevelKernel(int d, int* __re
     split = e - b > leaf_cap && nd.depth[node] < kOctreeLevels;
     if (split) {
       bd[0] = b;
-      for (int o = 1; o < 8; ++o) bd[o] = FirstDigitAtLeast(keys, bd[o - 1], e, d, o);
+      OctantStarts(keys, b, e, d, bd);
       bd[8] = e;
       for (int o = 0; o < 8; ++o) c += bd[o + 1] > bd[o] ? 1u : 0u;
     }
@@ -386,7 +392,9 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   // The levels run back to back with no host round trip: level d's grid covers an upper bound of its
   // node count (at most 8 per parent, at most one per particle); its real count is read on the
   // device.  Level starts, the overflow and the look-back error come back once at the end; an
-  // overflow rebuilds with larger node arrays.
+  // overflow rebuilds with larger node arrays.  Levels are launched up to one below the previous
+  // build's deepest level; if that level still has nodes, the remaining levels follow after the
+  // read-back.
   const int max_tiles = (m + kLvThreads - 1) / kLvThreads;
   const int64_t lv_words = 2 + (int64_t)(kOctreeLevels + 1) * (1 + max_tiles);  // flags, counters, status
   void *lvl, *lvw;
@@ -396,24 +404,32 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   uint32_t* counters = flags + 2;
   uint32_t* status = counters + kOctreeLevels + 1;
   int lv[kOctreeLevels + 3];
+  const int hint = ctx->octree_depth[tag];  // 0 before the first build
   for (;;) {
     hipLaunchKernelGGL(ClearU32Kernel, dim3((unsigned)std::min<int64_t>(1024, (lv_words + 255) / 256)), dim3(256), 0,
                        st, (uint32_t*)lvw, lv_words);
     hipLaunchKernelGGL(RootNodeKernel, dim3(1), dim3(1), 0, st, nd, root, m, (int*)lvl);
     AM_HIP(hipGetLastError());
     int64_t ub = 1;
-    for (int d = 0; d <= kOctreeLevels; ++d) {
-      hipLaunchKernelGGL(OctreeLevelKernel, dim3((unsigned)((ub + kLvThreads - 1) / kLvThreads)), dim3(kLvThreads), 0,
-                         st, d, (int*)lvl, nd, (const uint64_t*)keys, leaf_cap, (int64_t)cap, counters + d,
-                         status + (size_t)d * max_tiles, flags);
-      ub = std::min<int64_t>(ub * 8, m);
-    }
-    AM_HIP(hipGetLastError());
+    int d = 0;
+    int upto = hint > 0 ? std::min(hint + 1, kOctreeLevels) : kOctreeLevels;
     uint32_t fl[2];
-    AM_HIP(hipMemcpyAsync(lv, lvl, sizeof(lv), hipMemcpyDeviceToHost, st));
-    AM_HIP(hipMemcpyAsync(fl, flags, sizeof(fl), hipMemcpyDeviceToHost, st));
-    AM_HIP(hipStreamSynchronize(st));
-    if (fl[1]) return Fail(AM_ERR_HIP, "octree build: a look-back wait on the device gave up");
+    for (;;) {
+      for (; d <= upto; ++d) {
+        hipLaunchKernelGGL(OctreeLevelKernel, dim3((unsigned)((ub + kLvThreads - 1) / kLvThreads)), dim3(kLvThreads),
+                           0, st, d, (int*)lvl, nd, (const uint64_t*)keys, leaf_cap, (int64_t)cap, counters + d,
+                           status + (size_t)d * max_tiles, flags);
+        ub = std::min<int64_t>(ub * 8, m);
+      }
+      AM_HIP(hipGetLastError());
+      AM_HIP(hipMemcpyAsync(lv, lvl, sizeof(lv), hipMemcpyDeviceToHost, st));
+      AM_HIP(hipMemcpyAsync(fl, flags, sizeof(fl), hipMemcpyDeviceToHost, st));
+      AM_HIP(hipStreamSynchronize(st));
+      if (fl[1]) return Fail(AM_ERR_HIP, "octree build: a look-back wait on the device gave up");
+      // lv[0 .. upto + 2] are set: the levels 0 .. upto + 1.  Go on while level upto + 1 has nodes.
+      if (fl[0] || upto == kOctreeLevels || lv[upto + 2] == lv[upto + 1]) break;
+      upto = kOctreeLevels;
+    }
     if (!fl[0]) break;
     if (cap >= (size_t)kLvVal) return Fail(AM_ERR_CAPACITY, "octree build: too many nodes");
     cap = std::min<size_t>(cap * 2, (size_t)kLvVal);  // rebuild with larger node arrays
@@ -426,6 +442,7 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   }
   t->level_start = {0};
   for (int d = 0; d <= kOctreeLevels + 1 && lv[d + 1] > lv[d]; ++d) t->level_start.push_back(lv[d + 1]);
+  ctx->octree_depth[tag] = (int)t->level_start.size() - 2;
   const int total = t->level_start.back();
   const int nn = total;
   void* over;

@@ -46,6 +46,32 @@ def test_tree_order_and_leaves_match_reference(store):
     assert t["deep_overflow"] == 0
 
 
+def test_tree_depth_changes_between_builds(store):
+    # Shallow and deep trees built one after another on one context.  A build first launches its
+    # levels to one below the previous tree's depth and must carry on when the new tree is deeper.
+    rng = np.random.default_rng(31)
+    base = rng.uniform(-1, 1, (3000, 3)) * 1e-3
+    deep = base.copy()
+    deep[:150] = 2e-4 + rng.uniform(-1, 1, (150, 3)) * 1e-8  # a tight clump: about 16 levels deeper
+    sp = rng.choice([0, 1], 3000).astype(np.uint8)
+    m, q = oracle.mass_charge(sp)
+    depths = []
+    for pos in (base, deep, base, deep, deep):
+        store.upload(pos, np.zeros_like(pos), sp)
+        store.compute_barnes_hut_coulomb_forces(0.5, am.FP64)
+        t = store.bh_tree()
+        ref = oracle.bh_tree(pos, q, 8)
+        np.testing.assert_array_equal(t["order"], ref["order"])
+        np.testing.assert_array_equal(t["begin"], ref["begin"])
+        np.testing.assert_array_equal(t["end"], ref["end"])
+        np.testing.assert_array_equal(t["leaf"], ref["leaf"])
+        np.testing.assert_array_equal(t["qp"], ref["qp"])
+        np.testing.assert_array_equal(t["qn"], ref["qn"])
+        assert t["deep_overflow"] == 0
+        depths.append(int(ref["depth"].max()))
+    assert depths[1] > depths[0] and depths[2] == depths[0] and depths[3] == depths[1]
+
+
 @pytest.mark.parametrize("theta", [0.0, 0.5, 0.9])
 def test_fp64_forces_and_counts_bit_identical(store, theta):
     pos, sp, ann = cloud(4000, seed=22)

@@ -228,6 +228,40 @@ def test_annihilation_matches_oracle(ctx, p):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["clumps", "chain", "long-list"])
+@pytest.mark.parametrize("p", [1.0, 0.5])
+def test_annihilation_dense_cases_match_oracle(ctx, case, p):
+    # clumps: several candidates per particle, so the greedy choices interact; chain: a line of
+    # alternating e+/e- at 0.9 contact radii, numbered along the line, so each decision waits for
+    # the previous one all the way down (longer than the parallel rounds go); long-list: more
+    # candidate entries than the in-LDS resolution holds.
+    rng = np.random.default_rng(40)
+    ra = 6e-6
+    if case == "clumps":
+        n = 2500  # about 4,500 candidate entries, up to 8 per particle
+        centres = rng.uniform(-1, 1, (150, 3)) * 1e-4
+        pos = centres[rng.integers(0, 150, n)] + rng.normal(size=(n, 3)) * 3e-6
+        sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    elif case == "chain":
+        n = 300
+        pos = np.zeros((n, 3))
+        pos[:, 0] = np.arange(n) * 0.9 * ra
+        sp = np.where(np.arange(n) % 2 == 0, 1, 3).astype(np.uint8)
+    else:
+        n = 20000
+        pos = rng.uniform(-1, 1, (n, 3)) * 4e-5
+        sp = rng.choice([1, 3], n).astype(np.uint8)
+    ann = np.zeros(n, np.uint8)
+    s = upload(ctx, pos, np.zeros((n, 3)), sp, ann)
+    k = s.annihilation(ra, p, 7, 3)
+    g = s.download()
+    a2 = ann.copy()
+    cnt, _ = oracle.annihilate(pos, sp, a2, ra, p, 7, 3)
+    assert k == cnt and cnt > 0
+    np.testing.assert_array_equal(g["annihilated"], a2)
+    s.close()
+
+
 def test_energy_matches_oracle(ctx):
     n = 700
     pos, vel, sp = mixed_cloud(n, radius=3e-5)
