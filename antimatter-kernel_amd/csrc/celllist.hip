@@ -272,9 +272,11 @@ __global__ void ContactCountKernel(int64_t n, const double* __restrict__ x, cons
                                    const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    uint32_t* __restrict__ count) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
-  if (k >= n) return;
-  const int64_t i = idx[k];  // a wave's particles share cells and table lines
+  // Particle order: the particle's own position, species and flags are read contiguously.  (In
+  // cell-sorted order they are gathered, about five scattered lines per particle, which costs more
+  // than the table-line sharing gains when contacts are rare.)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   uint32_t c = 0;
   if (!(flags[i] & kFlagAnnihilated))
     ForEachCandidate(i, n, x, y, z, species, flags, tab, idx, inv_h, ra2, [&](int64_t) { ++c; });
@@ -288,9 +290,8 @@ __global__ void ContactWriteKernel(int64_t n, const double* __restrict__ x, cons
                                    const uint8_t* __restrict__ flags, CellTable tab,
                                    const uint32_t* __restrict__ idx, double inv_h, double ra2,
                                    const uint32_t* __restrict__ off, uint2* __restrict__ pairs) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
-  if (k >= n) return;
-  const int64_t i = idx[k];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // particle order, as the count
+  if (i >= n) return;
   const uint32_t b = off[i], e = off[i + 1];
   if (b == e) return;
   uint32_t w = b;
@@ -422,30 +423,22 @@ __global__ __launch_bounds__(256) void ContactResolveRoundsKernel(const uint2* _
   if (threadIdx.x == 0) *n_marked = s_count;
 }
 
-// Greedy resolution in scan order.  entries: the (i, j) list sorted by (i, j); candidates are
-// symmetric, so the distinct i values are exactly the involved particles.  All threads first
-// compute for every entry whether its draw passes (U < p) and the local slot of j; thread 0
-// then walks the list once.  marked[] is indexed by local slot (position of the particle's
-// first entry's run in the list).
+// Greedy resolution in scan order for lists longer than ContactResolveRoundsKernel holds.
+// entries: the (i, j) list sorted by (i, j); candidates are symmetric, so the distinct i values are
+// exactly the involved particles.  All threads first compute for every entry whether its draw
+// passes (U < p) and the slot of j (the position of j's first entry); thread 0 then walks the list
+// once.  mark[] is indexed by slot.
 __global__ __launch_bounds__(256) void ContactResolveKernel(const uint2* __restrict__ pairs,
                                                             const uint32_t* __restrict__ total_ptr, double p,
                                                             uint64_t seed, uint64_t step,
                                                             uint8_t* __restrict__ flags, float4* __restrict__ src,
                                                             unsigned int* __restrict__ n_marked,
-                                                            uint32_t* __restrict__ gslot,
-                                                            uint8_t* __restrict__ gpass,
-                                                            uint8_t* __restrict__ gmark) {
-  __shared__ uint32_t lslot[kResolveLds];
-  __shared__ uint8_t lpass[kResolveLds];
-  __shared__ uint8_t lmark[kResolveLds];
+                                                            uint32_t* __restrict__ slot,
+                                                            uint8_t* __restrict__ pass,
+                                                            uint8_t* __restrict__ mark) {
   const uint32_t m = *total_ptr;
-  const bool use_lds = m <= (uint32_t)kResolveLds;
-  uint32_t* slot = use_lds ? lslot : gslot;
-  uint8_t* pass = use_lds ? lpass : gpass;
-  uint8_t* mark = use_lds ? lmark : gmark;
   for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
     const uint2 pr = pairs[e];
-    // slot of particle pr.y = index of the first entry whose i == pr.y
     uint32_t lo = 0, hi = m;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -550,21 +543,23 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
   AM_HIP(hipStreamSynchronize(ctx->stream));
   if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
   if (total > 0) {
-    void *pairs, *gslot, *gpass, *gmark;
+    void* pairs;
     if ((rc = Scratch(ctx, "contact_pairs", (size_t)total * 8, &pairs))) return rc;
-    if ((rc = Scratch(ctx, "contact_gslot", (size_t)total * 4, &gslot))) return rc;
-    if ((rc = Scratch(ctx, "contact_gpass", (size_t)total, &gpass))) return rc;
-    if ((rc = Scratch(ctx, "contact_gmark", (size_t)total, &gmark))) return rc;
     hipLaunchKernelGGL(ContactWriteKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, s->x, s->y, s->z,
                        s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (const uint32_t*)off, (uint2*)pairs);
     AM_HIP(hipGetLastError());
-    if (total <= (uint32_t)kResolveLds)
+    if (total <= (uint32_t)kResolveLds) {
       hipLaunchKernelGGL(ContactResolveRoundsKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
                          (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked);
-    else
+    } else {
+      void *gslot, *gpass, *gmark;
+      if ((rc = Scratch(ctx, "contact_gslot", (size_t)total * 4, &gslot))) return rc;
+      if ((rc = Scratch(ctx, "contact_gpass", (size_t)total, &gpass))) return rc;
+      if ((rc = Scratch(ctx, "contact_gmark", (size_t)total, &gmark))) return rc;
       hipLaunchKernelGGL(ContactResolveKernel, dim3(1), dim3(256), 0, ctx->stream, (const uint2*)pairs,
                          (const uint32_t*)off + n, probability, seed, step, s->flags, s->src, (unsigned int*)marked,
                          (uint32_t*)gslot, (uint8_t*)gpass, (uint8_t*)gmark);
+    }
     AM_HIP(hipGetLastError());
     unsigned int hm = 0;
     AM_HIP(hipMemcpyAsync(&hm, marked, 4, hipMemcpyDeviceToHost, ctx->stream));
