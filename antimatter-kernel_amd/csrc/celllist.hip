@@ -44,14 +44,14 @@ __global__ void CellKeyKernel(int64_t n, const double* __restrict__ x, const dou
   keys[i] = CellKey(CellCoord(x[i], inv_h), CellCoord(y[i], inv_h), CellCoord(z[i], inv_h));
 }
 
-__global__ void CellTableClearKernel(uint64_t size, uint64_t* __restrict__ hkeys) {
+__global__ void CellTableClearKernel(uint64_t size, CellSlot* __restrict__ slots) {
   const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h < size) hkeys[h] = kCellEmpty;
+  if (h < size) slots[h] = CellSlot{kCellEmpty, 0u, 0u};
 }
 
 // One thread per sorted position; the first position of each run inserts the run.
-__global__ void CellTableInsertKernel(int64_t n, const uint64_t* __restrict__ skeys, uint64_t* __restrict__ hkeys,
-                                      uint2* __restrict__ hrange, uint64_t mask) {
+__global__ void CellTableInsertKernel(int64_t n, const uint64_t* __restrict__ skeys, CellSlot* __restrict__ slots,
+                                      uint64_t mask) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint64_t key = skeys[k];
@@ -61,21 +61,21 @@ __global__ void CellTableInsertKernel(int64_t n, const uint64_t* __restrict__ sk
   uint64_t h = CellHashMix(key) & mask;
   for (;;) {  // a free slot exists: at most n of the >= 2n slots are ever taken
     const unsigned long long prev =
-        atomicCAS(reinterpret_cast<unsigned long long*>(hkeys + h), (unsigned long long)kCellEmpty,
+        atomicCAS(reinterpret_cast<unsigned long long*>(&slots[h].key), (unsigned long long)kCellEmpty,
                   (unsigned long long)key);
     if (prev == kCellEmpty) {
-      hrange[h] = make_uint2((uint32_t)k, (uint32_t)(e - k));
+      slots[h].start = (uint32_t)k;
+      slots[h].count = (uint32_t)(e - k);
       return;
     }
     h = (h + 1) & mask;
   }
 }
 
-int LaunchCellTable(hipStream_t st, const uint64_t* skeys, int64_t n, uint64_t* hkeys, uint2* hrange, uint64_t size) {
-  hipLaunchKernelGGL(CellTableClearKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size, hkeys);
+int LaunchCellTable(hipStream_t st, const uint64_t* skeys, int64_t n, CellSlot* slots, uint64_t size) {
+  hipLaunchKernelGGL(CellTableClearKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size, slots);
   if (n > 0)
-    hipLaunchKernelGGL(CellTableInsertKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, skeys, hkeys, hrange,
-                       size - 1);
+    hipLaunchKernelGGL(CellTableInsertKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, skeys, slots, size - 1);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -126,13 +126,11 @@ static int BuildCellList(am_store* s, double radius, const char* tag, CellList* 
   AM_HIP(hipGetLastError());
   if ((rc = RadixSortPairs(ctx, cl->keys, cl->idx, n, 64))) return rc;
   const uint64_t size = CellTableSize(n);
-  void *hk, *hr;
-  std::string hkn = std::string(tag) + "_hkeys", hrn = std::string(tag) + "_hrange";
-  if ((rc = Scratch(ctx, hkn.c_str(), (size_t)size * 8, &hk))) return rc;
-  if ((rc = Scratch(ctx, hrn.c_str(), (size_t)size * sizeof(uint2), &hr))) return rc;
-  if ((rc = LaunchCellTable(ctx->stream, cl->keys, n, (uint64_t*)hk, (uint2*)hr, size))) return rc;
-  cl->tab.keys = (const uint64_t*)hk;
-  cl->tab.range = (const uint2*)hr;
+  void* hs;
+  const std::string hsn = std::string(tag) + "_hslots";
+  if ((rc = Scratch(ctx, hsn.c_str(), (size_t)size * sizeof(CellSlot), &hs))) return rc;
+  if ((rc = LaunchCellTable(ctx->stream, cl->keys, n, (CellSlot*)hs, size))) return rc;
+  cl->tab.slots = (const CellSlot*)hs;
   cl->tab.mask = size - 1;
   return AM_OK;
 }

@@ -10,9 +10,16 @@ namespace am {
 
 constexpr uint64_t kCellEmpty = ~0ull;  // also the key of particles in no cell (never inserted)
 
+// One table slot: the cell key and its run, in one 16-byte record, so that an insert and a hit
+// touch one line.
+struct __align__(16) CellSlot {
+  uint64_t key;    // kCellEmpty if free
+  uint32_t start;  // run in the sorted order
+  uint32_t count;
+};
+
 struct CellTable {
-  const uint64_t* keys = nullptr;
-  const uint2* range = nullptr;  // (start, count) in the sorted order
+  const CellSlot* slots = nullptr;
   uint64_t mask = 0;
 };
 
@@ -30,9 +37,9 @@ __host__ __device__ inline uint64_t CellHashMix(uint64_t k) {
 __device__ inline uint2 CellLookup(const CellTable& t, uint64_t key) {
   uint64_t h = CellHashMix(key) & t.mask;
   for (;;) {
-    const uint64_t k = t.keys[h];
-    if (k == key) return t.range[h];
-    if (k == kCellEmpty) return make_uint2(0u, 0u);
+    const CellSlot s = t.slots[h];
+    if (s.key == key) return make_uint2(s.start, s.count);
+    if (s.key == kCellEmpty) return make_uint2(0u, 0u);
     h = (h + 1) & t.mask;
   }
 }
@@ -45,7 +52,7 @@ inline uint64_t CellTableSize(int64_t n) {
 }
 
 // Clears the table (fill kernel, safe under stream capture) and inserts every run of the sorted
-// keys skeys[0..n) other than kCellEmpty.  hkeys / hrange hold `size` entries.
-int LaunchCellTable(hipStream_t st, const uint64_t* skeys, int64_t n, uint64_t* hkeys, uint2* hrange, uint64_t size);
+// keys skeys[0..n) other than kCellEmpty.  slots holds `size` entries.
+int LaunchCellTable(hipStream_t st, const uint64_t* skeys, int64_t n, CellSlot* slots, uint64_t size);
 
 }  // namespace am

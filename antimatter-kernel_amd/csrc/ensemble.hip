@@ -445,8 +445,7 @@ struct am_ensemble {
   // contact buffers
   uint64_t* ckeys = nullptr;
   uint32_t* cidx = nullptr;
-  uint64_t* hkeys = nullptr;  // cell table of the sorted contact keys
-  uint2* hrange = nullptr;
+  CellSlot* hslots = nullptr;  // cell table of the sorted contact keys
   uint64_t hsize = 0;
   // force buffers: sort keys, sorted layout, tiles
   uint64_t* fkeys = nullptr;
@@ -575,10 +574,9 @@ static int EnsEnqueueStep(am_ensemble* e, uint64_t seed) {
                        (const uint32_t*)s->trap, (const TrapRec*)e->d_traps, e->ckeys, e->cidx);
     AM_HIP(hipGetLastError());
     if ((rc = RadixSortPairs(ctx, e->ckeys, e->cidx, n, 64))) return rc;
-    if ((rc = LaunchCellTable(st, e->ckeys, n, e->hkeys, e->hrange, e->hsize))) return rc;
+    if ((rc = LaunchCellTable(st, e->ckeys, n, e->hslots, e->hsize))) return rc;
     CellTable tab;
-    tab.keys = e->hkeys;
-    tab.range = e->hrange;
+    tab.slots = e->hslots;
     tab.mask = e->hsize - 1;
     hipLaunchKernelGGL(EnsContactCountKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->x, s->y, s->z,
                        s->species, s->flags, (const uint32_t*)s->trap, (const TrapRec*)e->d_traps, tab,
@@ -702,10 +700,8 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
     if ((rc = EnsAlloc(e, (size_t)n * 4, &p))) return fail(rc);
     e->cidx = (uint32_t*)p;
     e->hsize = CellTableSize(n);
-    if ((rc = EnsAlloc(e, (size_t)e->hsize * 8, &p))) return fail(rc);
-    e->hkeys = (uint64_t*)p;
-    if ((rc = EnsAlloc(e, (size_t)e->hsize * sizeof(uint2), &p))) return fail(rc);
-    e->hrange = (uint2*)p;
+    if ((rc = EnsAlloc(e, (size_t)e->hsize * sizeof(CellSlot), &p))) return fail(rc);
+    e->hslots = (CellSlot*)p;
     if ((rc = EnsAlloc(e, (size_t)n * 4, &p))) return fail(rc);
     e->ccount = (uint32_t*)p;
     if ((rc = EnsAlloc(e, (size_t)(n + 1) * 4, &p))) return fail(rc);
