@@ -7,7 +7,8 @@
 // A cell list sorts particles by the 63-bit key (cx, cy, cz) of their cell of edge h (slightly
 // larger than the interaction radius, so every pair within the radius lies in the 27-cell
 // stencil) with the stable radix sort; a query looks each stencil cell's run up in a cell table
-// (open addressing, built once per call).
+// (open addressing, built once per call).  The contact search, whose results do not depend on the
+// order within a cell, groups particles through the table instead of sorting them.
 // Cell coordinates are clamped to +-2^20 cells (clamping is 1-Lipschitz, so pairs within the
 // radius stay within the stencil; only far-flung particles share boundary cells).
 #include <algorithm>
@@ -131,6 +132,91 @@ static int BuildCellList(am_store* s, double radius, const char* tag, CellList* 
   if ((rc = Scratch(ctx, hsn.c_str(), (size_t)size * sizeof(CellSlot), &hs))) return rc;
   if ((rc = LaunchCellTable(ctx->stream, cl->keys, n, (CellSlot*)hs, size))) return rc;
   cl->tab.slots = (const CellSlot*)hs;
+  cl->tab.mask = size - 1;
+  return AM_OK;
+}
+
+// ---- contact cells: grouped through the table, not sorted ---------------------------------------
+// The contact search needs each cell's particles contiguous, in no particular order: a particle's
+// candidates are sorted by index before they are resolved, and the counts do not depend on the
+// order.  So the contact cell list is built from the table itself: every live particle finds or
+// inserts its cell key and takes a rank in the cell with an atomic add on the slot's count; a scan
+// of the slot counts gives each cell's start; each particle is placed at start + rank.  (The
+// cutoff force and energy sums keep the sorted cell list: their summation order is the order of
+// the cells' particles.)
+constexpr uint32_t kNoSlot = ~0u;
+
+__global__ void ContactCellInsertKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                        const double* __restrict__ z, const uint8_t* __restrict__ flags,
+                                        double inv_h, CellSlot* __restrict__ slots, uint64_t mask,
+                                        uint32_t* __restrict__ slot_of, uint32_t* __restrict__ rank) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i] & kFlagAnnihilated) {
+    slot_of[i] = kNoSlot;
+    return;
+  }
+  const uint64_t key = CellKey(CellCoord(x[i], inv_h), CellCoord(y[i], inv_h), CellCoord(z[i], inv_h));
+  uint64_t h = CellHashMix(key) & mask;
+  for (;;) {  // a free slot exists: at most n of the >= 2n slots are ever taken
+    unsigned long long* kp = reinterpret_cast<unsigned long long*>(&slots[h].key);
+    unsigned long long k = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == (unsigned long long)kCellEmpty) k = atomicCAS(kp, (unsigned long long)kCellEmpty, (unsigned long long)key);
+    if (k == (unsigned long long)kCellEmpty || k == (unsigned long long)key) break;
+    h = (h + 1) & mask;
+  }
+  slot_of[i] = (uint32_t)h;
+  rank[i] = atomicAdd(&slots[h].count, 1u);
+}
+
+__global__ void SlotCountKernel(uint64_t size, const CellSlot* __restrict__ slots, uint32_t* __restrict__ cnt) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size) cnt[h] = slots[h].count;
+}
+
+__global__ void SlotStartKernel(uint64_t size, const uint32_t* __restrict__ off, CellSlot* __restrict__ slots) {
+  const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < size) slots[h].start = off[h];
+}
+
+__global__ void ContactCellPlaceKernel(int64_t n, const uint32_t* __restrict__ slot_of,
+                                       const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
+                                       uint32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || slot_of[i] == kNoSlot) return;
+  idx[off[slot_of[i]] + rank[i]] = (uint32_t)i;
+}
+
+static int BuildContactCells(am_store* s, double radius, CellList* cl) {
+  am_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s->n;
+  const uint64_t size = CellTableSize(n);
+  void *hs, *idx, *slot_of, *rank, *cnt, *off;
+  int rc;
+  if ((rc = Scratch(ctx, "cl_contact_hslots", (size_t)size * sizeof(CellSlot), &hs))) return rc;
+  if ((rc = Scratch(ctx, "cl_contact_idx", (size_t)n * 4, &idx))) return rc;
+  if ((rc = Scratch(ctx, "cl_contact_slot", (size_t)n * 4, &slot_of))) return rc;
+  if ((rc = Scratch(ctx, "cl_contact_rank", (size_t)n * 4, &rank))) return rc;
+  if ((rc = Scratch(ctx, "cl_contact_cnt", (size_t)size * 4, &cnt))) return rc;
+  if ((rc = Scratch(ctx, "cl_contact_off", (size_t)(size + 1) * 4, &off))) return rc;
+  cl->keys = nullptr;
+  cl->idx = (uint32_t*)idx;
+  cl->inv_h = 1.0 / (radius * (1.0 + 1e-9));
+  CellSlot* slots = (CellSlot*)hs;
+  hipLaunchKernelGGL(CellTableClearKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size, slots);
+  hipLaunchKernelGGL(ContactCellInsertKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->x, s->y, s->z, s->flags,
+                     cl->inv_h, slots, size - 1, (uint32_t*)slot_of, (uint32_t*)rank);
+  hipLaunchKernelGGL(SlotCountKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size,
+                     (const CellSlot*)slots, (uint32_t*)cnt);
+  AM_HIP(hipGetLastError());
+  if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)off, (int64_t)size))) return rc;
+  hipLaunchKernelGGL(SlotStartKernel, dim3(Blocks((int64_t)size, 256)), dim3(256), 0, st, size, (const uint32_t*)off,
+                     slots);
+  hipLaunchKernelGGL(ContactCellPlaceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, (const uint32_t*)slot_of,
+                     (const uint32_t*)rank, (const uint32_t*)off, (uint32_t*)idx);
+  AM_HIP(hipGetLastError());
+  cl->tab.slots = (const CellSlot*)slots;
   cl->tab.mask = size - 1;
   return AM_OK;
 }
@@ -521,7 +607,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
   am_ctx* ctx = s->ctx;
   const int64_t n = s->n;
   CellList cl;
-  int rc = BuildCellList(s, kContactCellEdge * contact_radius * kContactReach, "cl_contact", &cl);
+  int rc = BuildContactCells(s, kContactCellEdge * contact_radius * kContactReach, &cl);
   if (rc) return rc;
   void *cnt, *off, *marked;
   if ((rc = Scratch(ctx, "contact_count", (size_t)n * 4, &cnt))) return rc;
@@ -534,7 +620,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
                      s->species, s->flags, cl.tab, cl.idx, cl.inv_h, ra2, (uint32_t*)cnt);
   AM_HIP(hipGetLastError());
   if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt, (uint32_t*)off, n))) return rc;
-  uint32_t total = 0, sort_err = 0;  // (the cell list's sort is checked here)
+  uint32_t total = 0, sort_err = 0;  // (the sticky sort error word: any sort on this context so far)
   AM_HIP(hipMemcpyAsync(&total, (uint32_t*)off + n, 4, hipMemcpyDeviceToHost, ctx->stream));
   if (const uint32_t* w = SortErrorWord(ctx))
     AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
