@@ -329,6 +329,46 @@ __global__ void LeafSortKernel(int nn, NodeArrays nd, uint32_t* __restrict__ val
   if (nd.depth[node] >= kOctreeLevels && e - b > 8) atomicAdd(over, 1u);
 }
 
+// The same order by G lanes per node (G >= the leaf capacity, a power of two <= 64): lane l of a
+// leaf holding m <= G particles loads the l-th index and counts, by G shuffles within its group,
+// the indices below it (they are distinct); that count is its position.  All loads of a group
+// precede its stores, since the group is part of one wave.  A leaf longer than G (only at the depth
+// limit) is sorted serially by its group's first lane.
+template <int G>
+__global__ __launch_bounds__(256) void LeafSortGroupKernel(int nn, NodeArrays nd, uint32_t* __restrict__ vals,
+                                                           uint32_t* __restrict__ over) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group size");
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int node = (int)(t / G);
+  const int l = (int)(threadIdx.x % G);
+  const int gbase = (int)(threadIdx.x & 63) & ~(G - 1);
+  const bool leaf = node < nn && nd.leaf[node];
+  int b = 0, e = 0;
+  if (leaf) {
+    b = nd.b[node];
+    e = nd.e[node];
+  }
+  const int m = e - b;
+  const bool small = leaf && m <= G;
+  const uint32_t v = small && l < m ? vals[b + l] : 0xffffffffu;
+  int rank = 0;
+#pragma unroll
+  for (int j = 0; j < G; ++j) rank += __shfl(v, gbase + j) < v ? 1 : 0;
+  if (small && l < m) vals[b + rank] = v;
+  if (leaf && !small && l == 0) {
+    for (int k = b + 1; k < e; ++k) {
+      const uint32_t w = vals[k];
+      int j = k - 1;
+      while (j >= b && vals[j] > w) {
+        vals[j + 1] = vals[j];
+        --j;
+      }
+      vals[j + 1] = w;
+    }
+  }
+  if (leaf && l == 0 && nd.depth[node] >= kOctreeLevels && m > 8) atomicAdd(over, 1u);
+}
+
 __global__ void ZeroU32Kernel(uint32_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -448,8 +488,15 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   void* over;
   if ((rc = Scratch(ctx, Name(tag, "over").c_str(), 4, &over))) return rc;
   hipLaunchKernelGGL(ZeroU32Kernel, dim3(1), dim3(64), 0, st, (uint32_t*)over, 1);
-  hipLaunchKernelGGL(LeafSortKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, nd, (uint32_t*)vals,
-                     (uint32_t*)over);
+  if (leaf_cap <= 8)
+    hipLaunchKernelGGL(LeafSortGroupKernel<8>, dim3(Blocks((int64_t)nn * 8, 256)), dim3(256), 0, st, nn, nd,
+                       (uint32_t*)vals, (uint32_t*)over);
+  else if (leaf_cap <= 32)
+    hipLaunchKernelGGL(LeafSortGroupKernel<32>, dim3(Blocks((int64_t)nn * 32, 256)), dim3(256), 0, st, nn, nd,
+                       (uint32_t*)vals, (uint32_t*)over);
+  else
+    hipLaunchKernelGGL(LeafSortKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, nd, (uint32_t*)vals,
+                       (uint32_t*)over);
   AM_HIP(hipGetLastError());
   void *pkeys, *pids, *inv;
   if ((rc = Scratch(ctx, Name(tag, "pkeys").c_str(), (size_t)nn * 8, &pkeys))) return rc;
