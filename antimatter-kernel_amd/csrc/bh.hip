@@ -58,13 +58,12 @@ struct Moments {  // BFS order
 
 // Tree-ordered FP64 copy of (x, y, z, q), one record per sorted position, so that the leaf sums of
 // the moment passes read their particles contiguously instead of gathering them level by level.
-__global__ void GatherTreeF64Kernel(int m, const uint32_t* __restrict__ order, const double* __restrict__ x,
-                                    const double* __restrict__ y, const double* __restrict__ z,
-                                    const double* __restrict__ q, double4* __restrict__ sxq) {
+// xq holds the same records by particle index (written by the Morton pass).
+__global__ void GatherTreeF64Kernel(int m, const uint32_t* __restrict__ order, const double4* __restrict__ xq,
+                                    double4* __restrict__ sxq) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
-  const uint32_t i = order[k];
-  sxq[k] = make_double4(x[i], y[i], z[i], q[i]);
+  sxq[k] = xq[order[k]];
 }
 
 // Dual monopoles (barnes-hut.cc:138-163): leaves sum q x over their particles in order; internal
@@ -453,8 +452,10 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   hipLaunchKernelGGL(BHRootKernel, dim3(1), dim3(1), 0, st, (const double*)box, (RootBox*)root);
   AM_HIP(hipGetLastError());
   Octree& o = t->tree;
-  if ((rc = BuildOctree(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, (const RootBox*)root, leaf_cap, "bh",
-                        &o)))
+  void* xq;
+  if ((rc = Scratch(ctx, "bh_xq", (size_t)n * sizeof(double4), &xq))) return rc;
+  if ((rc = BuildOctree(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, s->q, (const RootBox*)root, leaf_cap,
+                        "bh", &o, (double4*)xq)))
     return rc;
   const int nn = o.nn;
   const int nlevels = (int)o.level_start.size() - 1;
@@ -480,8 +481,8 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   }
   void* sxq;
   if ((rc = Scratch(ctx, "bh_sxq", (size_t)m * sizeof(double4), &sxq))) return rc;
-  hipLaunchKernelGGL(GatherTreeF64Kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order, s->x,
-                     s->y, s->z, s->q, (double4*)sxq);
+  hipLaunchKernelGGL(GatherTreeF64Kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
+                     (const double4*)xq, (double4*)sxq);
   for (int d = nlevels - 1; d >= 0; --d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
     if (n1 <= n0) continue;

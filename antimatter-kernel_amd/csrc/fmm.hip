@@ -208,14 +208,13 @@ __global__ void RescaleKernel(int m, const uint32_t* __restrict__ idx, const dou
   rz[i] = (z[i] - fr->mid[2]) / L;
 }
 
-// Tree-ordered particle copies (fast-multipole.cc:401-410).
-__global__ void SortedCopyKernel(int m, const uint32_t* __restrict__ order, const double* __restrict__ rx,
-                                 const double* __restrict__ ry, const double* __restrict__ rz,
-                                 const double* __restrict__ q, double4* __restrict__ sp) {
+// Tree-ordered particle copies (fast-multipole.cc:401-410).  xq: the rescaled (x, y, z, q) by
+// particle index, one record each (written by the Morton pass).
+__global__ void SortedCopyKernel(int m, const uint32_t* __restrict__ order, const double4* __restrict__ xq,
+                                 double4* __restrict__ sp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
-  const uint32_t i = order[k];
-  sp[k] = make_double4(rx[i], ry[i], rz[i], q[i]);
+  sp[k] = xq[order[k]];
 }
 
 // FP32 hi/lo source mirror in tree order (for the FP32 near field).
@@ -1425,8 +1424,10 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                      s->z, (const FmmFrame*)frame, (double*)rx, (double*)ry, (double*)rz);
   AM_HIP(hipGetLastError());
   Octree o;
+  void* xq;
+  if ((rc = Scratch(ctx, "fmm_xq", (size_t)n * sizeof(double4), &xq))) return rc;
   if ((rc = BuildOctree(ctx, (const uint32_t*)gathered, m, (const double*)rx, (const double*)ry, (const double*)rz,
-                        root, 32, "fmm", &o)))
+                        s->q, root, 32, "fmm", &o, (double4*)xq)))
     return rc;
   const int nn = o.nn;
   const int nlevels = (int)o.level_start.size() - 1;
@@ -1441,7 +1442,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   if ((rc = Scratch(ctx, "fmm_M", (size_t)nn * tab.nm * 8, &M))) return rc;
   if ((rc = Scratch(ctx, "fmm_L", (size_t)nn * tab.nm * 8, &L))) return rc;
   hipLaunchKernelGGL(SortedCopyKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
-                     (const double*)rx, (const double*)ry, (const double*)rz, s->q, (double4*)sp);
+                     (const double4*)xq, (double4*)sp);
   for (int d = nlevels - 1; d >= 0; --d) {
     const int n0 = o.level_start[d], n1 = o.level_start[d + 1];
     hipLaunchKernelGGL(RadiusKernel, dim3(Blocks(n1 - n0, 256)), dim3(256), 0, st, n0, n1, o.nd,

@@ -117,12 +117,13 @@ int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const
 // move the centre by +-half/2 in FP64 (barnes-hut.cc:30-33,124-136; fast-multipole.cc:234-237,299-307).
 __global__ void MortonKernel(int m, const uint32_t* __restrict__ idx, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             const RootBox* __restrict__ root, uint64_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
+                             const double* __restrict__ q, const RootBox* __restrict__ root,
+                             uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, double4* __restrict__ xq) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const uint32_t i = idx[k];
   const double px = x[i], py = y[i], pz = z[i];
+  xq[i] = make_double4(px, py, pz, q[i]);
   double cx = root->cx, cy = root->cy, cz = root->cz, h = root->half;
   uint64_t key = 0;
   for (int l = 0; l < kOctreeLevels; ++l) {
@@ -392,7 +393,7 @@ __global__ void InversePermKernel(int nn, const uint32_t* __restrict__ pre, uint
 }
 
 int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, const double* y, const double* z,
-                const RootBox* root, int leaf_cap, const char* tag, Octree* t) {
+                const double* q, const RootBox* root, int leaf_cap, const char* tag, Octree* t, double4* xq) {
   hipStream_t st = ctx->stream;
   int rc;
   t->m = m;
@@ -402,8 +403,8 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   void *keys, *vals;
   if ((rc = Scratch(ctx, Name(tag, "keys").c_str(), (size_t)m * 8, &keys))) return rc;
   if ((rc = Scratch(ctx, Name(tag, "vals").c_str(), (size_t)m * 4, &vals))) return rc;
-  hipLaunchKernelGGL(MortonKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, particles, x, y, z, root,
-                     (uint64_t*)keys, (uint32_t*)vals);
+  hipLaunchKernelGGL(MortonKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, particles, x, y, z, q, root,
+                     (uint64_t*)keys, (uint32_t*)vals, xq);
   AM_HIP(hipGetLastError());
   if ((rc = RadixSortPairs(ctx, (uint64_t*)keys, (uint32_t*)vals, m, 3 * kOctreeLevels))) return rc;
 

@@ -45,9 +45,11 @@ struct Octree {
   uint32_t deep_overflow = 0;   // depth-21 leaves holding more than 8 particles
 };
 
-// particles: m indices (ascending) into x/y/z; root: device pointer; tag prefixes scratch names.
+// particles: m indices (ascending) into x/y/z/q; root: device pointer; tag prefixes scratch names.
+// xq (indexed like x) receives (x, y, z, q) of the m particles as one record each, so that later
+// tree-order gathers read one line per particle.
 int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, const double* y, const double* z,
-                const RootBox* root, int leaf_cap, const char* tag, Octree* t);
+                const double* q, const RootBox* root, int leaf_cap, const char* tag, Octree* t, double4* xq);
 
 // Compacts the indices i < n with keep(flags[i], q[i]) into out (ascending); returns count.
 // mode 0: alive; mode 1: alive and q != 0.
