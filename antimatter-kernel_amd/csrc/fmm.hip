@@ -927,9 +927,10 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
         for (int rb = 0; rb < NRB; ++rb)
           acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv0[ks], acc[rb], 0, 0, 0);
       // Lane (j, h) holds the four consecutive coefficients 16 rb + 4h .. + 3 of pair j: one
-      // 16-byte write into the pair's row.
+      // 16-byte write into the pair's row.  Rows are stored in the class-sorted order of the
+      // pairs, so a group's rows are contiguous.
       if (sel) {
-        float* orow = out + (size_t)q0 * nm4;
+        float* orow = out + (size_t)(grp * 16 + j) * nm4;
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb) {
           const int g = rb * 16 + 4 * h;
@@ -947,14 +948,22 @@ __global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t*
   }
 }
 
+// Where the pair at each position of the target-grouped list has its result row: pos[vals[r]] = r,
+// vals the class-sorted pairs' list positions.
+__global__ void RowPositionKernel(int64_t np, const uint32_t* __restrict__ vals, uint32_t* __restrict__ pos) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < np) pos[vals[r]] = (uint32_t)r;
+}
+
 // L of each target: its pairs' result rows summed in FP64, times 1/gamma!.  out holds a row of nm4
-// floats per pair (nm4 a multiple of 4, at most 64).  Lane l reads the 16-byte chunk l % nch
-// (nch = nm4 / 4) of rows b + l / nch, + rpi, + 2 rpi, ... (rpi = 64 / nch rows per load
-// instruction), and the rpi partial sums of a chunk are then added in row-group order: the order is
-// fixed, so the sum is deterministic.
+// floats per pair (nm4 a multiple of 4, at most 64), the pair at list position q in row pos[q].
+// Lane l reads the 16-byte chunk l % nch (nch = nm4 / 4) of the rows of list positions
+// b + l / nch, + rpi, + 2 rpi, ... (rpi = 64 / nch rows per load instruction), and the rpi partial
+// sums of a chunk are then added in row-group order: the order is fixed, so the sum is
+// deterministic.
 __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm, int nm4,
-                                                   const float* __restrict__ out, const double* __restrict__ inv_fact,
-                                                   double* __restrict__ L) {
+                                                   const float* __restrict__ out, const uint32_t* __restrict__ pos,
+                                                   const double* __restrict__ inv_fact, double* __restrict__ L) {
   __shared__ double part[64][4];
   const int p = blockIdx.x, lane = threadIdx.x;
   const int64_t b = off[p], e = off[p + 1];
@@ -966,7 +975,7 @@ __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __rest
     for (; q + 3 * (int64_t)rpi < e; q += 4 * (int64_t)rpi) {
       f32x4v v[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const f32x4v*>(out + (size_t)(q + u * rpi) * nm4 + 4 * ch);
+      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const f32x4v*>(out + (size_t)pos[q + u * rpi] * nm4 + 4 * ch);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         a0 += (double)v[u][0];
@@ -976,7 +985,7 @@ __global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __rest
       }
     }
     for (; q < e; q += rpi) {
-      const f32x4v v = *reinterpret_cast<const f32x4v*>(out + (size_t)q * nm4 + 4 * ch);
+      const f32x4v v = *reinterpret_cast<const f32x4v*>(out + (size_t)pos[q] * nm4 + 4 * ch);
       a0 += (double)v[0];
       a1 += (double)v[1];
       a2 += (double)v[2];
@@ -1597,7 +1606,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                          (const double*)Dc, tab, (const double*)M, (double*)L);
     };
     if (mfma) {
-      void *Dc32, *M32, *ckeys, *cvals, *outp;
+      void *Dc32, *M32, *ckeys, *cvals, *outp, *rowpos;
       if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
       const int nrb = tab.nm <= 36 ? 3 : 4, nks = tab.nm <= 36 ? 9 : 14;  // the instance below
       const int nkp = (nks + 3) & ~3;
@@ -1606,6 +1615,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       if ((rc = Scratch(ctx, "fmm_cls_sortkeys", (size_t)n_m2l * 8, &ckeys))) return rc;
       if ((rc = Scratch(ctx, "fmm_cls_sortvals", (size_t)n_m2l * 4, &cvals))) return rc;
       if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * nm4 * 4, &outp))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_rowpos", (size_t)n_m2l * 4, &rowpos))) return rc;
       const int64_t nt = (int64_t)n_cls * tab.nd;
       if (nt > 0)
         hipLaunchKernelGGL(TensorToF32Kernel, dim3(Blocks(nt, 256)), dim3(256), 0, st, nt, (const double*)Dc,
@@ -1632,8 +1642,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
       if (tab.nm <= 36) mf(M2LMfmaKernel<3, 9>);
       else mf(M2LMfmaKernel<4, 14>);
       TimerEnd(ctx, mslot);
+      if (n_m2l > 0)
+        hipLaunchKernelGGL(RowPositionKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                           (const uint32_t*)cvals, (uint32_t*)rowpos);
       hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm, nm4,
-                         (const float*)outp, tab.inv_fact, (double*)L);
+                         (const float*)outp, (const uint32_t*)rowpos, tab.inv_fact, (double*)L);
       AM_HIP(hipGetLastError());
     } else if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
     else if (dpl <= 3) launch(M2LKernel<3, false>);
