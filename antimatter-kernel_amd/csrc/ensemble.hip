@@ -314,9 +314,9 @@ __global__ void EnsContactCountKernel(int64_t n, const double* __restrict__ x, c
                                       const uint8_t* __restrict__ flags, const uint32_t* __restrict__ trap,
                                       const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, uint32_t* __restrict__ count) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
-  if (k >= n) return;
-  const int64_t i = idx[k];  // a wave's particles share cells and table lines
+  // Particle order: the particle's own trap, flags, species and position are read contiguously.
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   uint32_t c = 0;
   const uint32_t t = trap[i];
   if (t < (uint32_t)kMaxTraps && !(flags[i] & kFlagAnnihilated) && traps[t].annihilation)
@@ -330,9 +330,8 @@ __global__ void EnsContactWriteKernel(int64_t n, const double* __restrict__ x, c
                                       const TrapRec* __restrict__ traps, CellTable tab,
                                       const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
                                       int64_t cap, uint2* __restrict__ pairs, uint32_t* __restrict__ overflow) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // cell-sorted position
-  if (k >= n) return;
-  const int64_t i = idx[k];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // particle order, as the count
+  if (i >= n) return;
   const uint32_t b = off[i], e = off[i + 1];
   if (b == e) return;
   if ((int64_t)e > cap) {
