@@ -410,57 +410,147 @@ __device__ inline int TraverseKind(const double4 T, const double4 S, uint32_t ti
   return (s_leaf || (!t_leaf && T.w >= S.w)) ? 2 : 3;  // split the target / the source
 }
 
-__global__ void TraverseCountKernel(int np, const uint2* __restrict__ pairs, const double4* __restrict__ trv,
-                                    const uint32_t* __restrict__ tinfo, double theta2, uint32_t* __restrict__ cnt) {
-  uint32_t* n_m2l = cnt;
-  uint32_t* n_p2p = cnt + np;
-  uint32_t* n_new = cnt + 2 * (size_t)np;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const uint2 pr = pairs[i];
-  const uint32_t ti = tinfo[pr.x], si = tinfo[pr.y];
-  const int kind = TraverseKind(trv[pr.x], trv[pr.y], ti, si, theta2);
-  n_m2l[i] = kind == 0 ? 1u : 0u;
-  n_p2p[i] = kind == 1 ? 1u : 0u;
-  n_new[i] = kind == 2 ? (ti >> 1) : (kind == 3 ? (si >> 1) : 0u);
+// One traversal step over the frontier pairs in tiles of kTrvTile pairs (item u of thread t is pair
+// tile * kTrvTile + u * kTrvThreads + t, so loads are coalesced).  Each pair yields an M2L pair, a
+// P2P pair, or the pairs of the split cell's children; the three outputs keep the frontier order.
+constexpr int kTrvThreads = 256;
+constexpr int kTrvItems = 4;
+constexpr int kTrvTile = kTrvThreads * kTrvItems;
+
+__device__ inline uint32_t TraverseNew(int kind, uint32_t ti, uint32_t si) {
+  return kind == 2 ? (ti >> 1) : (kind == 3 ? (si >> 1) : 0u);
 }
 
-// off = exclusive scan of TraverseCountKernel's cnt (3 np + 1 entries).
-__global__ void TraverseWriteKernel(int np, const uint2* __restrict__ pairs, const double4* __restrict__ trv,
-                                    const uint32_t* __restrict__ tinfo, const Cell* __restrict__ cells,
-                                    double theta2, const uint32_t* __restrict__ off, uint64_t* __restrict__ m2l_out,
-                                    uint64_t* __restrict__ p2p_out, uint2* __restrict__ next) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const uint32_t* o_m2l = off;
-  const uint32_t b2 = off[np], b3 = off[2 * (size_t)np];
-  const uint32_t* o_p2p = off + np;
-  const uint32_t* o_new = off + 2 * (size_t)np;
-  const uint2 pr = pairs[i];
-  const int kind = TraverseKind(trv[pr.x], trv[pr.y], tinfo[pr.x], tinfo[pr.y], theta2);
-  const uint64_t key = ((uint64_t)pr.x << 32) | pr.y;
-  if (kind == 0) {
-    m2l_out[o_m2l[i]] = key;
-  } else if (kind == 1) {
-    p2p_out[o_p2p[i] - b2] = key;
-  } else {
-    uint2* out = next + (o_new[i] - b3);
-    if (kind == 2) {
-      const Cell& T = cells[pr.x];
-      for (int k = 0; k < T.nchild; ++k) out[k] = make_uint2((uint32_t)T.child[k], pr.y);
-    } else {
-      const Cell& S = cells[pr.y];
-      for (int k = 0; k < S.nchild; ++k) out[k] = make_uint2(pr.x, (uint32_t)S.child[k]);
+// Block-wide exclusive scan of three counts in thread order; returns the block totals in tot.
+// Two barriers; wsum may be reused after the second.
+__device__ inline void TraverseBlockScan(uint32_t (&c)[3], uint32_t (&ex)[3], uint32_t (&tot)[3],
+                                         uint32_t (*wsum)[kTrvThreads / 64]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x[3] = {c[0], c[1], c[2]};
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x[k], off);
+      if (lane >= off) x[k] += y;
     }
   }
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wsum[k][w] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t before = 0, all = 0;
+    for (int v = 0; v < kTrvThreads / 64; ++v) {
+      if (v < w) before += wsum[k][v];
+      all += wsum[k][v];
+    }
+    ex[k] = before + x[k] - c[k];
+    tot[k] = all;
+  }
+  __syncthreads();
 }
 
-// The three output counts of a traversal step: tot = (M2L pairs, P2P pairs, child pairs).
-__global__ void TraverseTotalsKernel(int np, const uint32_t* __restrict__ off, uint32_t* __restrict__ tot) {
-  if (threadIdx.x == 0) {
-    tot[0] = off[np];
-    tot[1] = off[2 * (size_t)np] - off[np];
-    tot[2] = off[3 * (size_t)np] - off[2 * (size_t)np];
+// Classifies the tile's pairs (kind per pair, kept for the write) and counts its outputs.
+__global__ __launch_bounds__(kTrvThreads) void TraverseTileCountKernel(int np, const uint2* __restrict__ pairs,
+                                                                       const double4* __restrict__ trv,
+                                                                       const uint32_t* __restrict__ tinfo,
+                                                                       double theta2, uint8_t* __restrict__ kinds,
+                                                                       uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t wsum[3][kTrvThreads / 64];
+  uint32_t c[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int u = 0; u < kTrvItems; ++u) {
+    const int64_t i = (int64_t)blockIdx.x * kTrvTile + u * kTrvThreads + threadIdx.x;
+    if (i < np) {
+      const uint2 pr = pairs[i];
+      const uint32_t ti = tinfo[pr.x], si = tinfo[pr.y];
+      const int kind = TraverseKind(trv[pr.x], trv[pr.y], ti, si, theta2);
+      kinds[i] = (uint8_t)kind;
+      c[0] += kind == 0 ? 1u : 0u;
+      c[1] += kind == 1 ? 1u : 0u;
+      c[2] += TraverseNew(kind, ti, si);
+    }
+  }
+  uint32_t ex[3], tot[3];
+  TraverseBlockScan(c, ex, tot, wsum);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tile_cnt[3 * (size_t)blockIdx.x + k] = tot[k];
+}
+
+// Exclusive scans over the tiles of each output's counts (one block); tot = the three totals.
+__global__ __launch_bounds__(kTrvThreads) void TraverseTileScanKernel(int64_t ntiles,
+                                                                      const uint32_t* __restrict__ tile_cnt,
+                                                                      uint32_t* __restrict__ tile_off,
+                                                                      uint32_t* __restrict__ tot) {
+  __shared__ uint32_t wsum[3][kTrvThreads / 64];
+  uint32_t run[3] = {0u, 0u, 0u};
+  for (int64_t t0 = 0; t0 < ntiles; t0 += kTrvThreads) {
+    const int64_t t = t0 + threadIdx.x;
+    uint32_t c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = t < ntiles ? tile_cnt[3 * t + k] : 0u;
+    uint32_t ex[3], all[3];
+    TraverseBlockScan(c, ex, all, wsum);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (t < ntiles) tile_off[3 * t + k] = run[k] + ex[k];
+      run[k] += all[k];
+    }
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tot[k] = run[k];
+}
+
+// Writes the tile's outputs at the tile's offsets plus the exclusive counts of the earlier pairs of
+// the tile, item round by item round (the frontier order).
+__global__ __launch_bounds__(kTrvThreads) void TraverseTileWriteKernel(int np, const uint2* __restrict__ pairs,
+                                                                       const uint8_t* __restrict__ kinds,
+                                                                       const uint32_t* __restrict__ tinfo,
+                                                                       const Cell* __restrict__ cells,
+                                                                       const uint32_t* __restrict__ tile_off,
+                                                                       uint64_t* __restrict__ m2l_out,
+                                                                       uint64_t* __restrict__ p2p_out,
+                                                                       uint2* __restrict__ next) {
+  __shared__ uint32_t wsum[3][kTrvThreads / 64];
+  uint32_t base[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) base[k] = tile_off[3 * (size_t)blockIdx.x + k];
+#pragma unroll 1
+  for (int u = 0; u < kTrvItems; ++u) {
+    const int64_t i = (int64_t)blockIdx.x * kTrvTile + u * kTrvThreads + threadIdx.x;
+    int kind = -1;
+    uint2 pr = make_uint2(0u, 0u);
+    uint32_t c[3] = {0u, 0u, 0u};
+    if (i < np) {
+      pr = pairs[i];
+      kind = kinds[i];
+      c[0] = kind == 0 ? 1u : 0u;
+      c[1] = kind == 1 ? 1u : 0u;
+      if (kind >= 2) c[2] = TraverseNew(kind, tinfo[pr.x], tinfo[pr.y]);
+    }
+    uint32_t ex[3], tot[3];
+    TraverseBlockScan(c, ex, tot, wsum);
+    const uint64_t key = ((uint64_t)pr.x << 32) | pr.y;
+    if (kind == 0) {
+      m2l_out[base[0] + ex[0]] = key;
+    } else if (kind == 1) {
+      p2p_out[base[1] + ex[1]] = key;
+    } else if (kind >= 2) {
+      uint2* out = next + base[2] + ex[2];
+      if (kind == 2) {
+        const Cell& T = cells[pr.x];
+        for (int k = 0; k < T.nchild; ++k) out[k] = make_uint2((uint32_t)T.child[k], pr.y);
+      } else {
+        const Cell& S = cells[pr.y];
+        for (int k = 0; k < S.nchild; ++k) out[k] = make_uint2(pr.x, (uint32_t)S.child[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) base[k] += tot[k];
   }
 }
 
@@ -1492,16 +1582,17 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   int64_t np = 1;
   TimerBegin(ctx, "fmm_traverse", &slot);
   while (np > 0) {
-    void *cnt3, *off3, *tot_dev;
-    if ((rc = Scratch(ctx, "fmm_trav_cnt", (size_t)np * 3 * 4, &cnt3))) return rc;
-    if ((rc = Scratch(ctx, "fmm_trav_off", ((size_t)np * 3 + 1) * 4, &off3))) return rc;
+    const int64_t ntiles = (np + kTrvTile - 1) / kTrvTile;
+    void *kinds, *tcnt, *toff, *tot_dev;
+    if ((rc = Scratch(ctx, "fmm_trav_kind", (size_t)np, &kinds))) return rc;
+    if ((rc = Scratch(ctx, "fmm_trav_tcnt", (size_t)ntiles * 3 * 4, &tcnt))) return rc;
+    if ((rc = Scratch(ctx, "fmm_trav_toff", (size_t)ntiles * 3 * 4, &toff))) return rc;
     if ((rc = Scratch(ctx, "fmm_trav_tot", 16, &tot_dev))) return rc;
-    hipLaunchKernelGGL(TraverseCountKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const double4*)trv, (const uint32_t*)tinfo, theta2, (uint32_t*)cnt3);
-    AM_HIP(hipGetLastError());
-    if ((rc = ExclusiveScan(ctx, (const uint32_t*)cnt3, (uint32_t*)off3, 3 * np))) return rc;
-    hipLaunchKernelGGL(TraverseTotalsKernel, dim3(1), dim3(64), 0, st, (int)np, (const uint32_t*)off3,
-                       (uint32_t*)tot_dev);
+    hipLaunchKernelGGL(TraverseTileCountKernel, dim3((unsigned)ntiles), dim3(kTrvThreads), 0, st, (int)np,
+                       (const uint2*)fa, (const double4*)trv, (const uint32_t*)tinfo, theta2, (uint8_t*)kinds,
+                       (uint32_t*)tcnt);
+    hipLaunchKernelGGL(TraverseTileScanKernel, dim3(1), dim3(kTrvThreads), 0, st, ntiles, (const uint32_t*)tcnt,
+                       (uint32_t*)toff, (uint32_t*)tot_dev);
     AM_HIP(hipGetLastError());
     uint32_t tot[3];
     AM_HIP(hipMemcpyAsync(tot, tot_dev, 12, hipMemcpyDeviceToHost, st));
@@ -1534,9 +1625,9 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if ((rc = grow("fmm_m2l", &m2l, cap_m2l, (size_t)n_m2l + tot[0], (size_t)n_m2l, 8))) return rc;
     if ((rc = grow("fmm_p2p", &p2p, cap_p2p, (size_t)n_p2p + tot[1], (size_t)n_p2p, 8))) return rc;
     if ((rc = grow("fmm_front_b", &fb, cap_fb, tot[2], 0, 8))) return rc;
-    hipLaunchKernelGGL(TraverseWriteKernel, dim3(Blocks(np, 256)), dim3(256), 0, st, (int)np, (const uint2*)fa,
-                       (const double4*)trv, (const uint32_t*)tinfo, (const Cell*)cells, theta2,
-                       (const uint32_t*)off3, (uint64_t*)m2l + n_m2l, (uint64_t*)p2p + n_p2p, (uint2*)fb);
+    hipLaunchKernelGGL(TraverseTileWriteKernel, dim3((unsigned)ntiles), dim3(kTrvThreads), 0, st, (int)np,
+                       (const uint2*)fa, (const uint8_t*)kinds, (const uint32_t*)tinfo, (const Cell*)cells,
+                       (const uint32_t*)toff, (uint64_t*)m2l + n_m2l, (uint64_t*)p2p + n_p2p, (uint2*)fb);
     AM_HIP(hipGetLastError());
     n_m2l += tot[0];
     n_p2p += tot[1];
