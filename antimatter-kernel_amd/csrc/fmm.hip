@@ -421,10 +421,11 @@ __device__ inline uint32_t TraverseNew(int kind, uint32_t ti, uint32_t si) {
   return kind == 2 ? (ti >> 1) : (kind == 3 ? (si >> 1) : 0u);
 }
 
-// Block-wide exclusive scan of three counts in thread order; returns the block totals in tot.
-// Two barriers; wsum may be reused after the second.
+// Block-wide exclusive scan of three counts in thread order (blocks of THREADS); returns the block
+// totals in tot.  Two barriers; wsum may be reused after the second.
+template <int THREADS = kTrvThreads>
 __device__ inline void TraverseBlockScan(uint32_t (&c)[3], uint32_t (&ex)[3], uint32_t (&tot)[3],
-                                         uint32_t (*wsum)[kTrvThreads / 64]) {
+                                         uint32_t (*wsum)[THREADS / 64]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t x[3] = {c[0], c[1], c[2]};
 #pragma unroll
@@ -442,7 +443,7 @@ __device__ inline void TraverseBlockScan(uint32_t (&c)[3], uint32_t (&ex)[3], ui
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     uint32_t before = 0, all = 0;
-    for (int v = 0; v < kTrvThreads / 64; ++v) {
+    for (int v = 0; v < THREADS / 64; ++v) {
       if (v < w) before += wsum[k][v];
       all += wsum[k][v];
     }
@@ -480,20 +481,22 @@ __global__ __launch_bounds__(kTrvThreads) void TraverseTileCountKernel(int np, c
     for (int k = 0; k < 3; ++k) tile_cnt[3 * (size_t)blockIdx.x + k] = tot[k];
 }
 
-// Exclusive scans over the tiles of each output's counts (one block); tot = the three totals.
-__global__ __launch_bounds__(kTrvThreads) void TraverseTileScanKernel(int64_t ntiles,
-                                                                      const uint32_t* __restrict__ tile_cnt,
-                                                                      uint32_t* __restrict__ tile_off,
-                                                                      uint32_t* __restrict__ tot) {
-  __shared__ uint32_t wsum[3][kTrvThreads / 64];
+// Exclusive scans over the tiles of each output's counts (one block of kTrvScanThreads); tot = the
+// three totals.
+constexpr int kTrvScanThreads = 1024;
+__global__ __launch_bounds__(kTrvScanThreads) void TraverseTileScanKernel(int64_t ntiles,
+                                                                          const uint32_t* __restrict__ tile_cnt,
+                                                                          uint32_t* __restrict__ tile_off,
+                                                                          uint32_t* __restrict__ tot) {
+  __shared__ uint32_t wsum[3][kTrvScanThreads / 64];
   uint32_t run[3] = {0u, 0u, 0u};
-  for (int64_t t0 = 0; t0 < ntiles; t0 += kTrvThreads) {
+  for (int64_t t0 = 0; t0 < ntiles; t0 += kTrvScanThreads) {
     const int64_t t = t0 + threadIdx.x;
     uint32_t c[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) c[k] = t < ntiles ? tile_cnt[3 * t + k] : 0u;
     uint32_t ex[3], all[3];
-    TraverseBlockScan(c, ex, all, wsum);
+    TraverseBlockScan<kTrvScanThreads>(c, ex, all, wsum);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (t < ntiles) tile_off[3 * t + k] = run[k] + ex[k];
@@ -1591,7 +1594,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     hipLaunchKernelGGL(TraverseTileCountKernel, dim3((unsigned)ntiles), dim3(kTrvThreads), 0, st, (int)np,
                        (const uint2*)fa, (const double4*)trv, (const uint32_t*)tinfo, theta2, (uint8_t*)kinds,
                        (uint32_t*)tcnt);
-    hipLaunchKernelGGL(TraverseTileScanKernel, dim3(1), dim3(kTrvThreads), 0, st, ntiles, (const uint32_t*)tcnt,
+    hipLaunchKernelGGL(TraverseTileScanKernel, dim3(1), dim3(kTrvScanThreads), 0, st, ntiles, (const uint32_t*)tcnt,
                        (uint32_t*)toff, (uint32_t*)tot_dev);
     AM_HIP(hipGetLastError());
     uint32_t tot[3];
