@@ -70,6 +70,7 @@ def lib():
             "orc_standard_energy": (_d, [_i64, _vp, _vp, _vp, _d]),
             "orc_bh_forces": (None, [_i64, _vp, _vp, _vp, _d, _i64, _vp, _vp]),
             "orc_bh_potential_energy": (_d, [_i64, _vp, _vp, _vp, _d]),
+            "orc_bh_potentials": (None, [_i64, _vp, _vp, _vp, _d, _i64, _vp]),
             "orc_bh_tree": (_i64, [_i64, _vp, _vp, _i64, _i64] + [_vp] * 12),
             "orc_fmm_forces": (None, [_i64, _vp, _vp, _vp, ctypes.c_int, _d, _i64, _vp, _vp]),
             "orc_fmm_potential_energy": (_d, [_i64, _vp, _vp, _vp, ctypes.c_int, _d]),
@@ -257,6 +258,15 @@ def bh_forces(pos, q, annihilated, theta, leaf_cap=8, want_counts=False):
 def bh_potential_energy(pos, q, annihilated, theta) -> float:
     return float(lib().orc_bh_potential_energy(len(q), _p(_c(pos, np.float64)), _p(_c(q, np.float64)),
                                                _p(_c(annihilated, np.uint8)), theta))
+
+
+def bh_potentials(pos, q, annihilated, theta, leaf_cap=8):
+    """Per-particle BarnesHutTree::ComputePotential(x_i, i) in C/m (no Coulomb constant)."""
+    pos = _c(pos, np.float64)
+    phi = np.zeros(len(q))
+    lib().orc_bh_potentials(len(q), _p(pos), _p(_c(q, np.float64)), _p(_c(annihilated, np.uint8)), theta,
+                            leaf_cap, _p(phi))
+    return phi
 
 
 def bh_tree(pos, q, leaf_cap=8):

@@ -18,7 +18,10 @@
 //       -> brute-force two-body and three-body closed forms, Newton's third law, and
 //          equality with orc_standard_forces at a cutoff larger than the cloud.
 //   orc_bh_* -> theta = 0 equals all-pairs (barnes-hut.h:22-24); node charge sums equal
-//          totals; leaf membership equals an independent recursive partition in tests.
+//          totals; leaf membership equals an independent recursive partition in tests; the far
+//          field at theta > 0 (MAC from the geometric centre, width 2h, both class monopoles)
+//          equals hand sums on a cloud whose accept/open decisions are fixed by construction,
+//          and copies of this file with each of those three rules broken fail that test.
 //   orc_fmm_* -> theta = 0 equals all-pairs (fast-multipole.h:42-44); D_alpha closed
 //          forms for low orders; geometric error decay in P.
 //   orc_philox4x32_10 -> Random123 known-answer vectors.
@@ -1000,6 +1003,29 @@ double orc_bh_potential_energy(int64_t n, const double* pos, const double* q, co
     sum += tree.q[k] * phi;
   }
   return 0.5 * kCoulomb * sum;
+}
+
+// Per-particle Barnes-Hut potential phi_i = BarnesHutTree::ComputePotential(x_i, i)
+// (barnes-hut.cc:175-181, 223-269) in C/m (without k_C), 0 for annihilated particles; the walk of
+// orc_bh_potential_energy with the leaf capacity as an argument.
+void orc_bh_potentials(int64_t n, const double* pos, const double* q, const uint8_t* annihilated, double theta,
+                       int64_t leaf_cap, double* phi_out) {
+  SetThreads();
+  for (int64_t i = 0; i < n; ++i) phi_out[i] = 0.0;
+  const std::vector<int64_t> idx = Gather(n, annihilated);
+  if (idx.empty()) return;
+  BHTree tree;
+  tree.theta2 = theta * theta;
+  tree.leaf_cap = leaf_cap < 1 ? 1 : leaf_cap;
+  for (int64_t i : idx) { tree.pos.push_back(ld(pos, i)); tree.q.push_back(q[i]); }
+  tree.Build();
+  const int64_t m = (int64_t)idx.size();
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    double phi = 0.0;
+    tree.AccumulatePotential(0, tree.pos[k], k, &phi);
+    phi_out[idx[k]] = phi;
+  }
 }
 
 // BH tree structure for tests: returns node count; fills (if non-null, sized >= count)
