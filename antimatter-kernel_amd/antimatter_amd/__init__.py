@@ -90,6 +90,7 @@ SIGNATURES = {
     "am_store_count": (_int, [_vp, _vp, _vp]),
     "am_compute_standard_coulomb_forces": (_int, [_vp, _d, _int]),
     "am_compute_allpairs_coulomb_forces": (_int, [_vp, _int]),
+    "am_allpairs_forces_noalloc": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_barnes_hut_coulomb_forces": (_int, [_vp, _d, _int]),
     "am_barnes_hut_forces_counted": (_int, [_vp, _d, _int, _vp]),
     "am_bh_tree_stats": (_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -17,6 +17,7 @@
 // with the reference pair term, no FMA contraction: the same operations in the same order as
 // the CPU restatement, so results are expected to match it bit for bit.
 #include <algorithm>
+#include <mutex>
 
 #include "am_internal.hpp"
 #include "pairtile.hpp"
@@ -262,7 +263,7 @@ __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const d
 int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const double* y, const double* z,
                  double* box_dev, const char* tag);
 
-static int ChunkCount(am_ctx* ctx, int target_blocks, int n_pad);
+static int ChunkCount(int num_cus, int target_blocks, int n_pad);
 
 int AllPairsForcesF32Sorted(am_store* s) {
   am_ctx* ctx = s->ctx;
@@ -293,7 +294,7 @@ int AllPairsForcesF32Sorted(am_store* s) {
                      s->z, (const float4*)s->src, (float4*)shl, (float4*)sloc, (TileRec*)tiles);
   AM_HIP(hipGetLastError());
   const int tblocks = n_pad / kAllPairsThreads;
-  const int chunks = ChunkCount(ctx, tblocks, n_pad);
+  const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
   const int tpc = (n_tiles + chunks - 1) / chunks;
   const int nch = (n_tiles + tpc - 1) / tpc;
   if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
@@ -310,8 +311,8 @@ int AllPairsForcesF32Sorted(am_store* s) {
   return AM_OK;
 }
 
-static int ChunkCount(am_ctx* ctx, int target_blocks, int n_pad) {
-  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+static int ChunkCount(int num_cus, int target_blocks, int n_pad) {
+  const int cus = num_cus > 0 ? num_cus : 256;
   const int want = 8 * cus;  // >= 8 workgroups per CU
   int chunks = (want + target_blocks - 1) / target_blocks;
   chunks = std::max(1, std::min(chunks, n_pad / kTile));
@@ -324,7 +325,7 @@ int AllPairsForcesF32(am_store* s) {
   const int n = (int)s->n;
   const int n_pad = PadN(n);
   const int tblocks = n_pad / kAllPairsThreads;
-  const int chunks = ChunkCount(ctx, tblocks, n_pad);
+  const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
   int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
   const int nch = (n_pad + chunk - 1) / chunk;
   void* part = nullptr;
@@ -379,7 +380,7 @@ int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_s
   }
   const int n_pad = PadN(n);
   const int tblocks = n_pad / kAllPairsThreads;
-  const int chunks = ChunkCount(ctx, tblocks, n_pad);
+  const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
   int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
   const int nch = (n_pad + chunk - 1) / chunk;
   void* part = nullptr;
@@ -407,4 +408,118 @@ extern "C" int am_compute_allpairs_coulomb_forces(am_store* s, int precision) {
   AM_HIP(hipSetDevice(s->ctx->device));
   if (precision == AM_FP64) return AllPairsForcesF64(s);
   return s->n >= kSortedMinN ? AllPairsForcesF32Sorted(s) : AllPairsForcesF32(s);
+}
+
+// ---- allocation-free variant (am_allpairs_forces_noalloc) ----------------------------------------
+// The kernels of the unsorted FP32 path on caller-owned device buffers, with the FP32 source mirror
+// (32 B per padded particle) and the chunk partials (24 B per padded particle and source chunk) in a
+// file-scope device array.  Each GPU holds its own copy of the array for the life of the process;
+// its address on a device is taken once, with that device current, by hipGetSymbolAddress, and only
+// that address is ever handed to kernels.  Every kernel writes all the scratch it reads (padding
+// entries of the mirror included), so nothing needs clearing between calls.
+namespace am {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;  // 16 MiB: the mirror and partials at n = 65,536
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+  void* ptr = nullptr;  // device address of `scratch` on this device
+  int num_cus = 0;
+  bool ready = false;
+};
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+static std::mutex g_noalloc_mu;
+
+static int NoAllocSetup(int dev, NoAllocDevice** out) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  NoAllocDevice& d = g_noalloc[dev];
+  if (!d.ready) {
+    void* p = nullptr;
+    AM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return Fail(AM_ERR_HIP, "am_allpairs_forces_noalloc: no device address for the scratch array");
+    int cus = 0;
+    AM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    d.ptr = p;
+    d.num_cus = cus;
+    d.ready = true;
+  }
+  *out = &d;
+  return AM_OK;
+}
+
+// RefreshSourceKernel's mirror from interleaved positions (pos[3i..3i+2]) and caller charges.
+__global__ void PackSourceAosKernel(int n, int n_pad, const double* __restrict__ pos, const double* __restrict__ q,
+                                    const uint8_t* __restrict__ annihilated, float4* __restrict__ src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  if (i >= n) {
+    src[2 * i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    src[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const double xs = pos[3 * i] * kUnitScale, ys = pos[3 * i + 1] * kUnitScale, zs = pos[3 * i + 2] * kUnitScale;
+  const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
+  const bool dead = annihilated && annihilated[i];
+  const float qe = dead ? 0.f : (float)(q[i] / kElementaryCharge);
+  src[2 * i] = make_float4(xh, yh, zh, qe);
+  src[2 * i + 1] = make_float4((float)(xs - (double)xh), (float)(ys - (double)yh), (float)(zs - (double)zh), 0.f);
+}
+
+// AllPairsReduceKernel with interleaved output force[3i..3i+2]; zero for annihilated or neutral i.
+__global__ void AllPairsReduceAosKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
+                                        const double* __restrict__ q, const uint8_t* __restrict__ annihilated,
+                                        double* __restrict__ force) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const double* p = part + (size_t)c * 3 * n_pad;
+    sx += p[i];
+    sy += p[n_pad + i];
+    sz += p[2 * (size_t)n_pad + i];
+  }
+  const bool live = !(annihilated && annihilated[i]) && q[i] != 0.0;
+  const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
+  force[3 * i] = scale * sx;
+  force[3 * i + 1] = scale * sy;
+  force[3 * i + 2] = scale * sz;
+}
+
+}  // namespace am
+
+extern "C" int am_allpairs_forces_noalloc(int64_t n, const double* pos, const double* charge,
+                                          const uint8_t* annihilated, double* force, void* stream) {
+  if (n < 0 || n > AM_NOALLOC_MAX_N)
+    return Fail(AM_ERR_CAPACITY, "am_allpairs_forces_noalloc: n must be in [0, AM_NOALLOC_MAX_N = 65536]");
+  if (n == 0) return AM_OK;
+  if (!pos || !charge || !force) return Fail(AM_ERR_INVALID, "am_allpairs_forces_noalloc: NULL buffer");
+  int dev = -1;
+  AM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return Fail(AM_ERR_INVALID, "am_allpairs_forces_noalloc: device id");
+  NoAllocDevice* d = nullptr;
+  int rc = NoAllocSetup(dev, &d);
+  if (rc) return rc;
+  const int ni = (int)n;
+  const int n_pad = PadN(n);
+  const int tblocks = n_pad / kAllPairsThreads;
+  const int chunks = ChunkCount(d->num_cus, tblocks, n_pad);
+  const int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
+  const int nch = (n_pad + chunk - 1) / chunk;
+  const size_t mirror_bytes = (size_t)2 * n_pad * sizeof(float4);
+  const size_t part_bytes = (size_t)nch * 3 * n_pad * sizeof(double);
+  if (mirror_bytes + part_bytes > kNoAllocScratchBytes)
+    return Fail(AM_ERR_CAPACITY, "am_allpairs_forces_noalloc: scratch of this size exceeds the 16 MiB array");
+  float4* src = static_cast<float4*>(d->ptr);
+  double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(d->ptr) + mirror_bytes);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(PackSourceAosKernel, dim3(Blocks(n_pad, 256)), dim3(256), 0, st, ni, n_pad, pos, charge,
+                     annihilated, src);
+  hipLaunchKernelGGL(AllPairsF32Kernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, st, (const float4*)src, n_pad,
+                     chunk, part);
+  hipLaunchKernelGGL(AllPairsReduceAosKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, ni, n_pad, nch,
+                     (const double*)part, charge, annihilated, force);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
 }

@@ -140,6 +140,19 @@ int am_compute_standard_coulomb_forces(am_store* store, double cutoff_radius, in
 /* Exact all-pairs force without cutoff: the theta = 0 limit of ComputeBarnesHutCoulombForces
  * (src/interaction/barnes-hut.h:22-24) with the AddPointCharge pair term (barnes-hut.cc:19-27). */
 int am_compute_allpairs_coulomb_forces(am_store* store, int precision);
+/* All-pairs FP32 forces that allocate nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The arithmetic is that of am_compute_allpairs_coulomb_forces(AM_FP32)
+ * below 4,096 particles (the unsorted tile kernel, bit-identical there); the inputs and output are
+ * caller-owned DEVICE buffers: pos 3n doubles (x0,y0,z0,x1,...) in m, charge n doubles in C,
+ * annihilated n bytes or NULL, force 3n doubles out in N (zero for annihilated or neutral
+ * particles).  Work is queued on `stream` (a hipStream_t; NULL = the null stream) of the current
+ * device, and the call returns without synchronising.  Scratch is a 16 MiB file-scope device array
+ * per GPU, held until the process ends; n above AM_NOALLOC_MAX_N returns AM_ERR_CAPACITY before any
+ * launch.  Calls that overlap on one GPU share that array and race: serialise them. */
+#define AM_NOALLOC_MAX_N 65536
+int am_allpairs_forces_noalloc(int64_t n, const double* pos, const double* charge, const uint8_t* annihilated,
+                               double* force, void* stream);
 /* ComputeBarnesHutCoulombForces (src/interaction/barnes-hut.h:76, .cc:271-299): dual-monopole
  * octree, leaf capacity 8, MAC (2h)^2 < theta^2 |x - c|^2 against the geometric centre. */
 int am_compute_barnes_hut_coulomb_forces(am_store* store, double theta, int precision);
