@@ -28,7 +28,7 @@ def counts_array(populations: dict):
 
 
 def derive_dt(populations: dict, b0: float) -> float:
-    """PhysicsParams dt rule (src/params/physics_params.cc:115-129)."""
+    """PhysicsParams dt rule (src/params/physics_params.cc:60-74)."""
     _, listed = counts_array(populations)
     return float(lib().am_derive_dt(_p(listed), b0))
 

@@ -106,12 +106,12 @@ SPECIES4 = ("positron", "electron", "antiproton", "proton")
 
 def make_config5(n_traps: int = 512, seed: int = 42, contact_radius: float = 5e-6, probability: float = 1.0,
                  bath_temp: float = 4.0, tau_scale: float = 1.0, n_min: int = 1024, n_max: int = 32768,
-                 annihilation: bool = True):
+                 annihilation: bool = True, radii_range=(1e-4, 2e-3)):
     """The batched workload: per trap, drawn from one seed, N log-uniform in [n_min, n_max], species
     fractions Dirichlet(1,1,1,1) over {e+, e-, pbar, p}, T log-uniform in [4, 1e4] K (all species),
-    B0 uniform in [1, 5] T, U0 uniform in [5, 100] V, cloud radii uniform in [1e-4, 2e-3] m per axis,
-    z0 = rho0 = 0.01 m; positions and velocities from the reference generators with seed + t.
-    Returns (pos, vel, species, trap descriptors, per-trap parameter dicts)."""
+    B0 uniform in [1, 5] T, U0 uniform in [5, 100] V, cloud radii uniform in radii_range (the workload's
+    [1e-4, 2e-3] m) per axis, z0 = rho0 = 0.01 m; positions and velocities from the reference
+    generators with seed + t.  Returns (pos, vel, species, trap descriptors, per-trap parameter dicts)."""
     rng = np.random.default_rng(seed)
     pos_l, vel_l, sp_l, traps, params = [], [], [], [], []
     begin = 0
@@ -123,7 +123,7 @@ def make_config5(n_traps: int = 512, seed: int = 42, contact_radius: float = 5e-
         T = float(math.exp(rng.uniform(math.log(4.0), math.log(1e4))))
         b0 = float(rng.uniform(1.0, 5.0))
         u0 = float(rng.uniform(5.0, 100.0))
-        radii = tuple(float(r) for r in rng.uniform(1e-4, 2e-3, 3))
+        radii = tuple(float(r) for r in rng.uniform(radii_range[0], radii_range[1], 3))
         pops = {name: int(c) for name, c in zip(SPECIES4, counts) if c > 0}
         temps = {name: T for name in pops}
         pos, vel, sp = initial_conditions(pops, temps, radii, seed + t)

@@ -53,7 +53,9 @@ __global__ __launch_bounds__(kAllPairsThreads) void AllPairsF32Kernel(const floa
 
 // Sum chunk partials in order and scale to newtons: F = k_C q_i e * 2^40 * acc.
 __global__ void AllPairsReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
-                                     const double* __restrict__ q, const uint8_t* __restrict__ flags,
+                                     const double* __restrict__ x, const double* __restrict__ y,
+                                     const double* __restrict__ z, const double* __restrict__ q,
+                                     const uint8_t* __restrict__ flags,
                                      double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -64,7 +66,7 @@ __global__ void AllPairsReduceKernel(int n, int n_pad, int chunks, const double*
     sy += p[n_pad + i];
     sz += p[2 * (size_t)n_pad + i];
   }
-  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0;
+  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0 && Finite3(x[i], y[i], z[i]);
   const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
   fx[i] = scale * sx;
   fy[i] = scale * sy;
@@ -77,7 +79,8 @@ __global__ void PackF64SourceKernel(int n, const double* __restrict__ x, const d
                                     const uint8_t* __restrict__ flags, double4* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = make_double4(x[i], y[i], z[i], (flags[i] & kFlagAnnihilated) ? 0.0 : q[i]);
+  const bool skip = (flags[i] & kFlagAnnihilated) || !Finite3(x[i], y[i], z[i]);
+  out[i] = make_double4(x[i], y[i], z[i], skip ? 0.0 : q[i]);
 }
 
 // AddPointCharge (barnes-hut.cc:19-27) in FP64, sources in ascending index order.
@@ -240,7 +243,9 @@ __global__ __launch_bounds__(kAllPairsThreads) void AllPairsSortedKernel(
 }
 
 __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
-                                           const uint32_t* __restrict__ perm, const double* __restrict__ q,
+                                           const uint32_t* __restrict__ perm, const double* __restrict__ x,
+                                           const double* __restrict__ y, const double* __restrict__ z,
+                                           const double* __restrict__ q,
                                            const uint8_t* __restrict__ flags, double* __restrict__ fx,
                                            double* __restrict__ fy, double* __restrict__ fz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,7 +258,7 @@ __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const d
     sz += p[2 * (size_t)n_pad + k];
   }
   const uint32_t i = perm[k];
-  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0;
+  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0 && Finite3(x[i], y[i], z[i]);
   const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
   fx[i] = scale * sx;
   fy[i] = scale * sy;
@@ -281,7 +286,11 @@ int AllPairsForcesF32Sorted(am_store* s) {
     hipLaunchKernelGGL(SpatialKeyKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, (int64_t)n, s->x, s->y, s->z,
                        (const double*)box, (uint64_t*)keys, s->perm);
     AM_HIP(hipGetLastError());
+    s->perm_n = -1;  // until the sort is known to have finished
     if ((rc = RadixSortPairs(ctx, (uint64_t*)keys, s->perm, n, 30))) return rc;
+    // The permutation is kept for kResortInterval calls, so a sort whose look-back gave up must not
+    // be kept: check it now (one synchronisation per resort).
+    if ((rc = CheckSortError(ctx))) return rc;
     s->perm_n = n;
     s->perm_age = 0;
   }
@@ -306,7 +315,8 @@ int AllPairsForcesF32Sorted(am_store* s) {
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   hipLaunchKernelGGL(AllPairsSortedReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, n_pad, nch,
-                     (const double*)part, (const uint32_t*)s->perm, s->q, s->flags, s->fx, s->fy, s->fz);
+                     (const double*)part, (const uint32_t*)s->perm, s->x, s->y, s->z, s->q, s->flags, s->fx, s->fy,
+                     s->fz);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -338,7 +348,7 @@ int AllPairsForcesF32(am_store* s) {
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   hipLaunchKernelGGL(AllPairsReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, n_pad, nch,
-                     (const double*)part, s->q, s->flags, s->fx, s->fy, s->fz);
+                     (const double*)part, s->x, s->y, s->z, s->q, s->flags, s->fx, s->fy, s->fz);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -449,7 +459,8 @@ static int NoAllocSetup(int dev, NoAllocDevice** out) {
   return AM_OK;
 }
 
-// RefreshSourceKernel's mirror from interleaved positions (pos[3i..3i+2]) and caller charges.
+// RefreshSourceKernel's mirror from interleaved positions (pos[3i..3i+2]) and caller charges (a zero
+// entry for annihilated or non-finite particles).
 __global__ void PackSourceAosKernel(int n, int n_pad, const double* __restrict__ pos, const double* __restrict__ q,
                                     const uint8_t* __restrict__ annihilated, float4* __restrict__ src) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,17 +470,14 @@ __global__ void PackSourceAosKernel(int n, int n_pad, const double* __restrict__
     src[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const double xs = pos[3 * i] * kUnitScale, ys = pos[3 * i + 1] * kUnitScale, zs = pos[3 * i + 2] * kUnitScale;
-  const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
-  const bool dead = annihilated && annihilated[i];
-  const float qe = dead ? 0.f : (float)(q[i] / kElementaryCharge);
-  src[2 * i] = make_float4(xh, yh, zh, qe);
-  src[2 * i + 1] = make_float4((float)(xs - (double)xh), (float)(ys - (double)yh), (float)(zs - (double)zh), 0.f);
+  WriteMirror(src, i, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], q[i], annihilated && annihilated[i]);
 }
 
-// AllPairsReduceKernel with interleaved output force[3i..3i+2]; zero for annihilated or neutral i.
+// AllPairsReduceKernel with interleaved output force[3i..3i+2]; zero for annihilated, neutral or
+// non-finite i.
 __global__ void AllPairsReduceAosKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
-                                        const double* __restrict__ q, const uint8_t* __restrict__ annihilated,
+                                        const double* __restrict__ pos, const double* __restrict__ q,
+                                        const uint8_t* __restrict__ annihilated,
                                         double* __restrict__ force) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -480,7 +488,8 @@ __global__ void AllPairsReduceAosKernel(int n, int n_pad, int chunks, const doub
     sy += p[n_pad + i];
     sz += p[2 * (size_t)n_pad + i];
   }
-  const bool live = !(annihilated && annihilated[i]) && q[i] != 0.0;
+  const bool live =
+      !(annihilated && annihilated[i]) && q[i] != 0.0 && Finite3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
   const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
   force[3 * i] = scale * sx;
   force[3 * i + 1] = scale * sy;
@@ -519,7 +528,7 @@ extern "C" int am_allpairs_forces_noalloc(int64_t n, const double* pos, const do
   hipLaunchKernelGGL(AllPairsF32Kernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, st, (const float4*)src, n_pad,
                      chunk, part);
   hipLaunchKernelGGL(AllPairsReduceAosKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, ni, n_pad, nch,
-                     (const double*)part, charge, annihilated, force);
+                     (const double*)part, pos, charge, annihilated, force);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }

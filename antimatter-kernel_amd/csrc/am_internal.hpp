@@ -54,6 +54,21 @@ struct Buffer {
   size_t bytes = 0;
 };
 
+// Sizes of the most recent FMM call on a store (am_fmm_stats).
+struct FmmRunStats {
+  int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0, m2l_classes = 0;
+  int max_depth = 0;
+};
+
+// The most recent Barnes-Hut build on a store (am_bh_tree_stats / am_bh_tree_download).  order and
+// nodes point into the context's scratch: valid until the next Barnes-Hut build on that context.
+struct BHBuildInfo {
+  int m = 0, nn = 0, max_depth = 0;
+  int64_t deep_overflow = 0;
+  const uint32_t* order = nullptr;
+  const void* nodes = nullptr;  // WalkNode[nn]
+};
+
 }  // namespace am
 
 struct am_ctx {
@@ -95,6 +110,11 @@ struct am_store {
   uint32_t* perm = nullptr;
   int64_t perm_n = -1;
   int perm_age = 0;
+  // Bumped by every am_store_upload: objects built over the store (ensembles) record it at creation
+  // and refuse to run once the particle set they were sized for has been replaced.
+  uint64_t generation = 0;
+  am::FmmRunStats fmm_stats;
+  am::BHBuildInfo bh_info;
 };
 
 namespace am {
@@ -159,6 +179,27 @@ __device__ inline double Dot3(double ax, double ay, double az, double bx, double
   return __dadd_rn(__dadd_rn(__dmul_rn(ax, bx), __dmul_rn(ay, by)), __dmul_rn(az, bz));
 }
 
+__host__ __device__ inline bool Finite3(double x, double y, double z) {
+  return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
+// Particle i's entry of the FP32 source mirror (see am_store::src).  An annihilated particle, or one
+// with a non-finite coordinate, gets an all-zero entry (position 0, charge 0): it then acts on no
+// pair sum, as the direct solver skips non-finite positions (coulomb_interaction.cc:55-56), and it
+// cannot turn another particle's sum into NaN.
+__device__ inline void WriteMirror(float4* __restrict__ src, int64_t i, double x, double y, double z, double q,
+                                   bool dead) {
+  if (dead || !Finite3(x, y, z)) {
+    src[2 * i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    src[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const double xs = x * kUnitScale, ys = y * kUnitScale, zs = z * kUnitScale;
+  const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
+  src[2 * i] = make_float4(xh, yh, zh, (float)(q / kElementaryCharge));
+  src[2 * i + 1] = make_float4((float)(xs - (double)xh), (float)(ys - (double)yh), (float)(zs - (double)zh), 0.f);
+}
+
 // FP32 pair interaction on hi/lo coordinates (units of 2^-20 m): accumulates
 // q_j (x_i - x_j) / (r^2 + eps^2)^(3/2) into (px, py, pz).  dx = (xh_i - xh_j) + (xl_i - xl_j) keeps
 // ~2^-24 relative accuracy on dx even for pairs much closer than the coordinates' magnitude.
@@ -177,6 +218,9 @@ __device__ __forceinline__ void PairF32(const float4 th, const float4 tl, const 
 
 // Launch helpers ----------------------------------------------------------------------------------
 int RefreshSource(am_store* s);  // rebuild store->src from the FP64 state
+// Zeroes the Coulomb force of alive particles with a non-finite coordinate (the solvers leave them
+// out of their trees and pair sums, as coulomb_interaction.cc:55-56 skips them).
+int ZeroNonFiniteForces(am_store* s);
 int64_t CountActive(am_store* s, int* status);
 
 // Radix sort (stable, LSD) of 64-bit keys with 32-bit values on the keys' low key_bits bits (higher
@@ -187,11 +231,13 @@ int RadixSortPairs(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int k
 int RadixSortKeys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);
 // Deterministic FP64 sum of n device doubles into *host_out (synchronises the stream).
 int ReduceSum(am_ctx* ctx, const double* in, int64_t n, double* host_out);
-// The radix sort's look-back error word on the device (sticky: set by a wait that gave up, never
-// cleared); callers read it with a synchronisation they make anyway and pass it to
-// CheckSortErrorAfterSync.
+// The radix sort's look-back error word on the device (set by a wait that gave up); callers read it
+// with a synchronisation they make anyway and pass it to CheckSortErrorAfterSync, which fails the
+// call and clears the word, so one give-up is reported once.  CheckSortError reads it with a
+// synchronisation of its own.
 uint32_t* SortErrorWord(am_ctx* ctx);
-int CheckSortErrorAfterSync(uint32_t word);
+int CheckSortErrorAfterSync(am_ctx* ctx, uint32_t word);
+int CheckSortError(am_ctx* ctx);
 // Pre-sizes the scratch of RadixSortPairs / ExclusiveScan for n elements (no allocation later).
 int ReserveSortScratch(am_ctx* ctx, int64_t n);
 // Exclusive prefix sum of n uint32 counts into out (n+1 entries; out[n] = total).

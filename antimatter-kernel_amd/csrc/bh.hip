@@ -444,7 +444,7 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   if ((rc = Scratch(ctx, "bh_root", sizeof(RootBox) + 6 * 8, &root))) return rc;
   double* box = (double*)((char*)root + sizeof(RootBox));
   int m = 0;
-  if ((rc = GatherParticles(ctx, n, s->flags, s->q, 0, (uint32_t*)gathered, &m, "bh"))) return rc;
+  if ((rc = GatherParticles(ctx, n, s->flags, s->q, s->x, s->y, s->z, 0, (uint32_t*)gathered, &m, "bh"))) return rc;
   t->m = m;
   t->nn = 0;
   if (m == 0) return AM_OK;
@@ -516,13 +516,18 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   return AM_OK;
 }
 
-static BHTree g_last_tree_info;  // diagnostics of the most recent build (host copy)
-
 int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, double* oz, int64_t* counts) {
   BHTree t;
   int rc = BuildBHTree(s, 8, &t);
   if (rc) return rc;
-  g_last_tree_info = t;
+  BHBuildInfo& info = s->bh_info;
+  info.m = t.m;
+  info.nn = t.nn;
+  info.max_depth = t.max_depth;
+  info.deep_overflow = t.deep_overflow;
+  info.order = t.order;
+  info.nodes = t.nodes;
+  if (mode != 2 && (rc = ZeroNonFiniteForces(s))) return rc;  // they are in no tree
   if (t.m == 0) return AM_OK;
   am_ctx* ctx = s->ctx;
   const double theta2 = theta * theta;
@@ -602,10 +607,10 @@ int am_barnes_hut_forces_counted(am_store* s, double theta, int precision, int64
 // depth-21 leaves hold more than 8 particles (the reference would split those further).
 int am_bh_tree_stats(am_store* s, int64_t* n_alive, int64_t* n_nodes, int32_t* max_depth, int64_t* deep_overflow) {
   if (!s) return Fail(AM_ERR_INVALID, "am_bh_tree_stats: NULL store");
-  if (n_alive) *n_alive = g_last_tree_info.m;
-  if (n_nodes) *n_nodes = g_last_tree_info.nn;
-  if (max_depth) *max_depth = g_last_tree_info.max_depth;
-  if (deep_overflow) *deep_overflow = g_last_tree_info.deep_overflow;
+  if (n_alive) *n_alive = s->bh_info.m;
+  if (n_nodes) *n_nodes = s->bh_info.nn;
+  if (max_depth) *max_depth = s->bh_info.max_depth;
+  if (deep_overflow) *deep_overflow = s->bh_info.deep_overflow;
   return AM_OK;
 }
 
@@ -615,12 +620,12 @@ int am_bh_tree_download(am_store* s, uint32_t* order, int32_t* begin, int32_t* e
                         double* qn) {
   if (!s) return Fail(AM_ERR_INVALID, "am_bh_tree_download: NULL store");
   am_ctx* ctx = s->ctx;
-  const int m = g_last_tree_info.m, nn = g_last_tree_info.nn;
+  const int m = s->bh_info.m, nn = s->bh_info.nn;
   if (order && m > 0)
-    AM_HIP(hipMemcpyAsync(order, g_last_tree_info.order, (size_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    AM_HIP(hipMemcpyAsync(order, s->bh_info.order, (size_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
   if (nn > 0 && (begin || end || leaf || qp || qn)) {
     std::vector<WalkNode> w(nn);
-    AM_HIP(hipMemcpyAsync(w.data(), g_last_tree_info.nodes, (size_t)nn * sizeof(WalkNode), hipMemcpyDeviceToHost,
+    AM_HIP(hipMemcpyAsync(w.data(), s->bh_info.nodes, (size_t)nn * sizeof(WalkNode), hipMemcpyDeviceToHost,
                           ctx->stream));
     AM_HIP(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < nn; ++k) {

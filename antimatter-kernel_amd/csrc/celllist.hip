@@ -1,6 +1,7 @@
 // Uniform cell lists and the kernels built on them:
 //   * ComputeStandardCoulombForces (src/interaction/coulomb_interaction.cc:23-80): pairs with
-//     r^2 < r_c^2 (the radius search of the reference's neighbour query), FP64.
+//     r^2 < r_c^2 (the radius search of the reference's neighbour query), in FP64 (the reference
+//     arithmetic) or with FP32 hi/lo pair math and the same pair set.
 //   * the shifted-cutoff standard energy (src/analysis/energy_analysis.cc:16-43), r^2 <= r_c^2.
 //   * AnnihilationInteraction (src/interaction/annihilation.cc:21-61), canonical order.
 //
@@ -257,6 +258,83 @@ __global__ __launch_bounds__(256) void StandardForceKernel(int64_t n, const doub
     fy[i] = ay;
     fz[i] = az;
   }
+}
+
+// The FP32 hi/lo mirror in cell-sorted order: a stencil cell's sources are then contiguous.
+__global__ void CellSortedMirrorKernel(int64_t n, const uint32_t* __restrict__ idx, const float4* __restrict__ src,
+                                       float4* __restrict__ ssrc) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  ssrc[2 * k] = src[2 * i];
+  ssrc[2 * k + 1] = src[2 * i + 1];
+}
+
+// Relative half-width of the band around r_c^2 inside which the FP32 r^2 (relative error below
+// ~3e-7) cannot decide the cutoff test: such pairs are decided in FP64, as the reference does.
+constexpr float kCutoffBand = 1e-5f;
+
+// ComputeStandardCoulombForces with FP32 pair math (coulomb_interaction.cc:60-73): one thread per
+// cell-sorted position, so the lanes of a wave mostly share their stencil cells; sources come
+// contiguously from the cell-sorted hi/lo mirror.  The pair set is the FP64 path's: r^2 < r_c^2
+// from the FP32 r^2 outside a narrow band around r_c^2, from the FP64 positions inside it.  Each
+// stencil cell is summed in FP32 and added to FP64 accumulators.  The 1e-30 m^2 softening makes
+// the self pair and coincident pairs contribute zero; annihilated particles are in no cell.
+__global__ __launch_bounds__(256) void StandardForceF32Kernel(
+    int64_t n, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ q, const uint8_t* __restrict__ flags, CellTable tab, const uint32_t* __restrict__ idx,
+    const float4* __restrict__ ssrc, double inv_h, double rc2, float rc2u, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  if (flags[i] & kFlagAnnihilated) return;
+  const double qi = q[i];
+  const double px = x[i], py = y[i], pz = z[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (qi != 0.0 && isfinite(px) && isfinite(py) && isfinite(pz)) {
+    const float4 th = ssrc[2 * k], tl = ssrc[2 * k + 1];
+    const float band = kCutoffBand * rc2u;
+    const int64_t cx = CellCoord(px, inv_h), cy = CellCoord(py, inv_h), cz = CellCoord(pz, inv_h);
+    const int64_t lim = (int64_t)1 << 21;
+    for (int dz = -1; dz <= 1; ++dz) {
+      if (cz + dz < 0 || cz + dz >= lim) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        if (cy + dy < 0 || cy + dy >= lim) continue;
+        for (int dx_ = -1; dx_ <= 1; ++dx_) {
+          if (cx + dx_ < 0 || cx + dx_ >= lim) continue;
+          const uint2 r = CellLookup(tab, CellKey(cx + dx_, cy + dy, cz + dz));
+          float sx = 0.f, sy = 0.f, sz = 0.f;
+          for (uint32_t kk = r.x; kk < r.x + r.y; ++kk) {
+            const float4 sh = ssrc[2 * kk], sl = ssrc[2 * kk + 1];
+            const float dx = (th.x - sh.x) + (tl.x - sl.x);
+            const float dy2 = (th.y - sh.y) + (tl.y - sl.y);
+            const float dz2 = (th.z - sh.z) + (tl.z - sl.z);
+            const float r2 = fmaf(dx, dx, fmaf(dy2, dy2, dz2 * dz2));
+            bool in = r2 < rc2u;
+            if (fabsf(r2 - rc2u) <= band) {
+              const uint32_t j = idx[kk];
+              const double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+              in = Dot3(rx, ry, rz, rx, ry, rz) < rc2;
+            }
+            if (!in) continue;
+            const float ri = __builtin_amdgcn_rsqf(r2 + kEps2Units);
+            const float s = (sh.w * ri) * (ri * ri);
+            sx = fmaf(s, dx, sx);
+            sy = fmaf(s, dy2, sy);
+            sz = fmaf(s, dz2, sz);
+          }
+          ax += (double)sx;
+          ay += (double)sy;
+          az += (double)sz;
+        }
+      }
+    }
+  }
+  const double scale = kCoulomb * qi * kElementaryCharge * kInvUnit2;
+  fx[i] = scale * ax;
+  fy[i] = scale * ay;
+  fz[i] = scale * az;
 }
 
 // energy_analysis.cc:22-41 over ordered pairs (the caller halves the sum): e_i = sum over
@@ -590,6 +668,21 @@ int am_compute_standard_coulomb_forces(am_store* s, double cutoff_radius, int pr
   int rc = BuildCellList(s, cutoff_radius, "cl_force", &cl);
   if (rc) return rc;
   int slot;
+  if (precision == AM_FP32) {
+    const double rcu = cutoff_radius * kUnitScale;
+    if (!(rcu * rcu < 1e30)) return Fail(AM_ERR_INVALID, "am_compute_standard_coulomb_forces: cutoff too large for FP32");
+    void* ssrc;
+    if ((rc = Scratch(ctx, "cl_force_ssrc", (size_t)s->n * 2 * sizeof(float4), &ssrc))) return rc;
+    hipLaunchKernelGGL(CellSortedMirrorKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, cl.idx,
+                       (const float4*)s->src, (float4*)ssrc);
+    TimerBegin(ctx, "standard_f32", &slot);
+    hipLaunchKernelGGL(StandardForceF32Kernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, s->x, s->y,
+                       s->z, s->q, s->flags, cl.tab, cl.idx, (const float4*)ssrc, cl.inv_h,
+                       cutoff_radius * cutoff_radius, (float)(rcu * rcu), s->fx, s->fy, s->fz);
+    TimerEnd(ctx, slot);
+    AM_HIP(hipGetLastError());
+    return AM_OK;
+  }
   TimerBegin(ctx, "standard", &slot);
   hipLaunchKernelGGL(StandardForceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, ctx->stream, s->n, s->x, s->y, s->z,
                      s->q, s->flags, cl.tab, cl.idx, cl.inv_h, cutoff_radius * cutoff_radius, s->fx, s->fy, s->fz);
@@ -625,7 +718,7 @@ int am_annihilation(am_store* s, double contact_radius, double probability, uint
   if (const uint32_t* w = SortErrorWord(ctx))
     AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
-  if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
+  if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
   if (total > 0) {
     void* pairs;
     if ((rc = Scratch(ctx, "contact_pairs", (size_t)total * 8, &pairs))) return rc;

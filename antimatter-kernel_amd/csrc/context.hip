@@ -97,11 +97,25 @@ __global__ void RefreshSourceKernel(int64_t n, int64_t n_pad, const double* __re
     src[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const double xs = x[i] * kUnitScale, ys = y[i] * kUnitScale, zs = z[i] * kUnitScale;
-  const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
-  const float qe = (flags[i] & kFlagAnnihilated) ? 0.f : (float)(q[i] / kElementaryCharge);
-  src[2 * i] = make_float4(xh, yh, zh, qe);
-  src[2 * i + 1] = make_float4((float)(xs - (double)xh), (float)(ys - (double)yh), (float)(zs - (double)zh), 0.f);
+  WriteMirror(src, i, x[i], y[i], z[i], q[i], (flags[i] & kFlagAnnihilated) != 0);
+}
+
+__global__ void ZeroNonFiniteForceKernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                         const double* __restrict__ z, const uint8_t* __restrict__ flags,
+                                         double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (flags[i] & kFlagAnnihilated) || Finite3(x[i], y[i], z[i])) return;
+  fx[i] = 0.0;
+  fy[i] = 0.0;
+  fz[i] = 0.0;
+}
+
+int ZeroNonFiniteForces(am_store* s) {
+  if (s->n == 0) return AM_OK;
+  hipLaunchKernelGGL(ZeroNonFiniteForceKernel, dim3(Blocks(s->n, 256)), dim3(256), 0, s->ctx->stream, s->n, s->x,
+                     s->y, s->z, s->flags, s->fx, s->fy, s->fz);
+  AM_HIP(hipGetLastError());
+  return AM_OK;
 }
 
 int RefreshSource(am_store* s) {
@@ -310,6 +324,7 @@ int am_store_upload(am_store* s, int64_t n, const double* pos, const double* vel
   hipStream_t st = s->ctx->stream;
   AM_HIP(hipSetDevice(s->ctx->device));
   s->n = n;
+  s->generation += 1;
   s->perm_n = -1;  // spatial order must be rebuilt
   if (n == 0) return AM_OK;
   std::vector<double> hq(n), hm(n);

@@ -203,8 +203,10 @@ __global__ __launch_bounds__(kEnsThreads) void EnsembleForceKernel(const EnsItem
 // One thread per position of the sorted layout: the fixed-order sum over chunks, scaled into N.
 __global__ void EnsembleReduceKernel(int64_t n_pos, const int* __restrict__ tile_trap,
                                      const TrapRec* __restrict__ traps, const uint32_t* __restrict__ perm,
-                                     const double* __restrict__ part, const double* __restrict__ q,
-                                     const uint8_t* __restrict__ flags, double* __restrict__ fx,
+                                     const double* __restrict__ part, const double* __restrict__ x,
+                                     const double* __restrict__ y, const double* __restrict__ z,
+                                     const double* __restrict__ q, const uint8_t* __restrict__ flags,
+                                     double* __restrict__ fx,
                                      double* __restrict__ fy, double* __restrict__ fz) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_pos) return;
@@ -220,7 +222,7 @@ __global__ void EnsembleReduceKernel(int64_t n_pos, const int* __restrict__ tile
     sz += p[2 * npad + local];
   }
   const uint32_t i = perm[k];
-  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0;
+  const bool live = !(flags[i] & kFlagAnnihilated) && q[i] != 0.0 && Finite3(x[i], y[i], z[i]);
   const double scale = live ? kCoulomb * q[i] * kElementaryCharge * kInvUnit2 : 0.0;
   fx[i] = scale * sx;
   fy[i] = scale * sy;
@@ -469,9 +471,21 @@ struct am_ensemble {
   uint64_t graph_seed = 0;
   bool graph_valid = false;
   std::vector<void*> owned;
+  // The store's particle set at creation: every buffer above is sized for n_at_create, and the
+  // trap ids written then must still be in place.
+  uint64_t store_generation = 0;
+  int64_t n_at_create = 0;
 };
 
 namespace am {
+
+// Fails (before any launch) when the store was re-uploaded after am_ensemble_create.
+static int EnsCheckStore(const am_ensemble* e, const char* who) {
+  if (e->store->generation != e->store_generation || e->store->n != e->n_at_create)
+    return Fail(AM_ERR_INVALID, std::string(who) + ": the store was re-uploaded after am_ensemble_create "
+                                                   "(buffers and trap ids are stale); create a new ensemble");
+  return AM_OK;
+}
 
 static int EnsAlloc(am_ensemble* e, size_t bytes, void** p) {
   AM_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
@@ -552,7 +566,7 @@ static int EnsForce(am_ensemble* e) {
   if (n_pos > 0)
     hipLaunchKernelGGL(EnsembleReduceKernel, dim3(Blocks(n_pos, 256)), dim3(256), 0, st, n_pos,
                        (const int*)e->tile_trap, (const TrapRec*)e->d_traps, (const uint32_t*)e->perm,
-                       (const double*)e->part, s->q, s->flags, s->fx, s->fy, s->fz);
+                       (const double*)e->part, s->x, s->y, s->z, s->q, s->flags, s->fx, s->fy, s->fz);
   AM_HIP(hipGetLastError());
   return AM_OK;
 }
@@ -713,6 +727,8 @@ int am_ensemble_create(am_store* s, int32_t n_traps, const am_trap_desc* desc, i
     e->gmark = (uint8_t*)p;
   }
   if ((rc = EnsBuildItems(e, 4096))) return fail(rc);
+  e->store_generation = s->generation;
+  e->n_at_create = s->n;
   *out = e;
   return AM_OK;
 }
@@ -729,6 +745,7 @@ int am_ensemble_destroy(am_ensemble* e) {
 
 int am_ensemble_set_chunk(am_ensemble* e, int32_t chunk) {
   if (!e || chunk < kEnsTile) return Fail(AM_ERR_INVALID, "am_ensemble_set_chunk: chunk must be >= 128");
+  if (int rc = EnsCheckStore(e, "am_ensemble_set_chunk")) return rc;
   return EnsBuildItems(e, chunk);
 }
 
@@ -737,6 +754,7 @@ int am_ensemble_set_chunk(am_ensemble* e, int32_t chunk) {
 int am_ensemble_autotune(am_ensemble* e, const int32_t* candidates, int32_t n_candidates, int32_t* chosen,
                          double* times_ms) {
   if (!e) return Fail(AM_ERR_INVALID, "am_ensemble_autotune: NULL ensemble");
+  if (int rc = EnsCheckStore(e, "am_ensemble_autotune")) return rc;
   static const int32_t kDefault[] = {1024, 2048, 4096, 8192, 16384, 32768};
   if (!candidates || n_candidates <= 0) {
     candidates = kDefault;
@@ -769,6 +787,7 @@ int am_ensemble_autotune(am_ensemble* e, const int32_t* candidates, int32_t n_ca
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  if (int rc = CheckSortError(ctx)) return rc;
   if (chosen) *chosen = best_c;
   return EnsBuildItems(e, best_c);
 }
@@ -777,6 +796,7 @@ int am_ensemble_autotune(am_ensemble* e, const int32_t* candidates, int32_t n_ca
 // rebuilt when the seed or the work layout changes).
 int am_ensemble_run(am_ensemble* e, int32_t steps, uint64_t seed, int32_t use_graph) {
   if (!e || steps < 0) return Fail(AM_ERR_INVALID, "am_ensemble_run: bad arguments");
+  if (int rc = EnsCheckStore(e, "am_ensemble_run")) return rc;
   am_ctx* ctx = e->store->ctx;
   hipStream_t st = ctx->stream;
   AM_HIP(hipSetDevice(ctx->device));
@@ -806,10 +826,12 @@ int am_ensemble_run(am_ensemble* e, int32_t steps, uint64_t seed, int32_t use_gr
     for (int k = 0; k < steps; ++k) AM_HIP(hipGraphLaunch(e->exec, st));
   }
   TimerEnd(ctx, slot);
+  uint32_t ov = 0, sort_err = 0;
+  AM_HIP(hipMemcpyAsync(&ov, e->overflow, 4, hipMemcpyDeviceToHost, st));
+  if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
   AM_HIP(hipStreamSynchronize(st));
-  uint32_t ov = 0;
-  AM_HIP(hipMemcpy(&ov, e->overflow, 4, hipMemcpyDeviceToHost));
   e->steps_done += steps;
+  if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
   if (ov) return Fail(AM_ERR_CAPACITY, "am_ensemble_run: contact candidate buffer overflowed; raise contact_capacity");
   return AM_OK;
 }

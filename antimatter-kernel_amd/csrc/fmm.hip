@@ -1477,20 +1477,19 @@ static void LaunchL2PForce(int order, hipStream_t st, int nn, const Cell* cells,
 #undef AM_L2P_CASE
 }
 
+// Neutral particles (fast-multipole.cc:690-691) and particles with a non-finite coordinate (left out of
+// the tree) get zero force.
 __global__ void ZeroNeutralForceKernel(int64_t n, const uint8_t* __restrict__ flags, const double* __restrict__ q,
-                                       double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz) {
+                                       const double* __restrict__ x, const double* __restrict__ y,
+                                       const double* __restrict__ z, double* __restrict__ fx,
+                                       double* __restrict__ fy, double* __restrict__ fz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || (flags[i] & kFlagAnnihilated) || q[i] != 0.0) return;
+  if (i >= n || (flags[i] & kFlagAnnihilated) || (q[i] != 0.0 && Finite3(x[i], y[i], z[i]))) return;
   fx[i] = 0.0;
   fy[i] = 0.0;
   fz[i] = 0.0;
 }
 
-struct FmmRunStats {
-  int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0, m2l_classes = 0;
-  int max_depth = 0;
-};
-static FmmRunStats g_fmm_stats;
 
 // want_forces: forces into the store (sources = alive, q != 0); otherwise the potential energy
 // (sources = all alive particles, fast-multipole.cc:335-340 via energy_analysis.cc:71-81).
@@ -1503,8 +1502,8 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   FmmTab tab;
   if ((rc = DeviceTables(ctx, order, &tab))) return rc;
   if (want_forces) {
-    hipLaunchKernelGGL(ZeroNeutralForceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->flags, s->q, s->fx,
-                       s->fy, s->fz);
+    hipLaunchKernelGGL(ZeroNeutralForceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, s->flags, s->q, s->x,
+                       s->y, s->z, s->fx, s->fy, s->fz);
     AM_HIP(hipGetLastError());
   }
   void *gathered, *frame, *rx, *ry, *rz;
@@ -1513,9 +1512,9 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
   RootBox* root = (RootBox*)((char*)frame + sizeof(FmmFrame));
   double* box = (double*)((char*)root + sizeof(RootBox));
   int m = 0;
-  if ((rc = GatherParticles(ctx, n, s->flags, s->q, want_forces ? 1 : 0, (uint32_t*)gathered, &m, "fmm"))) return rc;
+  if ((rc = GatherParticles(ctx, n, s->flags, s->q, s->x, s->y, s->z, want_forces ? 1 : 0, (uint32_t*)gathered, &m, "fmm"))) return rc;
   if (pe) *pe = 0.0;
-  g_fmm_stats = FmmRunStats();
+  s->fmm_stats = FmmRunStats();
   if (m == 0) return AM_OK;
   if ((rc = GatheredBBox(ctx, (const uint32_t*)gathered, m, s->x, s->y, s->z, box, "fmm"))) return rc;
   hipLaunchKernelGGL(FrameKernel, dim3(1), dim3(1), 0, st, (const double*)box, (FmmFrame*)frame, root);
@@ -1677,7 +1676,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     AM_HIP(hipMemcpyAsync(&n_cls, (uint32_t*)ids + tsize, 4, hipMemcpyDeviceToHost, st));
     if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
-    if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
+    if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
     void *ckey, *Dc;
     if ((rc = Scratch(ctx, "fmm_cls_key", (size_t)std::max<uint32_t>(n_cls, 1) * 8, &ckey))) return rc;
     if ((rc = Scratch(ctx, "fmm_cls_tensor", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 8, &Dc))) return rc;
@@ -1747,7 +1746,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     else if (dpl <= 5) launch(M2LKernel<5, false>);
     else if (dpl <= 8) launch(M2LKernel<8, false>);
     else launch(M2LKernel<16, false>);
-    g_fmm_stats.m2l_classes = n_cls;
+    s->fmm_stats.m2l_classes = n_cls;
   }
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
@@ -1784,7 +1783,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     AM_HIP(hipMemcpyAsync(&n_items, (uint32_t*)ibase + nn, 4, hipMemcpyDeviceToHost, st));
     if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
-    if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
+    if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
     if ((rc = Scratch(ctx, "fmm_p2p_items", (size_t)std::max<uint32_t>(n_items, 1) * sizeof(P2PItem), &items)))
       return rc;
     if ((rc = Scratch(ctx, "fmm_p2p_part", (size_t)std::max<uint32_t>(n_items, 1) * 32 * 3 * 8, &part))) return rc;
@@ -1808,11 +1807,11 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                        (const FmmFrame*)frame, want_forces ? 1 : 0, s->fx, s->fy, s->fz, (double*)phi);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
-  g_fmm_stats.cells = nn;
-  g_fmm_stats.m2l_pairs = n_m2l;
-  g_fmm_stats.p2p_cell_pairs = n_p2p;
-  g_fmm_stats.m = m;
-  g_fmm_stats.max_depth = nlevels - 1;
+  s->fmm_stats.cells = nn;
+  s->fmm_stats.m2l_pairs = n_m2l;
+  s->fmm_stats.p2p_cell_pairs = n_p2p;
+  s->fmm_stats.m = m;
+  s->fmm_stats.max_depth = nlevels - 1;
   if (!want_forces) {
     double sum = 0.0;
     if ((rc = ReduceSum(ctx, (const double*)phi, m, &sum))) return rc;
@@ -1843,8 +1842,9 @@ int am_compute_fast_multipole_coulomb_forces(am_store* s, int order, double thet
   return FastMultipoleRun(s, order, theta, true, nullptr);
 }
 
-// FMM with a choice of near-field arithmetic: AM_FP64 (the reference arithmetic) or AM_FP32 (P2P on
-// hi/lo coordinates, FP64 per-cell accumulation; expansions stay FP64).
+// FMM with a choice of arithmetic: AM_FP64 (the reference arithmetic) or AM_FP32 (P2P on hi/lo FP32
+// coordinates with FP64 accumulation per work item; for orders <= 5 the M2L contraction on FP32
+// matrix cores; the tree, P2M, M2M, L2L and L2P stay FP64).
 int am_compute_fast_multipole_coulomb_forces_ex(am_store* s, int order, double theta, int precision) {
   if (!s) return Fail(AM_ERR_INVALID, "am_compute_fast_multipole_coulomb_forces_ex: NULL store");
   if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
@@ -1859,12 +1859,12 @@ int am_compute_fast_multipole_coulomb_forces_ex(am_store* s, int order, double t
 int am_fmm_stats(am_store* s, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs, int64_t* p2p_cell_pairs,
                  int32_t* max_depth, int64_t* m2l_classes) {
   if (!s) return Fail(AM_ERR_INVALID, "am_fmm_stats: NULL store");
-  if (n_sources) *n_sources = g_fmm_stats.m;
-  if (n_cells) *n_cells = g_fmm_stats.cells;
-  if (m2l_pairs) *m2l_pairs = g_fmm_stats.m2l_pairs;
-  if (p2p_cell_pairs) *p2p_cell_pairs = g_fmm_stats.p2p_cell_pairs;
-  if (max_depth) *max_depth = g_fmm_stats.max_depth;
-  if (m2l_classes) *m2l_classes = g_fmm_stats.m2l_classes;
+  if (n_sources) *n_sources = s->fmm_stats.m;
+  if (n_cells) *n_cells = s->fmm_stats.cells;
+  if (m2l_pairs) *m2l_pairs = s->fmm_stats.m2l_pairs;
+  if (p2p_cell_pairs) *p2p_cell_pairs = s->fmm_stats.p2p_cell_pairs;
+  if (max_depth) *max_depth = s->fmm_stats.max_depth;
+  if (m2l_classes) *m2l_classes = s->fmm_stats.m2l_classes;
   return AM_OK;
 }
 

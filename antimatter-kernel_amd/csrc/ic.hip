@@ -5,7 +5,7 @@
 //   SetParticleVelocities (src/fields/penning_trap.cc:51-115): per species a fresh
 //       std::mt19937(seed) and normal_distribution(0, sqrt(k_B T / m)); 3*count draws, the
 //       species mean removed and the set rescaled by sqrt(count / (count - 1)).
-//   dt = pi * min(1e-6, min_listed m/|q|) / (2 b0)  (src/params/physics_params.cc:115-129).
+//   dt = pi * min(1e-6, min_listed m/|q|) / (2 b0)  (src/params/physics_params.cc:60-74).
 // These use libstdc++'s <random> (host code), so the draws are the ones the reference's build
 // makes with the same standard library.
 #include <cmath>
@@ -21,7 +21,7 @@ extern "C" {
 int am_set_particle_positions(const uint64_t counts[6], const double radii[3], int seed, double* out) {
   if (!counts || !radii || !out) return Fail(AM_ERR_INVALID, "am_set_particle_positions: NULL argument");
   if (!(radii[0] > 0.0 && radii[1] > 0.0 && radii[2] > 0.0))
-    return Fail(AM_ERR_INVALID, "All components of initial_cloud_radii must be > 0.");  // physics_params.cc:147-150
+    return Fail(AM_ERR_INVALID, "am_set_particle_positions: every cloud radius must be positive");  // physics_params.cc:148-151
   std::default_random_engine rng(seed);
   std::uniform_real_distribution<double> ux(-radii[0], radii[0]);
   std::uniform_real_distribution<double> uy(-radii[1], radii[1]);
@@ -50,7 +50,7 @@ int am_set_particle_velocities(const uint64_t counts[6], const double temperatur
   uint64_t total = 0;
   for (int s = 0; s < 6; ++s) total += counts[s];
   if (total == 0)  // penning_trap.cc:55
-    return Fail(AM_ERR_INVALID, "PenningTrapField: Total particles in PenningTrap should be > 0");
+    return Fail(AM_ERR_INVALID, "am_set_particle_velocities: the populations hold no particles");
   int64_t k = 0;
   for (int s = 0; s < 6; ++s) {
     const uint64_t count = counts[s];

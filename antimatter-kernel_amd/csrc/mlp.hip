@@ -903,6 +903,7 @@ int am_compute_barnes_hut_learned_forces(am_store* s, double theta, am_mlp* net,
   if ((rc = MlpRefreshFrag(net))) return rc;
   BHTree t;
   if ((rc = BuildBHTree(s, 8, &t, true))) return rc;
+  if ((rc = ZeroNonFiniteForces(s))) return rc;  // they are in no tree
   if (t.m == 0) return AM_OK;
   void *rec, *cnt, *dstats;
   if ((rc = Scratch(ctx, "bhl_rec", (size_t)t.m * rec_cap * 4, &rec))) return rc;

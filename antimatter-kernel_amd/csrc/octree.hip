@@ -7,11 +7,14 @@
 
 namespace am {
 
-__global__ void KeepFlagKernel(int64_t n, const uint8_t* __restrict__ flags, const double* __restrict__ q, int mode,
-                               uint32_t* __restrict__ out) {
+// Keeps alive particles with finite coordinates (non-finite ones stay out of every tree, as the
+// direct solver skips them, coulomb_interaction.cc:55-56), and in mode 1 only charged ones.
+__global__ void KeepFlagKernel(int64_t n, const uint8_t* __restrict__ flags, const double* __restrict__ q,
+                               const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, int mode, uint32_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const bool alive = !(flags[i] & kFlagAnnihilated);
+  const bool alive = !(flags[i] & kFlagAnnihilated) && Finite3(x[i], y[i], z[i]);
   out[i] = (alive && (mode == 0 || q[i] != 0.0)) ? 1u : 0u;
 }
 
@@ -23,14 +26,15 @@ __global__ void CompactFlagKernel(int64_t n, const uint32_t* __restrict__ keep, 
 
 static std::string Name(const char* tag, const char* what) { return std::string(tag) + "_" + what; }
 
-int GatherParticles(am_ctx* ctx, int64_t n, const uint8_t* flags, const double* q, int mode, uint32_t* out,
-                    int* m_out, const char* tag) {
+int GatherParticles(am_ctx* ctx, int64_t n, const uint8_t* flags, const double* q, const double* x, const double* y,
+                    const double* z, int mode, uint32_t* out, int* m_out, const char* tag) {
   hipStream_t st = ctx->stream;
   void *keep, *off;
   int rc;
   if ((rc = Scratch(ctx, Name(tag, "keep").c_str(), (size_t)n * 4, &keep))) return rc;
   if ((rc = Scratch(ctx, Name(tag, "keepoff").c_str(), (size_t)(n + 1) * 4, &off))) return rc;
-  hipLaunchKernelGGL(KeepFlagKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, flags, q, mode, (uint32_t*)keep);
+  hipLaunchKernelGGL(KeepFlagKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, flags, q, x, y, z, mode,
+                     (uint32_t*)keep);
   AM_HIP(hipGetLastError());
   if ((rc = ExclusiveScan(ctx, (const uint32_t*)keep, (uint32_t*)off, n))) return rc;
   hipLaunchKernelGGL(CompactFlagKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, (const uint32_t*)keep,
@@ -512,7 +516,7 @@ int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, 
   AM_HIP(hipMemcpyAsync(&ov, over, 4, hipMemcpyDeviceToHost, st));
   if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
   AM_HIP(hipStreamSynchronize(st));
-  if ((rc = CheckSortErrorAfterSync(sort_err))) return rc;
+  if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
   t->nn = nn;
   t->order = (uint32_t*)vals;
   t->pre = (uint32_t*)pids;

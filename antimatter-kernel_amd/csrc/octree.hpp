@@ -51,10 +51,10 @@ struct Octree {
 int BuildOctree(am_ctx* ctx, const uint32_t* particles, int m, const double* x, const double* y, const double* z,
                 const double* q, const RootBox* root, int leaf_cap, const char* tag, Octree* t, double4* xq);
 
-// Compacts the indices i < n with keep(flags[i], q[i]) into out (ascending); returns count.
-// mode 0: alive; mode 1: alive and q != 0.
-int GatherParticles(am_ctx* ctx, int64_t n, const uint8_t* flags, const double* q, int mode, uint32_t* out,
-                    int* m_out, const char* tag);
+// Compacts the indices i < n of alive particles with finite coordinates into out (ascending);
+// returns count.  mode 0: all of them; mode 1: those with q != 0.
+int GatherParticles(am_ctx* ctx, int64_t n, const uint8_t* flags, const double* q, const double* x, const double* y,
+                    const double* z, int mode, uint32_t* out, int* m_out, const char* tag);
 
 // Bounding box of the gathered particles: box_dev receives lo[3], hi[3] (device).
 int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const double* y, const double* z,

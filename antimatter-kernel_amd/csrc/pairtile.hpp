@@ -38,7 +38,8 @@ __device__ inline uint32_t Spread10(uint32_t v) {
 // coordinates local to the tile's box centre (sloc) and the box (tiles[tile]).  The tile's slice of
 // the permutation is reordered stably by charge sign (q > 0, q < 0, q = 0), so the pair loops can
 // fold a unit charge's sign into the direction of the coordinate difference.  Padding entries
-// (t >= nvalid) have q = 0 and keep their positions.
+// (t >= nvalid) have q = 0 and keep their positions.  The block box of a tile without a finite
+// particle is the zero box at the origin.
 __device__ inline void TilePrepBody(int first, int nvalid, int tile, uint32_t* __restrict__ perm,
                                     const double* __restrict__ x, const double* __restrict__ y,
                                     const double* __restrict__ z, const float4* __restrict__ src,
@@ -61,9 +62,12 @@ __device__ inline void TilePrepBody(int first, int nvalid, int tile, uint32_t* _
     h = src[2 * i];
     l = src[2 * i + 1];
   }
+  // A particle with a non-finite coordinate has a zero mirror entry (charge 0): it stays out of the
+  // box and gets a zero local coordinate, so it adds exactly nothing to any pair sum.
+  const bool fin = valid && Finite3(X[0], X[1], X[2]);
   for (int d = 0; d < 3; ++d) {
-    lo[d][t] = valid ? X[d] : INFINITY;
-    hi[d][t] = valid ? X[d] : -INFINITY;
+    lo[d][t] = fin ? X[d] : INFINITY;
+    hi[d][t] = fin ? X[d] : -INFINITY;
   }
   // stable partition by charge class (0: q > 0, 1: q < 0, 2: q = 0)
   const int cls = h.w > 0.f ? 0 : (h.w < 0.f ? 1 : 2);
@@ -94,7 +98,7 @@ __device__ inline void TilePrepBody(int first, int nvalid, int tile, uint32_t* _
   if (valid) perm[slot] = i;  // every lane read its entry before the first barrier
   shl[2 * slot] = h;
   shl[2 * slot + 1] = l;
-  sloc[slot] = valid ? make_float4((float)(X[0] - c[0]), (float)(X[1] - c[1]), (float)(X[2] - c[2]), h.w)
+  sloc[slot] = fin ? make_float4((float)(X[0] - c[0]), (float)(X[1] - c[1]), (float)(X[2] - c[2]), h.w)
                      : make_float4(0.f, 0.f, 0.f, 0.f);
   if (t == 0) {
     TileRec r;

@@ -127,10 +127,7 @@ __device__ inline void PushParticle(int64_t i, double* __restrict__ x, double* _
     x[i] = pos.x;
     y[i] = pos.y;
     z[i] = pos.z;
-    const double xs = pos.x * kUnitScale, ys = pos.y * kUnitScale, zs = pos.z * kUnitScale;
-    const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
-    src[2 * i] = make_float4(xh, yh, zh, (float)(qi / kElementaryCharge));
-    src[2 * i + 1] = make_float4((float)(xs - (double)xh), (float)(ys - (double)yh), (float)(zs - (double)zh), 0.f);
+    WriteMirror(src, i, pos.x, pos.y, pos.z, qi, false);
   }
 }
 

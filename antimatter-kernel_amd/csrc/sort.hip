@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kOsThreads) void OsScatterKernel(
 
 // Words of the single-pass sort's scratch (cleared by every sort): histograms [passes][256], digit
 // bases [passes][256], tile counters [passes], then the status words [passes][tiles][256].  The
-// look-back error word is kept apart (SortErrorWord) and is never cleared by a sort.
+// look-back error word is kept apart (SortErrorWord); no sort clears it, only its report does.
 static int64_t OsMetaWords(int passes) { return (int64_t)passes * 512 + passes; }
 
 __global__ void OsZeroWordKernel(uint32_t* w) { *w = 0u; }
@@ -428,8 +428,20 @@ uint32_t* SortErrorWord(am_ctx* ctx) {
   return (uint32_t*)b.ptr;
 }
 
-int CheckSortErrorAfterSync(uint32_t word) {
-  return word ? Fail(AM_ERR_HIP, "radix sort: a look-back wait on the device gave up") : AM_OK;
+int CheckSortErrorAfterSync(am_ctx* ctx, uint32_t word) {
+  if (!word) return AM_OK;
+  // Reported once: the word is cleared (in stream order) so later calls are not failed for it.
+  if (const uint32_t* w = SortErrorWord(ctx))
+    hipLaunchKernelGGL(OsZeroWordKernel, dim3(1), dim3(1), 0, ctx->stream, const_cast<uint32_t*>(w));
+  return Fail(AM_ERR_HIP, "radix sort: a look-back wait on the device gave up");
+}
+
+int CheckSortError(am_ctx* ctx) {
+  uint32_t word = 0;
+  if (const uint32_t* w = SortErrorWord(ctx))
+    AM_HIP(hipMemcpyAsync(&word, w, 4, hipMemcpyDeviceToHost, ctx->stream));
+  AM_HIP(hipStreamSynchronize(ctx->stream));
+  return CheckSortErrorAfterSync(ctx, word);
 }
 static int64_t OsWords(int64_t n, int passes) {
   return OsMetaWords(passes) + (int64_t)passes * ((n + kOsTile - 1) / kOsTile) * 256;
@@ -600,7 +612,7 @@ int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
   uint32_t err = 0;
   if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
-  return CheckSortErrorAfterSync(err);
+  return CheckSortErrorAfterSync(ctx, err);
 }
 
 int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits) {
@@ -615,7 +627,7 @@ int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, i
   uint32_t err = 0;
   if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&err, w, 4, hipMemcpyDeviceToHost, ctx->stream));
   AM_HIP(hipStreamSynchronize(ctx->stream));
-  return CheckSortErrorAfterSync(err);
+  return CheckSortErrorAfterSync(ctx, err);
 }
 
 }  // extern "C"

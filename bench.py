@@ -118,21 +118,6 @@ def tree_cloud(n=N_TREE, seed=42, temperature=4.0):
     return pos, vel, sp, am.make_trap(1.0, 10.0, 0.01, 0.01), am.derive_dt(pops, 1.0)
 
 
-def timed_steps(ctx, step, steps, warmup):
-    for k in range(warmup):
-        step(k + 1)
-    ctx.synchronize()
-    ctx.reset_timing()
-    ctx.enable_timing(True)
-    t0 = time.perf_counter()
-    for k in range(steps):
-        step(warmup + k + 1)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.enable_timing(False)
-    return elapsed
-
-
 def force_errors(am, store, f):
     """Relative force errors of f against the FP64 direct sum on the device (the all-pairs FP64
     kernel, itself bit-identical to the oracle on the parity tests)."""
@@ -188,13 +173,28 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
             stats.update(mlp.barnes_hut_learned_forces(store, theta, net, rec_cap=1024, ybound=1e30, amax=1e30))
         kernel, label = "bh_learned_correct", f"Barnes-Hut theta={theta} + learned correction (MLP 16-64-64-3)"
 
+    active = {"now": int(store.count()[1]), "sum": 0}
+
     def step(k):
+        active["sum"] += active["now"]  # the particles this step advances
         forces()
         store.push(trap, dt, 0.0, 0.0, 42, k)
-        store.annihilation(5e-6, 0.5, 42, k)
+        active["now"] -= store.annihilation(5e-6, 0.5, 42, k)
 
     log(f"{kind}: timing {steps} steps")
-    elapsed = timed_steps(ctx, step, steps, warmup)
+    for k in range(warmup):
+        step(k + 1)
+    ctx.synchronize()
+    active_before = active["sum"]
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k + 1)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.enable_timing(False)
+    particle_steps = active["sum"] - active_before
     avg_ms, _ = kernel_avg(ctx, kernel)
     parts = {name: kernel_avg(ctx, name)[0] for name in
              ("bh_walk_f32", "bh_learned_correct", "fmm_upward", "fmm_traverse", "fmm_m2l", "fmm_m2l_mfma", "fmm_downward",
@@ -205,10 +205,11 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
     err, med, mx = force_errors(am, store, f)
     out = {
         "metric": f"particle-steps/s, {label}, force error vs FP64 direct",
-        "value": n * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "value": particle_steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (reference initial-condition generators, seed 42)",
         "config": {"workload": f"{label}; push + annihilation per step; {TREE_CONFIG}", "n_particles": n},
+        "active_particle_steps": particle_steps, "active_at_end": active["now"],
         "force_rel_err_l2": err, "force_rel_err_median": med, "force_rel_err_max": mx,
         "kernel_ms": parts,
     }
@@ -431,6 +432,8 @@ def main():
         "force_rel_err_l2": err,
         "force_rel_err_max": float(per.max()),
         "force_rel_err_median": float(np.median(per)),
+        "force_err_gate": {"tolerance": 1e-5, "measure": "global L2 relative error vs the FP64 direct sum",
+                           "met": bool(err <= 1e-5), "per_particle_max_within": bool(per.max() <= 1e-5)},
         "energy_drift_over_run": drift,
         "pair_interactions_per_s": n * (n - 1) * args.steps / elapsed,
         "kernel_ms": {"allpairs_f32_avg": avg_ms, "push_avg": ms_push / max(launches, 1)},

@@ -101,7 +101,7 @@ int am_species_properties(int species, double* mass, double* charge);
 int am_set_particle_positions(const uint64_t counts[6], const double radii[3], int seed, double* out);
 /* PenningTrapField::SetParticleVelocities (src/fields/penning_trap.cc:51-115). */
 int am_set_particle_velocities(const uint64_t counts[6], const double temperatures[6], int seed, double* out);
-/* PhysicsParams dt rule (src/params/physics_params.cc:115-129): listed[s] != 0 for species named
+/* PhysicsParams dt rule (src/params/physics_params.cc:60-74): listed[s] != 0 for species named
  * in particle_populations. */
 double am_derive_dt(const int32_t listed[6], double b0);
 
@@ -135,7 +135,10 @@ int am_store_count(am_store* store, int64_t* n, int64_t* n_active);
 
 /* ---- Coulomb solvers: write the store's Coulomb force for every alive particle ---------------- */
 /* ComputeStandardCoulombForces (src/interaction/coulomb_interaction.h:10, .cc:23-80): pairs with
- * r^2 < r_c^2, cell list of cell size r_c.  cutoff_radius <= 0 sets all forces to zero. */
+ * r^2 < r_c^2, cell list of cell size r_c.  cutoff_radius <= 0 sets all forces to zero.  Targets
+ * with a non-finite coordinate get zero force (coulomb_interaction.cc:55-56).  precision: AM_FP64
+ * is the reference arithmetic; AM_FP32 sums pairs in FP32 on hi/lo coordinates (FP64 per stencil
+ * cell) with the same pair set: r^2 within 1e-5 relative of r_c^2 is decided in FP64. */
 int am_compute_standard_coulomb_forces(am_store* store, double cutoff_radius, int precision);
 /* Exact all-pairs force without cutoff: the theta = 0 limit of ComputeBarnesHutCoulombForces
  * (src/interaction/barnes-hut.h:22-24) with the AddPointCharge pair term (barnes-hut.cc:19-27). */
@@ -171,11 +174,13 @@ int am_bh_tree_download(am_store* store, uint32_t* order, int32_t* begin, int32_
                         double* qn);
 /* ComputeFastMultipoleCoulombForces (src/interaction/fast-multipole.h:73, .cc:684-713). */
 int am_compute_fast_multipole_coulomb_forces(am_store* store, int order, double theta);
-/* ComputeFastMultipoleCoulombForces with a choice of near-field (P2P) arithmetic: AM_FP64 is the
- * reference arithmetic (= am_compute_fast_multipole_coulomb_forces); AM_FP32 evaluates P2P on hi/lo
- * coordinates with FP64 per-cell accumulation (expansions stay FP64). */
+/* ComputeFastMultipoleCoulombForces with a choice of arithmetic: AM_FP64 is the reference arithmetic
+ * (= am_compute_fast_multipole_coulomb_forces).  AM_FP32 evaluates P2P on hi/lo FP32 coordinates
+ * with FP64 accumulation per work item and, for orders <= 5, the M2L contraction on FP32 matrix
+ * cores (v_mfma_f32_16x16x4f32) with FP32 multipoles and FP32 derivative tensors; the tree, P2M,
+ * M2M, L2L and L2P stay FP64. */
 int am_compute_fast_multipole_coulomb_forces_ex(am_store* store, int order, double theta, int precision);
-/* Statistics of the most recent FMM call (the sizes FastMultipoleSolver::Compute builds,
+/* Statistics of the most recent FMM call on this store (the sizes FastMultipoleSolver::Compute builds,
  * fast-multipole.cc:397-526): sources, cells, M2L cell pairs, P2P cell pairs, deepest level, and the
  * number of distinct M2L displacement classes (any pointer may be NULL). */
 int am_fmm_stats(am_store* store, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs,

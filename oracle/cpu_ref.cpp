@@ -705,7 +705,7 @@ int orc_max_threads(void) {
 // ---- species / config -----------------------------------------------------------------------
 void orc_species_props(int species, double* mass, double* charge) { SpeciesProps(species, mass, charge); }
 
-// dt rule (src/params/physics_params.cc:115-129): listed[s] != 0 marks species present in
+// dt rule (src/params/physics_params.cc:60-74): listed[s] != 0 marks species present in
 // "particle_populations" (even with count 0).
 double orc_derive_dt(const int32_t listed[6], double b0) {
   double ratio = 1e-6;

@@ -1,0 +1,180 @@
+"""Boundary behaviour of the C ABI on the GPU: the FP32 cutoff solver, non-finite positions in
+every solver, the ensemble's guard against a re-uploaded store, and the Barnes-Hut far field on
+the hand-built cloud whose accept/open decisions are fixed by construction.
+
+Tolerances: FP64 paths bit for bit where the oracle's summation order is kept (all-pairs, BH), the
+FMM to 1e-9; FP32 paths to the 1e-5 global relative error of the accuracy gate."""
+import numpy as np
+import pytest
+
+import antimatter_amd as am
+import oracle
+from test_oracle_bh_far_field import expectations, scenario
+
+pytestmark = pytest.mark.gpu
+
+E = oracle.E_CHARGE
+K = oracle.K_COULOMB
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def store_of(ctx, pos, sp, ann=None, vel=None):
+    s = am.Store(ctx, max(1, len(sp)))
+    s.upload(pos, np.zeros_like(pos) if vel is None else vel, sp, annihilated=ann)
+    return s
+
+
+# ---- FP32 cutoff solver (ComputeStandardCoulombForces) -------------------------------------------
+def test_standard_fp32_matches_oracle(ctx):
+    rng = np.random.default_rng(41)
+    n = 20000
+    pos = rng.uniform(-1, 1, (n, 3)) * 1e-4
+    sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    sp[::53] = 2  # neutral
+    ann = (rng.random(n) < 0.03).astype(np.uint8)
+    m, q = oracle.mass_charge(sp)
+    s = store_of(ctx, pos, sp, ann)
+    live = ann == 0
+    for rc in (6e-6, 2e-5):
+        print(f"standard FP32: n={n} rc={rc}", flush=True)
+        s.compute_standard_coulomb_forces(rc, am.FP32)
+        f = s.forces()
+        ref = oracle.standard_forces(pos, q, ann, rc)
+        assert rel_l2(f[live], ref[live]) < 1e-5
+        per = np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+        assert np.median(per[live & (np.linalg.norm(ref, axis=1) > 0)]) < 1e-6
+    s.compute_standard_coulomb_forces(0.0, am.FP32)
+    assert np.all(s.forces()[live] == 0)
+    s.close()
+
+
+def test_standard_fp32_pair_set_at_the_cutoff(ctx):
+    """Pairs at r_c (1 - 1e-9), r_c and r_c (1 + 1e-9), far apart from one another: the strict test
+    r^2 < r_c^2 keeps only the first, as the FP64 path does, although FP32 r^2 cannot tell them
+    apart (the band decides them in FP64)."""
+    rc = 1e-5
+    seps = rc * np.array([1 - 1e-9, 1.0, 1 + 1e-9, 1 - 3e-8, 1 + 3e-8, 0.5])
+    pos = []
+    for k, d in enumerate(seps):
+        base = np.array([k * 1e-3, 0.0, 0.0])
+        pos += [base, base + np.array([d, 0.0, 0.0])]
+    pos = np.array(pos)
+    sp = np.array([1, 3] * len(seps), np.uint8)
+    m, q = oracle.mass_charge(sp)
+    s = store_of(ctx, pos, sp)
+    s.compute_standard_coulomb_forces(rc, am.FP64)
+    f64 = s.forces()
+    s.compute_standard_coulomb_forces(rc, am.FP32)
+    f32 = s.forces()
+    ref = oracle.standard_forces(pos, q, None, rc)
+    np.testing.assert_array_equal(f64, ref)
+    on = np.linalg.norm(ref, axis=1) > 0
+    assert list(on.reshape(-1, 2)[:, 0]) == [d * d < rc * rc for d in seps]
+    np.testing.assert_array_equal(f32[~on], 0.0)
+    np.testing.assert_allclose(f32[on], ref[on], rtol=2e-6)
+    s.close()
+
+
+# ---- non-finite positions ------------------------------------------------------------------------
+SOLVERS = [
+    ("allpairs", am.FP64), ("allpairs", am.FP32), ("allpairs-sorted", am.FP32), ("standard", am.FP64),
+    ("standard", am.FP32), ("barnes-hut", am.FP64), ("barnes-hut", am.FP32), ("fmm", am.FP64), ("fmm", am.FP32),
+]
+
+
+@pytest.mark.parametrize("solver,precision", SOLVERS, ids=[f"{a}-{'fp64' if b == am.FP64 else 'fp32'}"
+                                                            for a, b in SOLVERS])
+def test_non_finite_positions_are_skipped(ctx, solver, precision):
+    """Particles with a NaN or infinite coordinate get zero force and act on no one; every other
+    force equals the solver's result on the cloud without them (coulomb_interaction.cc:55-56)."""
+    rng = np.random.default_rng(7)
+    n = 5000 if solver == "allpairs-sorted" else 1500
+    pos = rng.uniform(-1, 1, (n, 3)) * 2e-5
+    sp = rng.choice([1, 3], n).astype(np.uint8)
+    bad = np.zeros(n, bool)
+    bad[[3, 400, 901]] = True
+    posb = pos.copy()
+    posb[3, 0] = np.nan
+    posb[400, 2] = np.inf
+    posb[901] = [-np.inf, np.nan, 0.0]
+    m, q = oracle.mass_charge(sp)
+    print(f"non-finite: {solver} precision={precision} n={n}", flush=True)
+    s = store_of(ctx, posb, sp)
+    s.set_coulomb_force(np.full((n, 3), 7.0))  # stale values must not survive
+    rc = 1e-5
+
+    def run():
+        if solver.startswith("allpairs"):
+            s.compute_allpairs_coulomb_forces(precision)
+        elif solver == "standard":
+            s.compute_standard_coulomb_forces(rc, precision)
+        elif solver == "barnes-hut":
+            s.compute_barnes_hut_coulomb_forces(0.5, precision)
+        else:
+            s.compute_fast_multipole_coulomb_forces(4, 0.5, precision)
+        return s.forces()
+
+    f = run()
+    assert np.all(np.isfinite(f))
+    np.testing.assert_array_equal(f[bad], 0.0)
+    # the same solver on the cloud with the bad particles annihilated instead
+    ref_store = store_of(ctx, pos, sp, ann=bad.astype(np.uint8))
+    s.close()
+    s = ref_store
+    g = run()
+    ok = ~bad
+    if precision == am.FP64 and solver != "fmm":
+        np.testing.assert_array_equal(f[ok], g[ok])
+    else:
+        assert rel_l2(f[ok], g[ok]) < (1e-9 if precision == am.FP64 else 1e-5)
+    s.close()
+
+
+# ---- ensemble over a re-uploaded store ------------------------------------------------------------
+def test_ensemble_rejects_a_reuploaded_store(ctx):
+    rng = np.random.default_rng(2)
+    n = 600
+    pos = rng.uniform(-1, 1, (n, 3)) * 1e-4
+    sp = rng.choice([1, 3], n).astype(np.uint8)
+    s = am.Store(ctx, 2 * n)
+    s.upload(pos, np.zeros_like(pos), sp)
+    f = am.make_trap(1.0, 10.0, 0.01, 0.01)
+    traps = [am.TrapDesc(0, n, f.b0, f.trap_voltage, f.d, 1e-12, 0.0, 0.0, 0, 0.0, 0.0)]
+    ens = am.Ensemble(s, traps)
+    ens.run(1, seed=1, use_graph=False)
+    pos2 = np.concatenate([pos, pos + 3e-4])
+    s.upload(pos2, np.zeros_like(pos2), np.concatenate([sp, sp]))  # more particles, same capacity
+    for call in (lambda: ens.run(1, seed=1, use_graph=False), lambda: ens.run(1, seed=1, use_graph=True),
+                 lambda: ens.set_chunk(1024), lambda: ens.autotune([1024])):
+        with pytest.raises(am.AmError) as ei:
+            call()
+        assert ei.value.code == -1 and "re-uploaded" in str(ei.value)
+    s.upload(pos, np.zeros_like(pos), sp)  # same n again: still a different particle set
+    with pytest.raises(am.AmError):
+        ens.run(1, seed=1)
+    ens.close()
+    s.close()
+
+
+# ---- Barnes-Hut far field on the hand-built cloud ---------------------------------------------------
+def test_bh_far_field_matches_hand_sums(ctx):
+    pos, q = scenario()
+    ex = expectations()
+    sp = np.where(q > 0, 1, np.where(q < 0, 3, 2)).astype(np.uint8)
+    s = store_of(ctx, pos, sp)
+    for theta, want in ((0.45, ex["f_mono"]), (0.35, ex["f_exact"])):
+        s.compute_barnes_hut_coulomb_forces(theta, am.FP64)
+        f = s.forces()
+        assert rel_l2(f[0], want) < 1e-12
+        np.testing.assert_array_equal(f, oracle.bh_forces(pos, q, None, theta))
+        s.compute_barnes_hut_coulomb_forces(theta, am.FP32)
+        assert rel_l2(s.forces()[0], want) < 1e-5
+    trap = am.make_trap(1.0, 10.0, 0.01, 0.01)
+    for theta in (0.45, 0.35):
+        e = s.compute_total_energy(trap, "barnes-hut", barnes_hut_theta=theta)
+        phi = oracle.bh_potentials(pos, q, None, theta)
+        assert e.energy_potential == pytest.approx(0.5 * K * float(q @ phi), rel=1e-12)
+    s.close()
