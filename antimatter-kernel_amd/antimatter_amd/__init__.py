@@ -103,6 +103,7 @@ SIGNATURES = {
     "am_push": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
     "am_annihilation": (_int, [_vp, _d, _d, _u64, _u64, _vp]),
     "am_compute_total_energy": (_int, [_vp, _vp, _vp, _vp, _vp]),
+    "am_compute_total_energy_ex": (_int, [_vp, _vp, _vp, _vp, _int, _vp]),
     "am_ensemble_create": (_int, [_vp, _i32, _vp, _i64, _vp]),
     "am_ensemble_destroy": (_int, [_vp]),
     "am_ensemble_set_chunk": (_int, [_vp, _i32]),
@@ -370,12 +371,14 @@ class Store:
         return k.value
 
     def compute_total_energy(self, field: TrapField, method="standard", cutoff_radius=0.0, barnes_hut_theta=0.5,
-                             fast_multipole_theta=0.5, fast_multipole_order=4, history: EnergyHistory = None):
+                             fast_multipole_theta=0.5, fast_multipole_order=4, history: EnergyHistory = None,
+                             precision: int = FP64):
         st = EnergySettings(METHODS[method] if isinstance(method, str) else int(method), cutoff_radius,
                             barnes_hut_theta, fast_multipole_theta, float(fast_multipole_order))
         out = EnergyBreakdown()
-        _check(lib().am_compute_total_energy(self.h, ctypes.byref(field), ctypes.byref(st),
-                                             None if history is None else ctypes.byref(history), ctypes.byref(out)))
+        _check(lib().am_compute_total_energy_ex(self.h, ctypes.byref(field), ctypes.byref(st),
+                                                None if history is None else ctypes.byref(history), int(precision),
+                                                ctypes.byref(out)))
         return out
 
 

@@ -154,7 +154,9 @@ __global__ __launch_bounds__(kAllPairsThreads) void AllPairsPotentialF32Kernel(c
       const float dy = (th.y - sh.y) + (tl.y - sl.y);
       const float dz = (th.z - sh.z) + (tl.z - sl.z);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
-      const float w = (jt + k == i) ? 0.f : sh.w;  // the self term q_i / eps would swamp FP32
+      // pairs under the 1e-15 m floor, the target itself among them, are skipped (q_i / eps would
+      // swamp the FP32 sum)
+      const float w = r2 < 2.f * kEps2Units ? 0.f : sh.w;
       p = fmaf(w, __builtin_amdgcn_rsqf(r2), p);
     }
     acc += (double)p;
@@ -265,12 +267,106 @@ __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const d
   fz[i] = scale * sz;
 }
 
+// ---- sorted FP32 potential (the energy diagnostic) ----------------------------------------------
+// phi of source tile R at one target, sum_j q_j / r in e/unit.  Pairs closer than the reference's
+// 1e-15 m floor (energy_analysis.cc via barnes-hut.cc:244-247), the target itself among them, are
+// skipped; they can only sit in near tiles, where the test costs a compare and a select per pair.
+__device__ __forceinline__ float TilePotential(const TileRec& R, int tile_first, const BlockBox& bb, const float4 th,
+                                               const float4 tl, float lx, float ly, float lz,
+                                               const float4* __restrict__ shl, const float4* __restrict__ sloc) {
+  const double gx = fmax(0.0, fabs(R.cx - bb.bc[0]) - R.hx - bb.bh[0]);
+  const double gy = fmax(0.0, fabs(R.cy - bb.bc[1]) - R.hy - bb.bh[1]);
+  const double gz = fmax(0.0, fabs(R.cz - bb.bc[2]) - R.hz - bb.bh[2]);
+  const double size = fmax(fmax(fmax(R.hx, R.hy), R.hz), bb.bhmax);
+  const double lim = 2.0 * kNearGap * size;
+  const int near = __builtin_amdgcn_readfirstlane((gx * gx + gy * gy + gz * gz) < lim * lim ? 1 : 0);
+  const int nq = __builtin_amdgcn_readfirstlane(R.npos + R.nneg);  // charged sources come first
+  const int ge = (nq + 7) & ~7;
+  float p = 0.f;
+  if (near) {
+    const float4* sp = shl + 2 * tile_first;
+    for (int j0 = 0; j0 < ge; j0 += 8) {
+      const SrcHL8 g = LoadSrcHL8(sp, j0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const float4 sh = g.hi(u), sl = g.lo(u);
+        const float dx = (th.x - sh.x) + (tl.x - sl.x);
+        const float dy = (th.y - sh.y) + (tl.y - sl.y);
+        const float dz = (th.z - sh.z) + (tl.z - sl.z);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+        const float w = r2 < 2.f * kEps2Units ? 0.f : sh.w;
+        p = fmaf(w, __builtin_amdgcn_rsqf(r2), p);
+      }
+    }
+  } else {
+    const float tx = lx + (float)(bb.bc[0] - R.cx);
+    const float ty = ly + (float)(bb.bc[1] - R.cy);
+    const float tz = lz + (float)(bb.bc[2] - R.cz);
+    const float4* sp = sloc + tile_first;
+    for (int j0 = 0; j0 < ge; j0 += 8) {
+      const Src8 g = LoadSrc8(sp, j0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const float4 sv = g[u];
+        const float dx = tx - sv.x, dy = ty - sv.y, dz = tz - sv.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+        p = fmaf(sv.w, __builtin_amdgcn_rsqf(r2), p);
+      }
+    }
+  }
+  return p;
+}
+
+// part[chunk][k] = phi at sorted position k over the chunk's tiles (e/unit), FP32 per tile, FP64
+// across tiles.
+__global__ __launch_bounds__(kAllPairsThreads) void AllPairsPotentialSortedKernel(
+    int n, int n_pad, int tiles_per_chunk, const uint32_t* __restrict__ perm, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ shl,
+    const float4* __restrict__ sloc, const TileRec* __restrict__ tiles, double* __restrict__ part) {
+  const int k = blockIdx.x * kAllPairsThreads + threadIdx.x;
+  const BlockBox bb = MakeBlockBox(tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1],
+                                    2 * blockIdx.x * kTile + kTile >= n);
+  const float4 th = shl[2 * k], tl = shl[2 * k + 1];
+  float lx = 0.f, ly = 0.f, lz = 0.f;
+  if (k < n) {
+    const uint32_t i = perm[k];
+    lx = (float)(x[i] * kUnitScale - bb.bc[0]);
+    ly = (float)(y[i] * kUnitScale - bb.bc[1]);
+    lz = (float)(z[i] * kUnitScale - bb.bc[2]);
+  }
+  const int n_tiles = n_pad / kTile;
+  const int t0 = blockIdx.y * tiles_per_chunk;
+  const int t1 = min(n_tiles, t0 + tiles_per_chunk);
+  double acc = 0.0;
+  for (int T = t0; T < t1; ++T) acc += (double)TilePotential(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc);
+  part[(size_t)blockIdx.y * n_pad + k] = acc;
+}
+
+// out[i] = (q_i / e) phi_i in e^2/unit, zero for annihilated, neutral or non-finite particles.
+__global__ void AllPairsPotentialSortedReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
+                                                    const uint32_t* __restrict__ perm,
+                                                    const float4* __restrict__ shl, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double sum = 0.0;
+  for (int c = 0; c < chunks; ++c) sum += part[(size_t)c * n_pad + k];
+  out[perm[k]] = (double)shl[2 * k].w * sum;  // the mirror's charge is 0 for every skipped particle
+}
+
 int GatheredBBox(am_ctx* ctx, const uint32_t* idx, int m, const double* x, const double* y, const double* z,
                  double* box_dev, const char* tag);
 
 static int ChunkCount(int num_cus, int target_blocks, int n_pad);
 
-int AllPairsForcesF32Sorted(am_store* s) {
+// The store's Morton-sorted tiles, prepared for the current positions.
+struct SortedTiles {
+  const float4* shl = nullptr;
+  const float4* sloc = nullptr;
+  const TileRec* tiles = nullptr;
+  int n_tiles = 0;
+};
+
+static int PrepareSortedTiles(am_store* s, SortedTiles* out) {
   am_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int n = (int)s->n;
@@ -294,7 +390,7 @@ int AllPairsForcesF32Sorted(am_store* s) {
     s->perm_n = n;
     s->perm_age = 0;
   }
-  void *shl, *sloc, *tiles, *part;
+  void *shl, *sloc, *tiles;
   const int n_tiles = n_pad / kTile;
   if ((rc = Scratch(ctx, "ap_shl", (size_t)n_pad * 2 * sizeof(float4), &shl))) return rc;
   if ((rc = Scratch(ctx, "ap_sloc", (size_t)n_pad * sizeof(float4), &sloc))) return rc;
@@ -302,6 +398,23 @@ int AllPairsForcesF32Sorted(am_store* s) {
   hipLaunchKernelGGL(TilePrepKernel, dim3(n_tiles), dim3(kTile), 0, st, n, s->perm, s->x, s->y,
                      s->z, (const float4*)s->src, (float4*)shl, (float4*)sloc, (TileRec*)tiles);
   AM_HIP(hipGetLastError());
+  out->shl = (const float4*)shl;
+  out->sloc = (const float4*)sloc;
+  out->tiles = (const TileRec*)tiles;
+  out->n_tiles = n_tiles;
+  return AM_OK;
+}
+
+int AllPairsForcesF32Sorted(am_store* s) {
+  am_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = (int)s->n;
+  const int n_pad = PadN(n);
+  int rc;
+  SortedTiles T;
+  if ((rc = PrepareSortedTiles(s, &T))) return rc;
+  const int n_tiles = T.n_tiles;
+  void* part;
   const int tblocks = n_pad / kAllPairsThreads;
   const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
   const int tpc = (n_tiles + chunks - 1) / chunks;
@@ -310,8 +423,7 @@ int AllPairsForcesF32Sorted(am_store* s) {
   int slot;
   TimerBegin(ctx, "allpairs_f32", &slot);
   hipLaunchKernelGGL(AllPairsSortedKernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, st, n, n_pad, tpc,
-                     (const uint32_t*)s->perm, s->x, s->y, s->z, (const float4*)shl, (const float4*)sloc,
-                     (const TileRec*)tiles, (double*)part);
+                     (const uint32_t*)s->perm, s->x, s->y, s->z, T.shl, T.sloc, T.tiles, (double*)part);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   hipLaunchKernelGGL(AllPairsSortedReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, n_pad, nch,
@@ -391,6 +503,27 @@ int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_s
   const int n_pad = PadN(n);
   const int tblocks = n_pad / kAllPairsThreads;
   const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
+  *si_scale = kElementaryCharge * kElementaryCharge * kUnitScale;  // e^2/unit -> C^2/m
+  if (n >= kSortedMinN) {
+    SortedTiles T;
+    int rc = PrepareSortedTiles(s, &T);
+    if (rc) return rc;
+    const int tpc = (T.n_tiles + chunks - 1) / chunks;
+    const int nch = (T.n_tiles + tpc - 1) / tpc;
+    void* part = nullptr;
+    if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
+    int slot;
+    TimerBegin(ctx, "allpairs_potential_f32", &slot);
+    hipLaunchKernelGGL(AllPairsPotentialSortedKernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, ctx->stream, n,
+                       n_pad, tpc, (const uint32_t*)s->perm, s->x, s->y, s->z, T.shl, T.sloc, T.tiles,
+                       (double*)part);
+    TimerEnd(ctx, slot);
+    AM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(AllPairsPotentialSortedReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n,
+                       n_pad, nch, (const double*)part, (const uint32_t*)s->perm, T.shl, out);
+    AM_HIP(hipGetLastError());
+    return AM_OK;
+  }
   int chunk = (n_pad / chunks + kTile - 1) / kTile * kTile;
   const int nch = (n_pad + chunk - 1) / chunk;
   void* part = nullptr;
@@ -402,7 +535,6 @@ int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_s
   hipLaunchKernelGGL(AllPairsPotentialReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n, n_pad, nch,
                      (const double*)part, s->src, out);
   AM_HIP(hipGetLastError());
-  *si_scale = kElementaryCharge * kElementaryCharge * kUnitScale;  // e^2/unit -> C^2/m
   return AM_OK;
 }
 

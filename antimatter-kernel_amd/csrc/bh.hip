@@ -289,7 +289,19 @@ __device__ __forceinline__ void PairF64Acc(double tx, double ty, double tz, doub
   az = __dadd_rn(az, __dmul_rn(rz, c));
 }
 
-// MODE 0: FP32 force; MODE 1: FP64 force; MODE 2: FP64 potential (q_i phi_i into out_x).  COUNT:
+// FP32 potential of one source on hi/lo coordinates (e/unit); pairs under the reference's 1e-15 m
+// floor (the target itself among them) are skipped, as barnes-hut.cc:244-247 skips them.
+__device__ __forceinline__ float PotF32(const float4 th, const float4 tl, const float4 sh, const float4 sl) {
+  const float dx = (th.x - sh.x) + (tl.x - sl.x);
+  const float dy = (th.y - sh.y) + (tl.y - sl.y);
+  const float dz = (th.z - sh.z) + (tl.z - sl.z);
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+  const float w = r2 < 2.f * kEps2Units ? 0.f : sh.w;
+  return w * __builtin_amdgcn_rsqf(r2);
+}
+
+// MODE 0: FP32 force; MODE 1: FP64 force; MODE 2: FP64 potential (q_i phi_i into out_x); MODE 3:
+// FP32 potential ((q_i / e) phi_i in e^2/unit into out_x, FP32 per leaf, FP64 across nodes).  COUNT:
 // interactions are counted into counts (a compile-time switch, so the plain walk carries no counting
 // arithmetic in its loops).
 template <int MODE, bool COUNT>
@@ -327,7 +339,12 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
       if (nd.qp == 0.0 && nd.qn == 0.0) {
         next = nd.skip;
       } else if (nd.leaf) {
-        if (MODE == 0) {
+        if (MODE == 3) {
+          float ph = 0.f;
+          for (int kk = nd.b; kk < nd.e; ++kk) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
+          ax += ph;
+          if (COUNT) for (int kk = nd.b; kk < nd.e; ++kk) cnt += (ssrc[2 * kk].w != 0.f && kk != ks) ? 1 : 0;
+        } else if (MODE == 0) {
           // The leaf's sources are contiguous in the tree-ordered copy of the FP32 mirror.  The target
           // itself and charge-0 sources add exactly 0 (softened pair, q = 0), so they are not
           // tested; the interaction count leaves them out.
@@ -362,7 +379,11 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         const double d2 = Dot3(dx, dy, dz, dx, dy, dz);
         const double w = __dmul_rn(2.0, nd.half);
         if (__dmul_rn(w, w) < __dmul_rn(theta2, d2)) {  // accepted (barnes-hut.cc:201-215)
-          if (MODE == 0) {
+          if (MODE == 3) {
+            const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
+            ax += PotF32(th, tl, f.p_hi, f.p_lo) + PotF32(th, tl, f.n_hi, f.n_lo);
+            if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
+          } else if (MODE == 0) {
             const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
             float fx = 0.f, fy = 0.f, fz = 0.f;
             PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz);
@@ -417,8 +438,10 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
     out_x[i] = ax;
     out_y[i] = ay;
     out_z[i] = az;
-  } else {
+  } else if (MODE == 2) {
     out_x[i] = __dmul_rn(qi, ax);
+  } else {
+    out_x[i] = (double)th.w * ax;
   }
   if (COUNT) counts[i] = cnt;
 }
@@ -527,12 +550,13 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   info.deep_overflow = t.deep_overflow;
   info.order = t.order;
   info.nodes = t.nodes;
-  if (mode != 2 && (rc = ZeroNonFiniteForces(s))) return rc;  // they are in no tree
+  if (mode < 2 && (rc = ZeroNonFiniteForces(s))) return rc;  // they are in no tree
   if (t.m == 0) return AM_OK;
   am_ctx* ctx = s->ctx;
   const double theta2 = theta * theta;
   int slot;
-  TimerBegin(ctx, mode == 0 ? "bh_walk_f32" : (mode == 1 ? "bh_walk_f64" : "bh_potential"), &slot);
+  TimerBegin(ctx, mode == 0 ? "bh_walk_f32" : (mode == 1 ? "bh_walk_f64" : (mode == 2 ? "bh_potential" : "bh_potential_f32")),
+             &slot);
   const dim3 grid(Blocks(t.m, 256)), block(256);
   auto launch = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z,
@@ -540,7 +564,8 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   };
   if (mode == 0) counts ? launch(BHWalkKernel<0, true>) : launch(BHWalkKernel<0, false>);
   else if (mode == 1) counts ? launch(BHWalkKernel<1, true>) : launch(BHWalkKernel<1, false>);
-  else counts ? launch(BHWalkKernel<2, true>) : launch(BHWalkKernel<2, false>);
+  else if (mode == 2) counts ? launch(BHWalkKernel<2, true>) : launch(BHWalkKernel<2, false>);
+  else counts ? launch(BHWalkKernel<3, true>) : launch(BHWalkKernel<3, false>);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;
@@ -566,10 +591,13 @@ __global__ void ZeroF64Kernel(int64_t n, double* __restrict__ a) {
   if (i < n) a[i] = 0.0;
 }
 
-int BarnesHutPotentialTerms(am_store* s, double theta, double* out) {
+// Per-particle potential terms of the Barnes-Hut energy into out[n]; *si_scale turns their sum into
+// C^2/m (1 for the FP64 walk, e^2 2^20 for the FP32 walk's e^2/unit).
+int BarnesHutPotentialTerms(am_store* s, double theta, int precision, double* out, double* si_scale) {
   hipLaunchKernelGGL(ZeroF64Kernel, dim3(Blocks(s->n, 256)), dim3(256), 0, s->ctx->stream, s->n, out);
   AM_HIP(hipGetLastError());
-  return BarnesHutWalk(s, theta, 2, out, nullptr, nullptr, nullptr);
+  *si_scale = precision == AM_FP32 ? kElementaryCharge * kElementaryCharge * kUnitScale : 1.0;
+  return BarnesHutWalk(s, theta, precision == AM_FP32 ? 3 : 2, out, nullptr, nullptr, nullptr);
 }
 
 }  // namespace am

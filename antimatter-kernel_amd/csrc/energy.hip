@@ -10,7 +10,7 @@ namespace am {
 
 int StandardEnergyTerms(am_store* s, double rc, double* out);
 int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_scale);
-int BarnesHutPotentialTerms(am_store* s, double theta, double* out);
+int BarnesHutPotentialTerms(am_store* s, double theta, int precision, double* out, double* si_scale);
 int FastMultipolePotentialEnergy(am_store* s, int order, double theta, double* pe);
 
 constexpr int kReduceThreads = 256;
@@ -90,9 +90,11 @@ __global__ void KineticTrapKernel(int64_t n, const double* __restrict__ x, const
 
 using namespace am;
 
-extern "C" int am_compute_total_energy(am_store* s, const am_trap_field* field, const am_energy_settings* settings,
-                                       am_energy_history* history, am_energy_breakdown* out) {
+extern "C" int am_compute_total_energy_ex(am_store* s, const am_trap_field* field,
+                                          const am_energy_settings* settings, am_energy_history* history,
+                                          int precision, am_energy_breakdown* out) {
   if (!s || !field || !settings || !out) return Fail(AM_ERR_INVALID, "am_compute_total_energy: NULL argument");
+  if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
   AM_HIP(hipSetDevice(s->ctx->device));
   am_ctx* ctx = s->ctx;
   const int64_t n = s->n;
@@ -122,15 +124,16 @@ extern "C" int am_compute_total_energy(am_store* s, const am_trap_field* field, 
       }
       case AM_METHOD_BARNES_HUT:
         if (settings->barnes_hut_theta > 0.0) {
-          if ((rc = BarnesHutPotentialTerms(s, settings->barnes_hut_theta, pc))) return rc;
+          double scale = 1.0;
+          if ((rc = BarnesHutPotentialTerms(s, settings->barnes_hut_theta, precision, pc, &scale))) return rc;
           if ((rc = ReduceSum(ctx, pc, n, &sum))) return rc;
-          r.energy_potential = 0.5 * kCoulomb * sum;
+          r.energy_potential = 0.5 * kCoulomb * scale * sum;
           break;
         }
         [[fallthrough]];  // theta = 0 opens every node: the exact all-pairs potential
       case AM_METHOD_ALLPAIRS: {
         double scale = 1.0;
-        if ((rc = AllPairsPotentialTerms(s, AM_FP64, pc, &scale))) return rc;
+        if ((rc = AllPairsPotentialTerms(s, precision, pc, &scale))) return rc;
         if ((rc = ReduceSum(ctx, pc, n, &sum))) return rc;
         r.energy_potential = 0.5 * kCoulomb * scale * sum;
         break;
@@ -155,4 +158,9 @@ extern "C" int am_compute_total_energy(am_store* s, const am_trap_field* field, 
   }
   *out = r;
   return AM_OK;
+}
+
+extern "C" int am_compute_total_energy(am_store* s, const am_trap_field* field, const am_energy_settings* settings,
+                                       am_energy_history* history, am_energy_breakdown* out) {
+  return am_compute_total_energy_ex(s, field, settings, history, AM_FP64, out);
 }

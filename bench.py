@@ -401,6 +401,19 @@ def main():
     step(args.warmup + args.steps + 1)
     e1 = store.compute_total_energy(trap, "all-pairs", history=hist_end)
     drift = (e1.total() - e0.total()) / (1.602176634e-19 * abs(trap.trap_voltage))
+    # The energy diagnostic (outside the timed region): wall time of one ComputeTotalEnergy call with
+    # the all-pairs potential in FP64 and in FP32 pair math, and the step cost at the reference's log
+    # cadence for 1000 steps (interval 10: two calls per 10 steps, antimatter-kernel.cc:108,164-169).
+    energy_ms = {}
+    for name, prec in (("fp64", am.FP64), ("fp32", am.FP32)):
+        times = []
+        for _ in range(3):
+            ctx.synchronize()
+            ta = time.perf_counter()
+            ek = store.compute_total_energy(trap, "all-pairs", precision=prec)
+            times.append(1e3 * (time.perf_counter() - ta))
+        energy_ms[name] = statistics.median(times)
+        energy_ms[f"{name}_coulomb_J"] = ek.energy_potential
 
     avg_ms = ms_force / max(launches, 1)
     flops = FLOP_PER_PAIR * n * (n - 1)
@@ -435,6 +448,8 @@ def main():
         "force_err_gate": {"tolerance": 1e-5, "measure": "global L2 relative error vs the FP64 direct sum",
                            "met": bool(err <= 1e-5), "per_particle_max_within": bool(per.max() <= 1e-5)},
         "energy_drift_over_run": drift,
+        "energy_call_ms": energy_ms,
+        "ms_per_step_with_energy_at_log_cadence": ms_per_step + 0.2 * energy_ms["fp32"],
         "pair_interactions_per_s": n * (n - 1) * args.steps / elapsed,
         "kernel_ms": {"allpairs_f32_avg": avg_ms, "push_avg": ms_push / max(launches, 1)},
         "roofline": {

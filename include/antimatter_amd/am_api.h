@@ -214,6 +214,13 @@ int am_is_antiparticle_pair(int species_a, int species_b);
  * the potentials with the previous call. */
 int am_compute_total_energy(am_store* store, const am_trap_field* field, const am_energy_settings* settings,
                             am_energy_history* history, am_energy_breakdown* out);
+/* ComputeTotalEnergy with a choice of pair arithmetic for the Coulomb potential: AM_FP64 is
+ * am_compute_total_energy; AM_FP32 evaluates the all-pairs potential on the sorted FP32 tiles and the
+ * Barnes-Hut potential with the FP32 walk (hi/lo coordinates, FP32 per tile or leaf, FP64 across them
+ * and in every reduction); pairs under the 1e-15 m floor are skipped in both.  The standard and FMM
+ * potentials are FP64 in both modes. */
+int am_compute_total_energy_ex(am_store* store, const am_trap_field* field, const am_energy_settings* settings,
+                               am_energy_history* history, int precision, am_energy_breakdown* out);
 
 /* ---- batched ensemble of independent traps (no single reference counterpart: each trap is one
  * reference Simulate loop, src/antimatter-kernel.cc:153-197, with its own field and settings) ---- */

@@ -133,6 +133,37 @@ def test_non_finite_positions_are_skipped(ctx, solver, precision):
     s.close()
 
 
+# ---- FP32 Coulomb energy ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [3000, 65536])
+def test_energy_fp32_potentials(ctx, n):
+    """The all-pairs (sorted tiles from 4,096 particles, unsorted below) and Barnes-Hut potentials
+    with FP32 pair math against the FP64 paths, which equal the oracle (test_gpu_core, test_gpu_bh).
+    The clouds are quasi-neutral (antiprotons + positrons), so U is a difference of large terms:
+    the tolerance is on U relative to the sum of |q_i phi_i| terms' scale, stated as 1e-6 of
+    U_like = k/2 sum_i |q_i| sum_j |q_j| / r_ij (computed in FP64 on the device)."""
+    pops = {"antiproton": n // 2, "positron": n - n // 2}
+    pos, vel, sp = am.initial_conditions(pops, {"antiproton": 10.0, "positron": 100.0}, (1e-3, 1e-3, 4e-3), 42)
+    ann = np.zeros(n, np.uint8)
+    ann[::101] = 1
+    s = store_of(ctx, pos, sp, ann, vel)
+    trap = am.make_trap(3.0, 50.0, 0.01, 0.01)
+    like = store_of(ctx, pos, np.full(n, 1, np.uint8), ann, vel)  # all positrons: every term adds
+    for method, kw in (("all-pairs", {}), ("barnes-hut", {"barnes_hut_theta": 0.6})):
+        e64 = s.compute_total_energy(trap, method, precision=am.FP64, **kw)
+        e32 = s.compute_total_energy(trap, method, precision=am.FP32, **kw)
+        scale = like.compute_total_energy(trap, method, precision=am.FP64, **kw).energy_potential
+        err = abs(e32.energy_potential - e64.energy_potential) / scale
+        print(f"energy FP32 {method} n={n}: U={e64.energy_potential:.6e} |dU|/U_like={err:.2e}", flush=True)
+        assert np.isfinite(e32.energy_potential) and err < 1e-6
+        assert e32.kinetic == e64.kinetic and e32.trap_potential == e64.trap_potential
+    if n <= 3000:
+        m, q = oracle.mass_charge(sp)
+        ref = oracle.allpairs_potential_energy(pos, q, ann)
+        assert s.compute_total_energy(trap, "all-pairs").energy_potential == pytest.approx(ref, rel=1e-12)
+    like.close()
+    s.close()
+
+
 # ---- ensemble over a re-uploaded store ------------------------------------------------------------
 def test_ensemble_rejects_a_reuploaded_store(ctx):
     rng = np.random.default_rng(2)
