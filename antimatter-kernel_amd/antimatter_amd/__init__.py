@@ -93,6 +93,7 @@ SIGNATURES = {
     "am_allpairs_forces_noalloc": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_barnes_hut_coulomb_forces": (_int, [_vp, _d, _int]),
     "am_barnes_hut_forces_counted": (_int, [_vp, _d, _int, _vp]),
+    "am_barnes_hut_evaluate": (_int, [_vp, _d, _int, _i64, _vp, _vp, _vp, _vp, _vp]),
     "am_bh_tree_stats": (_int, [_vp, _vp, _vp, _vp, _vp]),
     "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
@@ -325,6 +326,19 @@ class Store:
         c = np.zeros(n, dtype=np.int64)
         _check(lib().am_barnes_hut_forces_counted(self.h, theta, precision, _p(c)))
         return c
+
+    def barnes_hut_evaluate(self, theta: float, points, charges=None, self_index=None, precision: int = FP64,
+                            want_force: bool = True, want_potential: bool = True):
+        """BarnesHutTree::ComputeForce / ComputePotential at given points over the store's particles:
+        (forces [k, 3] in N or None, potentials [k] in C/m or None)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        k = len(pts)
+        q = None if charges is None else np.ascontiguousarray(charges, dtype=np.float64)
+        si = None if self_index is None else np.ascontiguousarray(self_index, dtype=np.int64)
+        f = np.zeros((k, 3)) if want_force else None
+        phi = np.zeros(k) if want_potential else None
+        _check(lib().am_barnes_hut_evaluate(self.h, theta, int(precision), k, _p(pts), _p(q), _p(si), _p(f), _p(phi)))
+        return f, phi
 
     def bh_tree(self):
         """Tree of the most recent Barnes-Hut build (reference DFS preorder)."""

@@ -304,7 +304,16 @@ __device__ __forceinline__ float PotF32(const float4 th, const float4 tl, const 
 // FP32 potential ((q_i / e) phi_i in e^2/unit into out_x, FP32 per leaf, FP64 across nodes).  COUNT:
 // interactions are counted into counts (a compile-time switch, so the plain walk carries no counting
 // arithmetic in its loops).
-template <int MODE, bool COUNT>
+// External evaluation points of am_barnes_hut_evaluate: position pos[3k..3k+2], charge q[k], and the
+// source (particle index) left out as the point's "self", or 0xffffffff for none
+// (BarnesHutTree::ComputeForce / ComputePotential, barnes-hut.h:28-45).
+struct ExtTargets {
+  const double* pos = nullptr;
+  const double* q = nullptr;
+  const uint32_t* self = nullptr;
+};
+
+template <int MODE, bool COUNT, bool EXT = false>
 __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes,
                                                      const WalkNodeF32* __restrict__ nodes32,
                                                      const uint32_t* __restrict__ order,
@@ -315,18 +324,43 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
                                                      double* __restrict__ out_x, double* __restrict__ out_y,
                                                      double* __restrict__ out_z, int64_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec,
-                                                     int rec_cap, uint32_t* __restrict__ rec_cnt) {
+                                                     int rec_cap, uint32_t* __restrict__ rec_cnt,
+                                                     ExtTargets ext = ExtTargets()) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < m;
   const int kc = valid ? k : m - 1;
-  const int ks = tsel ? (int)tsel[kc] : kc;  // the target's position in the tree order
-  const uint32_t i = order[ks];
+  uint32_t i;
+  int ks = -1;  // the target's position in the tree order (store targets)
+  double px, py, pz, qi;
+  float4 th, tl;
+  bool want;
+  if (EXT) {  // hi/lo coordinates of the point as the source mirror builds them
+    i = ext.self[kc];
+    px = ext.pos[3 * kc];
+    py = ext.pos[3 * kc + 1];
+    pz = ext.pos[3 * kc + 2];
+    qi = ext.q[kc];
+    float4 pair[2];
+    WriteMirror(pair, 0, px, py, pz, qi, false);
+    th = pair[0];
+    tl = pair[1];
+    // ComputeForce returns zero for a neutral point (barnes-hut.cc:168); ComputePotential does not
+    // need a charge.
+    want = valid && Finite3(px, py, pz) && (MODE >= 2 || qi != 0.0);
+  } else {
+    ks = tsel ? (int)tsel[kc] : kc;
+    i = order[ks];
+    px = x[i];
+    py = y[i];
+    pz = z[i];
+    qi = q[i];
+    th = src[2 * i];
+    tl = src[2 * i + 1];
+    // ComputeForce returns zero for a neutral target (barnes-hut.cc:168); the energy sum skips
+    // neutral particles (energy_analysis.cc:64).
+    want = valid && qi != 0.0;
+  }
   uint32_t nrec = 0;
-  const double px = x[i], py = y[i], pz = z[i], qi = q[i];
-  const float4 th = src[2 * i], tl = src[2 * i + 1];
-  // ComputeForce returns zero for a neutral target (barnes-hut.cc:168); the energy sum skips
-  // neutral particles (energy_analysis.cc:64).
-  const bool want = valid && qi != 0.0;
   int next = want ? 0 : nn;
   double ax = 0.0, ay = 0.0, az = 0.0;
   int64_t cnt = 0;
@@ -341,7 +375,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
       } else if (nd.leaf) {
         if (MODE == 3) {
           float ph = 0.f;
-          for (int kk = nd.b; kk < nd.e; ++kk) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
+          for (int kk = nd.b; kk < nd.e; ++kk)
+            if (!EXT || order[kk] != i) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
           ax += ph;
           if (COUNT) for (int kk = nd.b; kk < nd.e; ++kk) cnt += (ssrc[2 * kk].w != 0.f && kk != ks) ? 1 : 0;
         } else if (MODE == 0) {
@@ -351,6 +386,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
           float fx = 0.f, fy = 0.f, fz = 0.f;
           for (int kk = nd.b; kk < nd.e; ++kk) {
             const float4 sh = ssrc[2 * kk];
+            if (EXT && order[kk] == i) continue;  // the point's self source, at any position
             PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
             if (COUNT) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
           }
@@ -428,6 +464,21 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
     c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : nd.skip);
   }
   if (!valid) return;
+  if (EXT) {  // per point: the force on charge q, or phi (C/m for MODE 2, e/unit for MODE 3)
+    if (MODE == 0) {
+      const double scale = want ? kCoulomb * qi * kElementaryCharge * kInvUnit2 : 0.0;
+      out_x[k] = scale * ax;
+      out_y[k] = scale * ay;
+      out_z[k] = scale * az;
+    } else if (MODE == 1) {
+      out_x[k] = ax;
+      out_y[k] = ay;
+      out_z[k] = az;
+    } else {
+      out_x[k] = ax;
+    }
+    return;
+  }
   if (rec_cnt) rec_cnt[k] = nrec;
   if (MODE == 0) {
     const double scale = qi != 0.0 ? kCoulomb * qi * kElementaryCharge * kInvUnit2 : 0.0;
@@ -560,7 +611,8 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   const dim3 grid(Blocks(t.m, 256)), block(256);
   auto launch = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z,
-                       s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr, nullptr, 0, nullptr);
+                       s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr, nullptr, 0, nullptr,
+                       ExtTargets());
   };
   if (mode == 0) counts ? launch(BHWalkKernel<0, true>) : launch(BHWalkKernel<0, false>);
   else if (mode == 1) counts ? launch(BHWalkKernel<1, true>) : launch(BHWalkKernel<1, false>);
@@ -580,7 +632,7 @@ int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* 
   hipLaunchKernelGGL((BHWalkKernel<0, false>), dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
                      t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
                      ox, oy, oz,
-                     (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt);
+                     (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt, ExtTargets());
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;
@@ -614,6 +666,79 @@ int am_compute_barnes_hut_coulomb_forces(am_store* s, double theta, int precisio
   if (s->n > (int64_t)1 << 30) return Fail(AM_ERR_CAPACITY, "Barnes-Hut: n too large");
   AM_HIP(hipSetDevice(s->ctx->device));
   return BarnesHutWalk(s, theta, precision == AM_FP32 ? 0 : 1, s->fx, s->fy, s->fz, nullptr);
+}
+
+// BarnesHutTree::Build over the store's alive particles, then ComputeForce / ComputePotential at
+// n_points given points (barnes-hut.h:28-45).  Host arrays: pos 3n, charge n (NULL: no force),
+// self_index n (NULL or -1: no source left out); force 3n in N, potential n in C/m (either NULL).
+int am_barnes_hut_evaluate(am_store* s, double theta, int precision, int64_t n_points, const double* pos,
+                           const double* charge, const int64_t* self_index, double* force, double* potential) {
+  if (!s || (n_points > 0 && !pos)) return Fail(AM_ERR_INVALID, "am_barnes_hut_evaluate: NULL argument");
+  if (precision != AM_FP32 && precision != AM_FP64) return Fail(AM_ERR_INVALID, "unknown precision");
+  if (!(theta >= 0.0)) return Fail(AM_ERR_INVALID, "barnes_hut_theta must be >= 0");
+  if (n_points < 0 || n_points > (int64_t)1 << 30) return Fail(AM_ERR_CAPACITY, "am_barnes_hut_evaluate: n_points");
+  if (force)
+    for (int64_t k = 0; k < 3 * n_points; ++k) force[k] = 0.0;
+  if (potential)
+    for (int64_t k = 0; k < n_points; ++k) potential[k] = 0.0;
+  if (n_points == 0 || s->n == 0 || (!force && !potential)) return AM_OK;
+  std::vector<uint32_t> self(n_points, 0xffffffffu);
+  if (self_index)
+    for (int64_t k = 0; k < n_points; ++k) {
+      if (self_index[k] >= s->n) return Fail(AM_ERR_INVALID, "am_barnes_hut_evaluate: self_index beyond the store");
+      if (self_index[k] >= 0) self[k] = (uint32_t)self_index[k];
+    }
+  std::vector<double> q(n_points, 0.0);
+  if (charge) std::copy(charge, charge + n_points, q.begin());
+  am_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  AM_HIP(hipSetDevice(ctx->device));
+  BHTree t;
+  int rc = BuildBHTree(s, 8, &t);
+  if (rc) return rc;
+  if (t.m == 0) return AM_OK;
+  const int m = (int)n_points;
+  void* buf;
+  // points (3m) + charges (m) + outputs (3m) + self (m uint32)
+  if ((rc = Scratch(ctx, "bh_eval", (size_t)m * (7 * 8 + 4), &buf))) return rc;
+  double* dpos = (double*)buf;
+  double* dq = dpos + 3 * (size_t)m;
+  double* out = dq + m;
+  uint32_t* dself = (uint32_t*)(out + 3 * (size_t)m);
+  AM_HIP(hipMemcpyAsync(dpos, pos, (size_t)m * 24, hipMemcpyHostToDevice, st));
+  AM_HIP(hipMemcpyAsync(dq, q.data(), (size_t)m * 8, hipMemcpyHostToDevice, st));
+  AM_HIP(hipMemcpyAsync(dself, self.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+  ExtTargets ext;
+  ext.pos = dpos;
+  ext.q = dq;
+  ext.self = dself;
+  const double theta2 = theta * theta;
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, t.nn, t.nodes, t.nodes32, t.order, s->x,
+                       s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, out, out + m, out + 2 * (size_t)m,
+                       (int64_t*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0, (uint32_t*)nullptr, ext);
+  };
+  std::vector<double> h(3 * (size_t)m);
+  if (force) {
+    precision == AM_FP32 ? launch(BHWalkKernel<0, false, true>) : launch(BHWalkKernel<1, false, true>);
+    AM_HIP(hipGetLastError());
+    AM_HIP(hipMemcpyAsync(h.data(), out, (size_t)m * 24, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < m; ++k) {
+      force[3 * (size_t)k] = h[k];
+      force[3 * (size_t)k + 1] = h[m + k];
+      force[3 * (size_t)k + 2] = h[2 * (size_t)m + k];
+    }
+  }
+  if (potential) {
+    precision == AM_FP32 ? launch(BHWalkKernel<3, false, true>) : launch(BHWalkKernel<2, false, true>);
+    AM_HIP(hipGetLastError());
+    AM_HIP(hipMemcpyAsync(h.data(), out, (size_t)m * 8, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipStreamSynchronize(st));
+    const double unit = precision == AM_FP32 ? kElementaryCharge * kUnitScale : 1.0;  // e/unit -> C/m
+    for (int k = 0; k < m; ++k) potential[k] = unit * h[k];
+  }
+  return AM_OK;
 }
 
 // Barnes-Hut with per-particle interaction counts (leaf pairs + accepted monopoles), n int64.

@@ -159,6 +159,15 @@ int am_allpairs_forces_noalloc(int64_t n, const double* pos, const double* charg
 /* ComputeBarnesHutCoulombForces (src/interaction/barnes-hut.h:76, .cc:271-299): dual-monopole
  * octree, leaf capacity 8, MAC (2h)^2 < theta^2 |x - c|^2 against the geometric centre. */
 int am_compute_barnes_hut_coulomb_forces(am_store* store, double theta, int precision);
+/* The standalone BarnesHutTree (src/interaction/barnes-hut.h:28-74): Build over the store's alive
+ * particles (leaf capacity 8), then ComputeForce(position, charge, self_index) and
+ * ComputePotential(position, self_index) at n_points points given on the host: pos 3n doubles,
+ * charge n doubles (NULL: zero charges, so zero force), self_index n int64 (NULL or -1: none; else
+ * the store particle left out of the sum).  Outputs (host, either may be NULL): force 3n in N,
+ * potential n in C/m (the sum of q_j / r_j, without the Coulomb constant).  AM_FP32 walks with the
+ * FP32 pair math of am_compute_barnes_hut_coulomb_forces. */
+int am_barnes_hut_evaluate(am_store* store, double theta, int precision, int64_t n_points, const double* pos,
+                           const double* charge, const int64_t* self_index, double* force, double* potential);
 /* ComputeBarnesHutCoulombForces plus the per-particle interaction count of BarnesHutTree::Accumulate
  * (barnes-hut.cc:183-221): leaf pairs + accepted monopoles, n int64 into counts (host). */
 int am_barnes_hut_forces_counted(am_store* store, double theta, int precision, int64_t* counts);

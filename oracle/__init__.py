@@ -71,6 +71,7 @@ def lib():
             "orc_bh_forces": (None, [_i64, _vp, _vp, _vp, _d, _i64, _vp, _vp]),
             "orc_bh_potential_energy": (_d, [_i64, _vp, _vp, _vp, _d]),
             "orc_bh_potentials": (None, [_i64, _vp, _vp, _vp, _d, _i64, _vp]),
+            "orc_bh_evaluate": (None, [_i64, _vp, _vp, _vp, _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
             "orc_bh_tree": (_i64, [_i64, _vp, _vp, _i64, _i64] + [_vp] * 12),
             "orc_fmm_forces": (None, [_i64, _vp, _vp, _vp, ctypes.c_int, _d, _i64, _vp, _vp]),
             "orc_fmm_potential_energy": (_d, [_i64, _vp, _vp, _vp, ctypes.c_int, _d]),
@@ -267,6 +268,18 @@ def bh_potentials(pos, q, annihilated, theta, leaf_cap=8):
     lib().orc_bh_potentials(len(q), _p(pos), _p(_c(q, np.float64)), _p(_c(annihilated, np.uint8)), theta,
                             leaf_cap, _p(phi))
     return phi
+
+
+def bh_evaluate(pos, q, annihilated, theta, points, charges=None, self_index=None, leaf_cap=8):
+    """BarnesHutTree::ComputeForce / ComputePotential at given points: (forces [k, 3], phi [k] in C/m)."""
+    pos = _c(pos, np.float64)
+    pts = _c(points, np.float64).reshape(-1, 3)
+    k = len(pts)
+    f = np.zeros((k, 3))
+    phi = np.zeros(k)
+    lib().orc_bh_evaluate(len(q), _p(pos), _p(_c(q, np.float64)), _p(_c(annihilated, np.uint8)), theta, leaf_cap, k,
+                          _p(pts), _p(_c(charges, np.float64)), _p(_c(self_index, np.int64)), _p(f), _p(phi))
+    return f, phi
 
 
 def bh_tree(pos, q, leaf_cap=8):

@@ -1028,6 +1028,40 @@ void orc_bh_potentials(int64_t n, const double* pos, const double* q, const uint
   }
 }
 
+// BarnesHutTree::Build over the alive particles, then ComputeForce(x, q, self) and
+// ComputePotential(x, self) (barnes-hut.cc:166-181) at m given points; self[k] is a particle index
+// (-1 or an annihilated particle: none).  force_out 3m (N), phi_out m (C/m); either may be null.
+void orc_bh_evaluate(int64_t n, const double* pos, const double* q, const uint8_t* annihilated, double theta,
+                     int64_t leaf_cap, int64_t m, const double* pts, const double* charges, const int64_t* self,
+                     double* force_out, double* phi_out) {
+  SetThreads();
+  const std::vector<int64_t> idx = Gather(n, annihilated);
+  std::vector<int64_t> gathered_of(n, -1);
+  for (size_t k = 0; k < idx.size(); ++k) gathered_of[idx[k]] = (int64_t)k;
+  BHTree tree;
+  tree.theta2 = theta * theta;
+  tree.leaf_cap = leaf_cap < 1 ? 1 : leaf_cap;
+  for (int64_t i : idx) { tree.pos.push_back(ld(pos, i)); tree.q.push_back(q[i]); }
+  tree.Build();
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t s = (self && self[k] >= 0 && self[k] < n) ? gathered_of[self[k]] : -1;
+    const V3 x = ld(pts, k);
+    const double qt = charges ? charges[k] : 0.0;
+    if (force_out) {
+      V3 f{0, 0, 0};
+      int64_t cnt = 0;
+      if (!tree.nodes.empty() && qt != 0.0) tree.Accumulate(0, x, qt, s, &f, &cnt);
+      st(force_out, k, f);
+    }
+    if (phi_out) {
+      double phi = 0.0;
+      if (!tree.nodes.empty()) tree.AccumulatePotential(0, x, s, &phi);
+      phi_out[k] = phi;
+    }
+  }
+}
+
 // BH tree structure for tests: returns node count; fills (if non-null, sized >= count)
 // order[n_alive] (gathered-index permutation), node ranges [begin,end), depth, leaf flag,
 // geometric centre and half size, charges and charge centres.  Nodes in the reference's

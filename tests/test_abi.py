@@ -29,7 +29,9 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(syms) if not hasattr(L, s)]
     assert not missing, missing
     # and the Python binding knows every declared entry point
-    assert not (syms - set(am.SIGNATURES)), sorted(syms - set(am.SIGNATURES))
+    from antimatter_amd import params
+    known = set(am.SIGNATURES) | set(params.SIGNATURES)
+    assert not (syms - known), sorted(syms - known)
 
 
 def test_initial_conditions_bit_identical_to_oracle():

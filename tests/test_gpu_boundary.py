@@ -190,6 +190,38 @@ def test_ensemble_rejects_a_reuploaded_store(ctx):
     s.close()
 
 
+# ---- standalone BarnesHutTree evaluation at given points --------------------------------------------
+def test_bh_evaluate_at_points(ctx):
+    rng = np.random.default_rng(19)
+    n = 5000
+    pos = rng.uniform(-1, 1, (n, 3)) * np.array([1e-4, 1e-4, 3e-4])
+    sp = rng.choice([0, 1, 3, 4], n).astype(np.uint8)
+    ann = (rng.random(n) < 0.03).astype(np.uint8)
+    m, q = oracle.mass_charge(sp)
+    s = store_of(ctx, pos, sp, ann)
+    pts = rng.uniform(-1.5, 1.5, (700, 3)) * 1e-4
+    pts[:100] = pos[:100]  # points on particles, 60 of them leaving that particle out
+    qt = rng.choice([-E, E, 0.0], 700)
+    self_idx = np.full(700, -1, np.int64)
+    self_idx[:60] = np.arange(60)
+    print("bh evaluate: 700 points over 5000 particles", flush=True)
+    for theta in (0.0, 0.6):
+        f, phi = s.barnes_hut_evaluate(theta, pts, qt, self_idx, am.FP64)
+        rf, rphi = oracle.bh_evaluate(pos, q, ann, theta, pts, qt, self_idx)
+        np.testing.assert_array_equal(f, rf)
+        np.testing.assert_allclose(phi, rphi, rtol=1e-13, atol=1e-13 * np.abs(rphi).max())
+        f32, phi32 = s.barnes_hut_evaluate(theta, pts, qt, self_idx, am.FP32)
+        ok = np.linalg.norm(rf, axis=1) > 0
+        assert rel_l2(f32[ok], rf[ok]) < 1e-5 and np.all(f32[~ok] == 0)
+        assert rel_l2(phi32, rphi) < 1e-5
+    # at every particle with itself left out: the solver's own forces
+    live = np.where(ann == 0)[0]
+    f, _ = s.barnes_hut_evaluate(0.6, pos[live], q[live], live, am.FP64, want_potential=False)
+    s.compute_barnes_hut_coulomb_forces(0.6, am.FP64)
+    np.testing.assert_array_equal(f, s.forces()[live])
+    s.close()
+
+
 # ---- Barnes-Hut far field on the hand-built cloud ---------------------------------------------------
 def test_bh_far_field_matches_hand_sums(ctx):
     pos, q = scenario()

@@ -284,7 +284,7 @@ def test_energy_matches_oracle(ctx):
 def test_config1_driver_matches_notebook_and_oracle(ctx):
     params = json.load(open(os.path.join(GOLDEN, "params_config1.json")))
     golden = json.load(open(os.path.join(GOLDEN, "config1_notebook.json")))
-    rows, fin = am.simulate(params, ctx=ctx)
+    rows = am.simulate(params)
     orows, ofin = oracle.simulate(params)
     assert rows.shape == orows.shape == (101, 8)
     np.testing.assert_array_equal(rows[:, 5], orows[:, 5])  # identical annihilation history

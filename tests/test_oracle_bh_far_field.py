@@ -158,6 +158,32 @@ def test_oracle_python_binding_on_scenario():
     assert oracle.bh_potential_energy(pos, q, None, 0.45) == pytest.approx(0.5 * K * float(q @ phi), rel=1e-12)
 
 
+def test_oracle_evaluate_at_points():
+    """BH evaluation at given points: at the particles themselves (self = own index) it is the pinned
+    force and potential walk; at theta = 0 it is the direct sum at arbitrary points."""
+    rng = np.random.default_rng(9)
+    n = 800
+    pos = rng.uniform(-1, 1, (n, 3)) * 1e-4
+    q = np.where(rng.random(n) < 0.5, E, -E)
+    ann = (rng.random(n) < 0.05).astype(np.uint8)
+    f, phi = oracle.bh_evaluate(pos, q, ann, 0.6, pos, q, np.arange(n))
+    live = ann == 0
+    np.testing.assert_array_equal(f[live], oracle.bh_forces(pos, q, ann, 0.6)[live])
+    np.testing.assert_array_equal(phi[live], oracle.bh_potentials(pos, q, ann, 0.6)[live])
+    pts = rng.uniform(-2, 2, (50, 3)) * 1e-4
+    qt = np.full(50, -E)
+    f0, phi0 = oracle.bh_evaluate(pos, q, ann, 0.0, pts, qt, None)
+    r = pts[:, None, :] - pos[None, live, :]
+    d = np.linalg.norm(r, axis=2)
+    want_f = (K * qt[:, None, None] * q[None, live, None] * r / d[..., None] ** 3).sum(1)
+    want_phi = (q[None, live] / d).sum(1)
+    np.testing.assert_allclose(f0, want_f, rtol=1e-12, atol=1e-12 * np.abs(want_f).max())
+    np.testing.assert_allclose(phi0, want_phi, rtol=1e-12, atol=1e-12 * np.abs(want_phi).max())
+    # self leaves exactly that source out, wherever the point is
+    f1, phi1 = oracle.bh_evaluate(pos, q, ann, 0.0, pts[:1], qt[:1], np.array([3]))
+    np.testing.assert_allclose(phi1[0], want_phi[0] - q[3] / d[0, np.where(live)[0] == 3][0], rtol=1e-12)
+
+
 # Mistakes planted in a copy of the oracle source: (name, [(text, replacement), ...]).  Each text
 # must occur in the source; every occurrence is replaced (the force and the potential walks).
 MUTANTS = [
