@@ -88,6 +88,8 @@ SIGNATURES = {
     "am_store_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "am_store_set_coulomb_force": (_int, [_vp, _vp]),
     "am_store_count": (_int, [_vp, _vp, _vp]),
+    "am_store_generate_cloud": (_int, [_vp, _vp, _vp, _vp, _u64]),
+    "am_store_download_species": (_int, [_vp, _vp]),
     "am_compute_standard_coulomb_forces": (_int, [_vp, _d, _int]),
     "am_compute_allpairs_coulomb_forces": (_int, [_vp, _int]),
     "am_allpairs_forces_noalloc": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
@@ -282,6 +284,25 @@ class Store:
         t = None if trap is None else np.ascontiguousarray(trap, dtype=np.uint32)
         _check(lib().am_store_upload(self.h, n, _p(pos), _p(vel), _p(species), _p(c), _p(m), _p(a), _p(t)))
         self.n = n
+
+    def generate_cloud(self, populations: dict, temperatures: dict, radii, seed: int):
+        """Seeded cloud drawn on the device (am_store_generate_cloud): the reference's distributions,
+        not its draws; for clouds of millions of particles."""
+        c = np.zeros(6, np.uint64)
+        t = np.zeros(6)
+        for name, cnt in populations.items():
+            c[SPECIES[name] if isinstance(name, str) else int(name)] = int(cnt)
+        for name, T in temperatures.items():
+            t[SPECIES[name] if isinstance(name, str) else int(name)] = float(T)
+        r = np.ascontiguousarray(radii, dtype=np.float64)
+        _check(lib().am_store_generate_cloud(self.h, _p(c), _p(r), _p(t), int(seed)))
+        self.n = int(c.sum())
+
+    def species(self):
+        n = self.count()[0]
+        sp = np.zeros(n, np.uint8)
+        _check(lib().am_store_download_species(self.h, _p(sp)))
+        return sp
 
     def download(self):
         n = ctypes.c_int64()

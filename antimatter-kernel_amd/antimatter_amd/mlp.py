@@ -179,5 +179,10 @@ def train_on_clouds(ctx: Context, clouds, theta: float, net: Mlp, targets_per_cl
     X = np.concatenate(xs)
     Y = np.concatenate(ys)
     net.fit_normalization(X, Y)
+    # The error bound of the correction: a prediction larger than almost every label seen in training
+    # (|y| above the 99.9th percentile) is not trusted, and the row takes the exact residual.
+    mags = np.linalg.norm(Y, axis=1)
+    net.label_bound = float(np.quantile(mags, 0.999)) if len(mags) else 0.0
+    net.max_h_over_r = float(X[:, 3].max()) if len(X) else 0.0
     losses = net.train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return len(X), losses

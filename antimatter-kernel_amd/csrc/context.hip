@@ -409,6 +409,14 @@ int am_store_set_coulomb_force(am_store* s, const double* force) {
   return AM_OK;
 }
 
+int am_store_download_species(am_store* s, uint8_t* species) {
+  if (!s || (!species && s->n > 0)) return Fail(AM_ERR_INVALID, "am_store_download_species: NULL argument");
+  if (s->n == 0) return AM_OK;
+  AM_HIP(hipMemcpyAsync(species, s->species, (size_t)s->n, hipMemcpyDeviceToHost, s->ctx->stream));
+  AM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return AM_OK;
+}
+
 int am_store_count(am_store* s, int64_t* n, int64_t* n_active) {
   if (!s) return Fail(AM_ERR_INVALID, "am_store_count: NULL store");
   if (n) *n = s->n;

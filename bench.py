@@ -15,8 +15,11 @@ Other workloads (--workload), each on its full configuration with its own JSON l
                   B0 = 1 T, U0 = 10 V; per step Barnes-Hut theta = 0.7 (FP32 walk) + push +
                   annihilation (5e-6 m, p = 0.5)
   fast-multipole  the same cloud and step with FMM order 4, theta = 0.5 (FP32 near field)
-  bh-learned      the same cloud and step with the learned correction at theta = 0.9, trained
-                  first (outside the timed region) on 16 clouds of 262,144 particles
+  bh-learned      N = 1,048,576 on the all-pairs workload's antiproton/positron cloud (radii (1e-3,
+                  1e-3, 4e-3) m, 10 K / 100 K, B0 = 3 T, U0 = 50 V); per step Barnes-Hut theta = 0.9
+                  with the learned correction under its error bound + push; trained first (outside
+                  the timed region) on 16 clouds of 262,144 particles at T log-uniform in 4..1e4 K;
+                  plain Barnes-Hut at several theta on the same state gives the equal-error ratio
   ensemble        512 independent traps (N log-uniform in 1,024..32,768), one hipGraph-replayed
                   step of all-pairs forces + push with cooling + annihilation
 --copy-calibration BYTES runs a device copy of that size (the reference for HBM byte counters).
@@ -104,6 +107,8 @@ N_TREE = 1 << 20
 TREE_CONFIG = ("N=1048576 positrons + electrons (50/50), radii (2e-3,2e-3,8e-3) m, 4 K, B0=1 T, U0=10 V, "
                "z0=rho0=0.01 m; annihilation 5e-6 m, p=0.5")
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+CONFIG4 = ("N=1048576 antiprotons + positrons (50/50), radii (1e-3,1e-3,4e-3) m, 10 K / 100 K, B0=3 T, U0=50 V, "
+           "z0=rho0=0.01 m (the all-pairs workload's cloud at 1M)")
 
 
 def log(msg):
@@ -124,7 +129,7 @@ def force_errors(am, store, f):
     store.compute_allpairs_coulomb_forces(am.FP64)
     ref = store.forces()
     per = np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
-    return float(np.linalg.norm(f - ref) / np.linalg.norm(ref)), float(np.median(per)), float(per.max())
+    return float(np.linalg.norm(f - ref) / np.linalg.norm(ref)), float(np.median(per)), float(per.max()), ref
 
 
 def kernel_avg(ctx, name):
@@ -136,7 +141,9 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
     import antimatter_amd as am
     from antimatter_amd import mlp
     log(f"{kind}: building the cloud")
-    pos, vel, sp, trap, dt = tree_cloud()
+    learned = kind == "bh-learned"
+    # the learned workload runs on the all-pairs workload's antiproton/positron cloud at N = 1M
+    pos, vel, sp, trap, dt = workload(N_TREE) if learned else tree_cloud()
     n = len(sp)
     ctx = am.Context(0)
     store = am.Store(ctx, n)
@@ -158,19 +165,27 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
         rng = np.random.default_rng(7)
         t0 = time.perf_counter()
         clouds = []
-        for k in range(16):
+        for k in range(16):  # 16 seeded clouds of 262,144, both species at T log-uniform in 4..1e4 K
             temp = float(np.exp(rng.uniform(np.log(4.0), np.log(1e4))))
-            clouds.append(tree_cloud(262144, 100 + k, temp)[:3])
+            c = am.initial_conditions({"antiproton": 131072, "positron": 131072},
+                                      {"antiproton": temp, "positron": temp}, (1e-3, 1e-3, 4e-3), 1 + k)
+            clouds.append(c)
         log("bh-learned: generating training rows and training")
-        rows, losses = mlp.train_on_clouds(ctx, clouds, theta, net, targets_per_cloud=512, rec_cap=2048, epochs=20,
+        rows, losses = mlp.train_on_clouds(ctx, clouds, theta, net, targets_per_cloud=1200, rec_cap=2048, epochs=20,
                                            batch=1024, lr=2e-3, seed=1)
+        del clouds
+        bound = {"ybound": net.label_bound, "amax": net.max_h_over_r}
         extra["training"] = {"clouds": 16, "cloud_n": 262144, "rows": rows, "epochs": 20,
                              "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                              "seconds": time.perf_counter() - t0}
+        extra["error_bound"] = dict(bound, rule="rows whose |prediction| exceeds the 99.9th percentile of the "
+                                                "training labels' |y|, or whose h/R exceeds the training range, "
+                                                "take the exact residual")
         stats = {}
 
         def forces():
-            stats.update(mlp.barnes_hut_learned_forces(store, theta, net, rec_cap=1024, ybound=1e30, amax=1e30))
+            stats.update(mlp.barnes_hut_learned_forces(store, theta, net, rec_cap=2048, ybound=bound["ybound"],
+                                                       amax=bound["amax"]))
         kernel, label = "bh_learned_correct", f"Barnes-Hut theta={theta} + learned correction (MLP 16-64-64-3)"
 
     active = {"now": int(store.count()[1]), "sum": 0}
@@ -179,7 +194,8 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
         active["sum"] += active["now"]  # the particles this step advances
         forces()
         store.push(trap, dt, 0.0, 0.0, 42, k)
-        active["now"] -= store.annihilation(5e-6, 0.5, 42, k)
+        if not learned:  # antiprotons and positrons never annihilate with one another
+            active["now"] -= store.annihilation(5e-6, 0.5, 42, k)
 
     log(f"{kind}: timing {steps} steps")
     for k in range(warmup):
@@ -202,13 +218,14 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
     forces()
     f = store.forces()
     log(f"{kind}: FP64 direct reference")
-    err, med, mx = force_errors(am, store, f)
+    err, med, mx, ref = force_errors(am, store, f)
     out = {
         "metric": f"particle-steps/s, {label}, force error vs FP64 direct",
         "value": particle_steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (reference initial-condition generators, seed 42)",
-        "config": {"workload": f"{label}; push + annihilation per step; {TREE_CONFIG}", "n_particles": n},
+        "config": {"workload": f"{label}; push per step; {CONFIG4}" if learned else
+                   f"{label}; push + annihilation per step; {TREE_CONFIG}", "n_particles": n},
         "active_particle_steps": particle_steps, "active_at_end": active["now"],
         "force_rel_err_l2": err, "force_rel_err_median": med, "force_rel_err_max": mx,
         "kernel_ms": parts,
@@ -234,6 +251,7 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
         rows = stats["model_rows"] + stats["fallback_rows"]
         flops = 2.0 * (16 * 64 + 64 * 64 + 64 * 3) * rows
         out["learned"] = dict(stats, **extra)
+        out["equal_error_comparison"] = learned_vs_plain(am, ctx, store, forces, ref, err)
         out["roofline"] = {"bound": "mfma", "achieved": flops / (avg_ms * 1e-3) / 1e12,
                            "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": flops / (avg_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
@@ -244,6 +262,31 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
     store.close()
     ctx.close()
     return out
+
+
+def learned_vs_plain(am, ctx, store, learned_forces, ref, learned_err):
+    """Plain FP32 Barnes-Hut at several theta on the same state: wall time per call (tree build
+    included) and force error against the same FP64 direct sum, beside the learned call; the plain
+    theta whose error first drops to the learned error gives the speed ratio at equal error."""
+    def timed(call):
+        call()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            call()
+        ctx.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / 3
+
+    learned_ms = timed(learned_forces)
+    rows = []
+    for th in (0.9, 0.8, 0.7, 0.6, 0.5, 0.4):
+        ms = timed(lambda: store.compute_barnes_hut_coulomb_forces(th, am.FP32))
+        e = float(np.linalg.norm(store.forces() - ref) / np.linalg.norm(ref))
+        rows.append({"theta": th, "ms_per_call": ms, "force_rel_err_l2": e})
+    match = next((r for r in rows if r["force_rel_err_l2"] <= learned_err), None)
+    return {"learned_ms_per_call": learned_ms, "learned_force_rel_err_l2": learned_err, "plain": rows,
+            "plain_theta_at_equal_error": None if match is None else match["theta"],
+            "learned_speed_vs_plain_at_equal_error": None if match is None else match["ms_per_call"] / learned_ms}
 
 
 def tree_cpu_baseline(kind, pos, sp, cpu_steps):

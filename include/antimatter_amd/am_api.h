@@ -105,6 +105,16 @@ int am_set_particle_velocities(const uint64_t counts[6], const double temperatur
  * in particle_populations. */
 double am_derive_dt(const int32_t listed[6], double b0);
 
+/* Seeded cloud on the device, for clouds of millions of particles (the reference draws its clouds
+ * serially, penning_trap.cc:51-161; am_set_particle_* reproduce those draws on the host).  Same
+ * distributions, counter-based draws keyed on (seed, particle index), so NOT bit-identical to the
+ * host generator: counts[s] particles of species s in enum order, positions uniform in the
+ * ellipsoid of the given radii, velocities Maxwellian at temperatures[s] with each species' mean
+ * removed and rescaled by sqrt(n / (n - 1)) as the reference does.  Fills the store (species
+ * defaults for charge and mass, nothing annihilated, trap 0) and sets its particle count. */
+int am_store_generate_cloud(am_store* store, const uint64_t counts[6], const double radii[3],
+                            const double temperatures[6], uint64_t seed);
+
 /* ---- context ------------------------------------------------------------------------------- */
 int am_ctx_create(int device, am_ctx** out);
 int am_ctx_destroy(am_ctx* ctx);
@@ -132,6 +142,8 @@ int am_store_download(am_store* store, double* pos, double* vel, double* coulomb
 /* Overwrite the Coulomb force buffer (SetCoulombForce for all agents); 3n doubles. */
 int am_store_set_coulomb_force(am_store* store, const double* force);
 int am_store_count(am_store* store, int64_t* n, int64_t* n_active);
+/* The species of the n particles (GetSpecies for every agent): n bytes. */
+int am_store_download_species(am_store* store, uint8_t* species);
 
 /* ---- Coulomb solvers: write the store's Coulomb force for every alive particle ---------------- */
 /* ComputeStandardCoulombForces (src/interaction/coulomb_interaction.h:10, .cc:23-80): pairs with

@@ -187,6 +187,51 @@ def test_fallback_everywhere_gives_the_direct_sum(ctx, small_cloud):
     net.close()
 
 
+def config2_cloud(n, seed, t_pbar=10.0, t_pos=100.0):
+    """The learned-correction workload's cloud (the all-pairs workload's setup): antiprotons and
+    positrons 50/50 in radii (1e-3, 1e-3, 4e-3) m."""
+    pops = {"antiproton": n // 2, "positron": n - n // 2}
+    return am.initial_conditions(pops, {"antiproton": t_pbar, "positron": t_pos}, (1e-3, 1e-3, 4e-3), seed)
+
+
+def test_learned_correction_at_theta_09_on_the_pbar_positron_cloud(ctx):
+    """The learned workload's setting: theta = 0.9 on the antiproton/positron cloud, trained on four
+    seeded clouds at temperatures log-uniform in 4..1e4 K, evaluated on the workload's own cloud
+    (10 K / 100 K, seed 42) with the error bound active: predictions beyond the 99.9th percentile of
+    the training labels, or nodes with h/R beyond the training range, take the exact residual.  The
+    correction lowers the plain walk's force error at the same theta by at least 30 %."""
+    theta = 0.9
+    n = 65536
+    rng = np.random.default_rng(5)
+    clouds = []
+    for seed in (61, 62, 63, 64):
+        t = float(np.exp(rng.uniform(np.log(4.0), np.log(1e4))))
+        clouds.append(config2_cloud(n, seed, t, t))
+    net = mlp.Mlp(ctx, seed=3)
+    rows, losses = mlp.train_on_clouds(ctx, clouds, theta, net, targets_per_cloud=768, rec_cap=4096, epochs=30,
+                                       batch=1024, lr=2e-3, seed=1)
+    print(f"{rows} rows; losses {losses[0]:.4f} -> {losses[-1]:.4f}; bound {net.label_bound:.3f} "
+          f"h/R max {net.max_h_over_r:.3f}")
+    assert losses[-1] < 0.2 * losses[0]
+    pos, vel, sp = config2_cloud(2 * n, 42)
+    s = am.Store(ctx, len(sp))
+    s.upload(pos, vel, sp)
+    m, q = oracle.mass_charge(sp)
+    tgt = np.random.default_rng(2).choice(len(sp), 512, replace=False)
+    ref = oracle.allpairs_forces_subset(pos, q, None, tgt)
+    s.compute_barnes_hut_coulomb_forces(theta, am.FP32)
+    e_plain = np.linalg.norm(s.forces()[tgt] - ref) / np.linalg.norm(ref)
+    st = mlp.barnes_hut_learned_forces(s, theta, net, rec_cap=4096, ybound=net.label_bound,
+                                       amax=net.max_h_over_r)
+    e_learn = np.linalg.norm(s.forces()[tgt] - ref) / np.linalg.norm(ref)
+    print(f"theta 0.9: plain {e_plain:.3e} learned {e_learn:.3e} stats {st}")
+    assert st["overflowed"] == 0 and st["model_rows"] > 0 and st["fallback_rows"] > 0
+    assert st["fallback_rows"] < 0.01 * st["model_rows"]
+    assert e_learn < 0.7 * e_plain
+    s.close()
+    net.close()
+
+
 def test_trained_network_beats_plain_barnes_hut(ctx):
     """Trained on four clouds, evaluated on a fifth (seeds differ), at theta = 0.6: the learned
     correction lowers the force error of the plain walk at the same theta at least twofold, and
