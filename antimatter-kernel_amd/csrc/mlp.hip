@@ -11,10 +11,11 @@
 // network predicts y by itself (default), or, in the quadrupole mode, what the analytic quadrupole
 // term of the same features leaves (octupole and beyond, which the features determine only in part).
 //
-// Inference: BF16 weights and activations on MFMA (v_mfma_f32_32x32x8bf16_1k), FP32 accumulation,
-// 32 rows per wave.  Training: the same BF16 rounding of weights and activations in the forward
-// pass, FP32 gradients (straight through the rounding), Adam; a fused per-32-row-tile kernel writes
-// per-tile gradient partials that a second kernel reduces in a fixed order.
+// Inference: BF16 weights and activations on MFMA (v_mfma_f32_32x32x16_bf16), FP32 accumulation,
+// 32 rows per batch, SiLU from an interpolated table in LDS.  Training: the same BF16 rounding of
+// weights and activations in the forward pass, exact SiLU, FP32 gradients (straight through the
+// rounding), Adam; a fused per-32-row-tile kernel writes per-tile gradient partials that a second
+// kernel reduces in a fixed order.
 // Fallback: a row whose prediction exceeds the bound |y| <= ybound, or whose h/R exceeds
 // amax, is corrected with the exact residual summed over the node's sign-s particles.
 #include <algorithm>
@@ -48,6 +49,21 @@ __host__ __device__ inline uint16_t Bf16Bits(float f) {
 // SiLU and its derivative, shared by the training and inference kernels.
 __device__ inline float SigmoidFast(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 __device__ inline float Silu(float z) { return z * SigmoidFast(z); }
+
+// The inference kernels take SiLU from a table in LDS: sigma at z_k = -16 + k/32 (k = 0..1024),
+// linearly interpolated, so the quarter-rate exp and reciprocal per hidden value become a few
+// full-rate operations and one LDS read.  Interpolation error: h^2/8 max|sigma''| = 1.2e-5 in sigma
+// (h = 1/32), far below the bf16 rounding of the activation that follows; outside [-16, 16] sigma
+// is clamped to its end values (within 1.2e-7 of 0 and 1).  Entry k holds (sigma_k,
+// sigma_{k+1} - sigma_k).
+constexpr int kSiluTab = 1024;
+
+__device__ inline float SiluTab(const float2* __restrict__ tab, float z) {
+  const float t = fminf(fmaxf(fmaf(z, 32.f, 512.f), 0.f), 1023.99f);  // NaN z gives t = 0
+  const int i = (int)t;
+  const float2 e = tab[i];
+  return z * fmaf(t - (float)i, e.y, e.x);
+}
 __device__ inline float SiluGrad(float z) {
   const float s = SigmoidFast(z);
   return s * (1.f + z * (1.f - s));
@@ -247,16 +263,29 @@ __host__ __device__ inline void QuadBaseline(const float* f, float y[3]) {
   y[2] = 2.5f * uqu * uz - quz;
 }
 
-// Per-wave network state: weight fragments in registers, FP32 biases of the hidden layers in LDS.
+// Per-wave network state: weight fragments in registers; FP32 biases of the hidden layers and the
+// SiLU table in LDS (one wave per workgroup).
 struct MlpWave {
   bf16x8 wf[kFrags];
   float b3[kOut], oscale[kOut];
 };
 
-__device__ inline void MlpWaveLoad(const MlpDev& net, float* bias_lds, MlpWave& s) {
+struct MlpLds {
+  float4 bias4[2 * kH / 4];
+  float2 silu[kSiluTab];
+  __device__ const float* bias() const { return reinterpret_cast<const float*>(bias4); }
+};
+
+__device__ inline void MlpWaveLoad(const MlpDev& net, MlpLds& lds, MlpWave& s) {
   const int l = threadIdx.x & 63;
-  bias_lds[l] = net.w[kOffB1 + l];
-  bias_lds[kH + l] = net.w[kOffB2 + l];
+  float* bias = reinterpret_cast<float*>(lds.bias4);
+  bias[l] = net.w[kOffB1 + l];
+  bias[kH + l] = net.w[kOffB2 + l];
+  for (int k = l; k < kSiluTab; k += 64) {
+    const float z = -16.f + (float)k * (1.f / 32.f);
+    const float s0 = SigmoidFast(z);
+    lds.silu[k] = make_float2(s0, SigmoidFast(z + 1.f / 32.f) - s0);
+  }
   const bf16x8* p = reinterpret_cast<const bf16x8*>(net.frag);
   for (int f = 0; f < kFrags; ++f) s.wf[f] = p[f * 64 + l];
   for (int o = 0; o < kOut; ++o) {
@@ -279,6 +308,23 @@ __device__ inline bf16x8 MlpInput(const MlpNorm& nm, const float f[kIn], bool ok
   return xb;
 }
 
+// Both halves of a row's standardised features (features 0..7 and 8..15), for a lane that owns a
+// whole row.
+__device__ inline void MlpInputHalves(const MlpNorm& nm, const float f[kIn], bool ok, bf16x8& lo, bf16x8& hi) {
+  for (int j = 0; j < 8; ++j) {
+    lo[j] = (__bf16)(ok ? (f[j] - nm.in_mean[j]) * nm.in_istd[j] : 0.f);
+    hi[j] = (__bf16)(ok ? (f[8 + j] - nm.in_mean[8 + j]) * nm.in_istd[8 + j] : 0.f);
+  }
+}
+
+// v from lane l ^ 32 (the other half of the wave).
+__device__ inline bf16x8 OtherHalf(const bf16x8 v) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 u = __builtin_bit_cast(u32x4, v);
+  for (int k = 0; k < 4; ++k) u[k] = (uint32_t)__shfl_xor((int)u[k], 32);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
 __device__ inline void HiddenFromBias(const float* bias, f32x16& acc) {
   const int h = (threadIdx.x & 63) >> 5;
   for (int g = 0; g < 4; ++g) {
@@ -290,27 +336,28 @@ __device__ inline void HiddenFromBias(const float* bias, f32x16& acc) {
   }
 }
 
-__device__ inline void SiluPack(const f32x16 acc[2], bf16x8 hb[4]) {
+__device__ inline void SiluPack(const float2* __restrict__ tab, const f32x16 acc[2], bf16x8 hb[4]) {
   for (int ks = 0; ks < 4; ++ks)
-    for (int j = 0; j < 8; ++j) hb[ks][j] = (__bf16)Silu(acc[ks >> 1][8 * (ks & 1) + j]);
+    for (int j = 0; j < 8; ++j) hb[ks][j] = (__bf16)SiluTab(tab, acc[ks >> 1][8 * (ks & 1) + j]);
 }
 
-// One batch of 32 rows (row n in lanes n and n + 32).  Returns the de-normalised outputs of row n
-// in lane n (lanes 32..63 return unspecified values).
-__device__ inline void MlpForwardRegs(const MlpWave& s, const float* bias_lds, const bf16x8 xb, float y[kOut]) {
+// One batch of 32 rows (features 0..7 of row n in lane n, 8..15 in lane n + 32).  Returns the
+// de-normalised outputs of row n in lane n (lanes 32..63 return unspecified values).
+__device__ inline void MlpForwardRegs(const MlpWave& s, const MlpLds& lds, const bf16x8 xb, float y[kOut]) {
+  const float* bias_lds = lds.bias();
   f32x16 acc[2];
   for (int mb = 0; mb < 2; ++mb) {
     HiddenFromBias(bias_lds + 32 * mb, acc[mb]);
     acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s.wf[mb], xb, acc[mb], 0, 0, 0);
   }
   bf16x8 hb[4];
-  SiluPack(acc, hb);
+  SiluPack(lds.silu, acc, hb);
   for (int mb = 0; mb < 2; ++mb) {
     HiddenFromBias(bias_lds + kH + 32 * mb, acc[mb]);
     for (int ks = 0; ks < 4; ++ks)
       acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(s.wf[kFragL1 + 4 * mb + ks], hb[ks], acc[mb], 0, 0, 0);
   }
-  SiluPack(acc, hb);
+  SiluPack(lds.silu, acc, hb);
   f32x16 o;
   for (int v = 0; v < 16; ++v) o[v] = 0.f;
   for (int ks = 0; ks < 4; ++ks)
@@ -321,10 +368,9 @@ __device__ inline void MlpForwardRegs(const MlpWave& s, const float* bias_lds, c
 // Batched inference (raw features in, de-normalised predictions out), one wave per 32 rows.
 __global__ __launch_bounds__(64) void MlpInferKernel(int rows, const float* __restrict__ X, MlpDev net,
                                                      float* __restrict__ Yout) {
-  __shared__ float4 bias4[2 * kH / 4];
-  float* bias_lds = reinterpret_cast<float*>(bias4);
+  __shared__ MlpLds lds;
   MlpWave ws;
-  MlpWaveLoad(net, bias_lds, ws);
+  MlpWaveLoad(net, lds, ws);
   const int l = threadIdx.x, n = l & 31;
   const int r = blockIdx.x * 32 + n;
   float f[kIn];
@@ -332,7 +378,7 @@ __global__ __launch_bounds__(64) void MlpInferKernel(int rows, const float* __re
   float yb[3] = {0.f, 0.f, 0.f};
   if (net.baseline) QuadBaseline(f, yb);
   float y[kOut];
-  MlpForwardRegs(ws, bias_lds, MlpInput(net.norm, f, true), y);
+  MlpForwardRegs(ws, lds, MlpInput(net.norm, f, true), y);
   if (l < 32 && r < rows)
     for (int o = 0; o < kOut; ++o) Yout[(size_t)r * kOut + o] = y[o] + yb[o];
 }
@@ -446,9 +492,13 @@ __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, 
   }
 }
 
-// Correction pass: one wave per target (sorted position k); rows = (recorded node, sign), 32 per
-// MLP batch.  Adds the predicted residual (or, past the bound, the exact one) to the force.
-__global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __restrict__ rec, int rec_cap,
+// Correction pass: one wave per target (sorted position k); rows = (recorded node, sign), one
+// per lane, 64 per pass as two MLP batches of 32.  Lane l computes the features of row base + l
+// once; the halves that the other batch's B operand needs cross the wave (lane l ^ 32), and batch
+// 2's outputs come back to the lanes of their rows the same way.  Adds the predicted residual (or,
+// past the bound, the exact one) to the force.  Each lane sums its rows' corrections in FP64 and
+// the wave adds the 64 partial sums once per target in a fixed order: deterministic.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void MlpCorrectKernel(int m, const uint32_t* __restrict__ rec, int rec_cap,
                                                        const uint32_t* __restrict__ rec_cnt,
                                                        const WalkNode* __restrict__ nodes,
                                                        const WalkNodeF32* __restrict__ nodes32,
@@ -458,11 +508,11 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
                                                        MlpDev net, float ybound, float amax,
                                                        double* __restrict__ fx, double* __restrict__ fy,
                                                        double* __restrict__ fz, unsigned long long* __restrict__ stats) {
-  __shared__ float4 bias4[2 * kH / 4];
-  float* bias_lds = reinterpret_cast<float*>(bias4);
+  __shared__ MlpLds lds;
   MlpWave ws;
-  MlpWaveLoad(net, bias_lds, ws);
-  const int l = threadIdx.x, n = l & 31;
+  MlpWaveLoad(net, lds, ws);
+  const int l = threadIdx.x;
+  const bool upper = l >= 32;
   unsigned long long n_model = 0, n_fallback = 0, n_over = 0;
   // Persistent over targets: the weight fragments stay in registers for the whole launch.
   for (int k = blockIdx.x; k < m; k += gridDim.x) {
@@ -473,8 +523,8 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
     n_over += (int)rec_cnt[k] > rec_cap ? 1 : 0;
     const int nrows = 2 * cnt;
     double cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int base = 0; base < nrows; base += 32) {
-      const int row = base + n;  // lanes n and n + 32 carry the same row
+    for (int base = 0; base < nrows; base += 64) {
+      const int row = base + l;
       bool ok = false;
       float R = 1.f, Qs = 0.f, hh = 0.f;
       float yb[3] = {0.f, 0.f, 0.f};
@@ -486,11 +536,21 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
         ok = RowFeatures(th, tl, nodes32[c], feat[c], row & 1, f, R, Qs, hh);
         if (ok && net.baseline) QuadBaseline(f, yb);
       }
+      bf16x8 lo, hi;
+      MlpInputHalves(net.norm, f, ok, lo, hi);
+      // batch 1 = rows base .. base+31, batch 2 = rows base+32 .. base+63: lane n < 32 feeds
+      // features 0..7 of row n of its batch, lane n + 32 features 8..15 of row n
+      const bf16x8 got = OtherHalf(upper ? lo : hi);
+      float y1[kOut], y2[kOut] = {0.f, 0.f, 0.f};
+      MlpForwardRegs(ws, lds, upper ? got : lo, y1);
+      if (base + 32 < nrows) MlpForwardRegs(ws, lds, upper ? hi : got, y2);
       float y[kOut];
-      MlpForwardRegs(ws, bias_lds, MlpInput(net.norm, f, ok), y);
-      ok = ok && l < 32;
-      float corr[3] = {0.f, 0.f, 0.f};
+      for (int o = 0; o < kOut; ++o) {
+        const float moved = __shfl_xor(y2[o], 32);  // batch 2's row n + 32 is output in lane n
+        y[o] = upper ? moved : y1[o];
+      }
       if (ok) {
+        float corr[3];
         const float p0 = y[0] + yb[0], p1 = y[1] + yb[1], p2 = y[2] + yb[2];
         const float a = hh / R;
         const float mag = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
@@ -504,15 +564,15 @@ __global__ __launch_bounds__(64) void MlpCorrectKernel(int m, const uint32_t* __
           ExactResidual(i, th, tl, nodes[c], nodes32[c], row & 1, order, src, corr);
           ++n_fallback;
         }
+        cx += corr[0];
+        cy += corr[1];
+        cz += corr[2];
       }
-      for (int off = 16; off > 0; off >>= 1) {
-        corr[0] += __shfl_xor(corr[0], off);
-        corr[1] += __shfl_xor(corr[1], off);
-        corr[2] += __shfl_xor(corr[2], off);
-      }
-      cx += corr[0];
-      cy += corr[1];
-      cz += corr[2];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      cx += __shfl_xor(cx, off);
+      cy += __shfl_xor(cy, off);
+      cz += __shfl_xor(cz, off);
     }
     if (l == 0 && qi != 0.0) {
       const double scale = kCoulomb * qi * kElementaryCharge * kInvUnit2;
