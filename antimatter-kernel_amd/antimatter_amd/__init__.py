@@ -100,6 +100,7 @@ SIGNATURES = {
     "am_bh_tree_download": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces": (_int, [_vp, _int, _d]),
     "am_fmm_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "am_fmm_m2l_stats": (_int, [_vp, _vp, _vp]),
     "am_compute_fast_multipole_coulomb_forces_ex": (_int, [_vp, _int, _d, _int]),
     "am_lorentz_force": (_int, [_vp, _vp, _d]),
     "am_cyclotron_cooling": (_int, [_vp, _vp, _d, _d, _d, _u64, _u64]),
@@ -129,6 +130,7 @@ SIGNATURES = {
     "am_debug_exclusive_scan": (_int, [_vp, _vp, _i64, _vp]),
     "am_debug_radix_sort": (_int, [_vp, _vp, _vp, _i64, _int]),
     "am_debug_radix_sort_keys": (_int, [_vp, _vp, _i64, _int, _int]),
+    "am_debug_fmm_m2l_options": (_int, [_vp, _int, _i64]),
 }
 
 _lib = None
@@ -230,6 +232,11 @@ class Context:
         k = np.array(keys, dtype=np.uint64)
         _check(lib().am_debug_radix_sort_keys(self.h, _p(k), len(k), int(bit0), int(key_bits)))
         return k
+
+    def fmm_m2l_options(self, classes: bool = True, max_unclassed: int = 1 << 20):
+        """FMM M2L diagnostics: classes=False gives every M2L pair its own tensor; past max_unclassed
+        unclassed pairs (always, when negative) FP32 FMM calls run the FP64 contraction."""
+        _check(lib().am_debug_fmm_m2l_options(self.h, int(bool(classes)), int(max_unclassed)))
 
     def enable_timing(self, on: bool = True):
         _check(lib().am_ctx_enable_timing(self.h, int(on)))
@@ -387,8 +394,10 @@ class Store:
         d = ctypes.c_int32()
         c = ctypes.c_int64()
         _check(lib().am_fmm_stats(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(d), ctypes.byref(c)))
+        u, it = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().am_fmm_m2l_stats(self.h, ctypes.byref(u), ctypes.byref(it)))
         return dict(sources=v[0].value, cells=v[1].value, m2l_pairs=v[2].value, p2p_cell_pairs=v[3].value,
-                    max_depth=d.value, m2l_classes=c.value)
+                    max_depth=d.value, m2l_classes=c.value, m2l_unclassed=u.value, m2l_items=it.value)
 
     # ---- behaviours ----
     def lorentz_force(self, field: TrapField, dt: float):

@@ -57,6 +57,7 @@ struct Buffer {
 // Sizes of the most recent FMM call on a store (am_fmm_stats).
 struct FmmRunStats {
   int64_t cells = 0, leaves = 0, m2l_pairs = 0, p2p_cell_pairs = 0, m = 0, m2l_classes = 0;
+  int64_t m2l_unclassed = 0, m2l_items = 0;  // pairs outside every class; MFMA M2L work items
   int max_depth = 0;
 };
 
@@ -76,6 +77,10 @@ struct am_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   bool timing = false;
+  // Unclassed M2L pairs that the FP32 MFMA M2L gives tensors of their own; past this count the FP64
+  // contraction runs instead.
+  int64_t fmm_max_unclassed = 1 << 20;
+  bool fmm_classes = true;  // false: no M2L displacement classes (every pair unclassed; diagnostic)
   struct Pending {
     std::string name;
     hipEvent_t start, stop;

@@ -619,15 +619,19 @@ __device__ inline void M2LClassDisplacement(uint64_t key, double root_half, doub
   xs[2] = (double)((int64_t)(key & kQMask) - kQOff) * hf;
 }
 
-__global__ void ClassClearKernel(uint64_t size, uint64_t* __restrict__ hkeys) {
+// Empties the table and zeroes the unclassed-pair count.
+__global__ void ClassClearKernel(uint64_t size, uint64_t* __restrict__ hkeys, uint32_t* __restrict__ n_uncl) {
   const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (h < size) hkeys[h] = kClassEmpty;
+  if (h == 0) *n_uncl = 0u;
 }
 
-// Inserts every pair's key; pair_slot[q] = its table slot, or kNoClass.  geo holds each cell's
-// (centre, half width): 32 bytes instead of the whole cell record per lookup.
+// Inserts every pair's key; pair_slot[q] = its table slot, or kNoClass (counted in *n_uncl).  geo
+// holds each cell's (centre, half width): 32 bytes instead of the whole cell record per lookup.
+// max_probes = 0 leaves every pair unclassed (a diagnostic of the unclassed paths).
 __global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pairs, const double4* __restrict__ geo,
-                                     uint64_t* __restrict__ hkeys, uint64_t mask, uint32_t* __restrict__ pair_slot) {
+                                     uint64_t* __restrict__ hkeys, uint64_t mask, int max_probes,
+                                     uint32_t* __restrict__ pair_slot, uint32_t* __restrict__ n_uncl) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
   const uint64_t pk = pairs[q];
@@ -636,7 +640,7 @@ __global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pa
   uint32_t slot = kNoClass;
   if (M2LClassKey(geo[pk >> 32], geo[(uint32_t)pk], root_exp, &key)) {
     uint64_t h = CellHashMix(key) & mask;
-    for (int probe = 0; probe < kClassProbes; ++probe) {
+    for (int probe = 0; probe < max_probes; ++probe) {
       const uint64_t cur = hkeys[h];
       if (cur == key) {
         slot = (uint32_t)h;
@@ -655,6 +659,7 @@ __global__ void M2LClassInsertKernel(int64_t np, const uint64_t* __restrict__ pa
     }
   }
   pair_slot[q] = slot;
+  if (slot == kNoClass) atomicAdd(n_uncl, 1u);
 }
 
 __global__ void ClassOccupiedKernel(uint64_t size, const uint64_t* __restrict__ hkeys, uint32_t* __restrict__ occ) {
@@ -862,16 +867,28 @@ __global__ void TensorToF32Kernel(int64_t n, const double* __restrict__ Dc, floa
 // For displacement class c the M2L is one fixed matrix: L_gamma += sum_alpha A_c[gamma][alpha] M_alpha
 // with A_c[gamma][alpha] = (-1)^|alpha| D_{alpha+gamma} / gamma!.  The sign is folded into the FP32
 // multipoles and 1/gamma! is applied in FP64 to each target's sum, so the product matrix is the plain
-// gather D_{alpha+gamma} from the class tensor.  Pairs sorted by class run as dense products
-// D_c [M'_S1 .. M'_S16] on v_mfma_f32_16x16x4f32, 16 pairs per product; the fragments of D_c stay in
-// registers while consecutive groups share the class.  Each pair's result row goes to its slot of
-// the target-grouped list straight from the product registers, and M2LSumKernel adds a target's
-// rows in list order, so the sum is deterministic.  FP32 rounding (~1e-7 relative) is far below the
-// order-4/5 truncation error.
+// gather D_{alpha+gamma} from the class tensor.  Pairs of one class run as dense products
+// D_c [M'_S1 .. M'_S16] on v_mfma_f32_16x16x4f32, up to 16 pairs (one per column) per product.
+//
+// Accumulation without per-pair rows: the pairs are sorted by (target group, class), a group being
+// kM2LGroupTargets consecutive BFS ids (cells of one level in Morton order, so neighbouring targets
+// share most of their classes: at N = 1M about 85% of the 16 columns of a product are filled).  Two
+// pairs of one class never share a target (cell centres are distinct, so a target and a displacement
+// fix the source), so the columns of a product belong to distinct targets of the group.  A
+// workgroup takes one item (a contiguous piece of a group's pair list), keeps the group's locals in
+// LDS in FP64, and writes them as the item's partial; a reduction adds a group's partials in item
+// order.  The waves of a workgroup take contiguous eighths of the item and add their products into
+// the LDS locals in a fixed turn order (round r: wave 0, 1, .., 7, a wave without a product passing
+// its turn), so every sum has a fixed order and the result is deterministic.  A product's FP32
+// result for one column depends on that column alone.  Unclassed pairs get classes of their own.
 // Operand layout (16x16x4): lane l = (j, h) with j = l%16, h = l/16 holds A[j][h] and B[h][j];
-// the result D[4h + v][j], v = 0..3.  Column j of the result depends on column j of B alone.
+// the result D[4h + v][j], v = 0..3.
 typedef float f32x4v __attribute__((ext_vector_type(4)));
-constexpr int kM2LGroupsPerWave = 16;  // groups of 16 class-sorted pairs per wave
+constexpr int kM2LGroupTargets = 256;  // target cells per group (consecutive BFS ids)
+constexpr int kM2LGroupBits = 8;
+constexpr int kM2LMfmaMaxOrder = 5;  // orders of the matrix-core M2L (nm <= 56)
+constexpr int kM2LWavesP4 = 12;     // waves per workgroup of the product kernel (orders <= 4)
+constexpr int kM2LWavesP5 = 8;      // (order 5: more registers per wave)
 
 // FP32 signed multipoles in the B-operand order of the products: cell S's row of 4 nkp floats holds
 // (-1)^|al| M_al at h * nkp + ks for al = 4 ks + h (0 past nm); nkp is the k-step count rounded up
@@ -885,217 +902,395 @@ __global__ void MultipoleToF32Kernel(int nn, int nm, int nkp, const double* __re
   M32[k] = al < nm ? (float)(sign[al] * M[S * nm + al]) : 0.f;
 }
 
-// Pairs grouped by class (unclassed pairs last, as class n_cls): the class alone is sorted on (a
-// stable sort of its few low bits); the source cell rides in the key's upper half, so the product
-// kernel reads it with the class instead of gathering it from the list; the pair index is the value.
-// pair_slot holds table slots (kNoClass for unclassed pairs); ids maps a slot to its class id.
-__global__ void PairClassKeyKernel(int64_t np, const uint32_t* __restrict__ pair_slot,
-                                   const uint32_t* __restrict__ ids, uint32_t n_cls,
-                                   const uint64_t* __restrict__ list, uint64_t* __restrict__ keys,
-                                   uint32_t* __restrict__ vals) {
+// Unclassed pairs (pair_slot kNoClass) get class ids n_cls + u, u their rank in list order.
+__global__ void UnclassedFlagKernel(int64_t np, const uint32_t* __restrict__ pair_slot, uint32_t* __restrict__ flag) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) flag[q] = pair_slot[q] == kNoClass ? 1u : 0u;
+}
+
+__global__ void UnclassedListKernel(int64_t np, const uint32_t* __restrict__ pair_slot,
+                                    const uint32_t* __restrict__ rank, uint32_t* __restrict__ ulist) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np && pair_slot[q] == kNoClass) ulist[rank[q]] = (uint32_t)q;
+}
+
+// One wave per unclassed pair: the FP32 tensor of its own displacement.
+__global__ __launch_bounds__(64) void UnclassedTensorKernel(int n_uncl, const uint32_t* __restrict__ ulist,
+                                                            const uint64_t* __restrict__ list,
+                                                            const Cell* __restrict__ cells, FmmTab t,
+                                                            float* __restrict__ Dc32) {
+  extern __shared__ double lds[];
+  const int u = blockIdx.x;
+  if (u >= n_uncl) return;
+  const uint64_t pk = list[ulist[u]];
+  const Cell& T = cells[(int)(pk >> 32)];
+  const Cell& S = cells[(uint32_t)pk];
+  const double xs[3] = {T.cx - S.cx, T.cy - S.cy, T.cz - S.cz};
+  DerivTensorWave(xs, t, t.rec, lds);
+  for (int k = threadIdx.x; k < t.nd; k += 64) Dc32[(size_t)u * t.nd + k] = (float)lds[k];
+}
+
+// Bit layout of the sorted pair keys: (group, class, target within the group) above the source
+// cell when all fit in 64 bits ("packed", the source then rides in the key), else (group, class,
+// target) with the source as the sort value.
+struct M2LKeyLayout {
+  int packed;
+  int sbits, cls_shift, gshift;
+  uint64_t smask, cls_mask;
+};
+
+__global__ void M2LGroupKeyKernel(int64_t np, const uint64_t* __restrict__ list, const uint32_t* __restrict__ pre,
+                                  const uint32_t* __restrict__ pair_slot, const uint32_t* __restrict__ ids,
+                                  const uint32_t* __restrict__ urank, uint32_t n_cls, M2LKeyLayout kl,
+                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
+  const uint64_t pk = list[q];
+  const uint32_t S = (uint32_t)pk;
+  const uint32_t bfs = pre[(uint32_t)(pk >> 32)];
   const uint32_t slot = pair_slot[q];
-  const uint32_t c = slot == kNoClass ? n_cls : ids[slot];
-  keys[q] = ((uint64_t)(uint32_t)list[q] << 32) | c;
-  vals[q] = (uint32_t)q;
+  const uint64_t cls = slot == kNoClass ? (uint64_t)n_cls + urank[q] : (uint64_t)ids[slot];
+  const uint64_t g = bfs >> kM2LGroupBits, loc = bfs & (kM2LGroupTargets - 1);
+  const uint64_t hi = (g << kl.gshift) | (cls << kl.cls_shift) | (loc << (kl.cls_shift - kM2LGroupBits));
+  if (kl.packed) {
+    keys[q] = hi | S;
+  } else {
+    keys[q] = hi;
+    vals[q] = S;
+  }
 }
 
-// LDS bytes of M2LMfmaKernel<NRB, NKS>: per wave Dd[nd] (doubles) and Df[nd + 1] (floats); the
-// per-lane fragment index table [NRB NKS][64]; the recurrence table [nd][10].
-static size_t M2LMfmaLds(int nd, int nrb, int nks) {
-  return sizeof(double) * (size_t)4 * (nd + (nd + 2) / 2) + sizeof(int) * ((size_t)nrb * nks * 64 + (size_t)nd * 10);
+// goff[g] = first sorted position of group g (g = 0 .. n_groups).
+__global__ void GroupOffsetsKernel(int n_groups, const uint64_t* __restrict__ keys, int64_t np, int gshift,
+                                   int64_t* __restrict__ goff) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > n_groups) return;
+  int64_t lo = 0, hi = np;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((keys[mid] >> gshift) < (uint64_t)g) lo = mid + 1;
+    else hi = mid;
+  }
+  goff[g] = lo;
 }
 
-template <int NRB, int NKS>
-__global__ __launch_bounds__(256) void M2LMfmaKernel(int64_t np, const uint64_t* __restrict__ skeys,
-                                                     const uint32_t* __restrict__ svals,
-                                                     const uint64_t* __restrict__ list, uint32_t n_cls,
-                                                     const float* __restrict__ Dc32, const Cell* __restrict__ cells,
-                                                     FmmTab t, const float* __restrict__ M32,
-                                                     float* __restrict__ out) {
+struct M2LItem {
+  int64_t b, e;  // sorted positions
+  int g, pad;    // group
+};
+
+// ---- exact, order-independent accumulation of the M2L products -------------------------------------
+// A product's FP32 result x for coefficient gamma of a target (|gamma| = n) is bounded by
+//     b_n = sum_a Dk_c(n + a) Ma_S(a),
+// Dk_c(k) = max |D| over the class tensor's entries of degree k, Ma_S(a) = sum over |alpha| = a of
+// |M'_alpha(S)| (the FP32 multipoles).  With B = 2 n_prod(group) max_pairs(b_n) per (group, n) --
+// n_prod, the group's product count, bounds any target's pair count, and 2 covers the FP32 rounding
+// of the product -- every x is rounded to the grid 2^eg with 2^(eg + 52) > B: (x + C) - C with
+// C = 1.5 2^(eg + 52).  Sums of grid multiples below 2^(eg + 53) are exact in FP64, so the atomic
+// LDS adds, and the reduction over a group's items, give the same bits in any order.  The rounding
+// error, at most 2^(eg - 1) = B 2^-53 per product, is far below the FP32 error of the product.
+
+// Ma[S][a] (a = 0 .. P): the per-degree L1 norms of each cell's FP32 multipoles.
+__global__ void CellNormKernel(int nn, FmmTab t, const double* __restrict__ M, float* __restrict__ ma) {
+  const int S = blockIdx.x * blockDim.x + threadIdx.x;
+  if (S >= nn) return;
+  for (int a = 0; a <= t.P; ++a) {
+    float s = 0.f;
+    for (int k = t.deg_start[a]; k < t.deg_start[a + 1]; ++k) s += fabsf((float)M[(size_t)S * t.nm + k]);
+    ma[(size_t)S * (t.P + 1) + a] = s;
+  }
+}
+
+// Dk[c][k] (k = 0 .. 2P) for every class tensor (one wave per class).
+__global__ __launch_bounds__(64) void ClassDegMaxKernel(int n_tens, const float* __restrict__ Dc32, FmmTab t,
+                                                        float* __restrict__ dk) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  if (c >= n_tens) return;
+  for (int d = 0; d <= 2 * t.P; ++d) {
+    float m = 0.f;
+    for (int k = t.deg_start[d] + lane; k < t.deg_start[d + 1]; k += 64) m = fmaxf(m, fabsf(Dc32[(size_t)c * t.nd + k]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (lane == 0) dk[(size_t)c * (2 * t.P + 1) + d] = m;
+  }
+}
+
+// Workgroup b's item: consecutive items go to one XCD (workgroups are dealt to the 8 XCDs in turn),
+// so each XCD works through a contiguous range of groups and its L2 holds their sources.
+__device__ inline int XcdItem(int b, int n) {
+  const int q = n / 8, r = n % 8, x = b % 8, i = b / 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Slice w of W of an item: the sorted positions that wave w of the item's workgroup takes.
+__device__ inline void M2LSlice(const M2LItem& it, int w, int W, int64_t& sb, int64_t& se) {
+  const int64_t span = it.e - it.b;
+  sb = it.b + span * w / W;
+  se = it.b + span * (w + 1) / W;
+}
+
+// Products of a slice: its sorted pairs cut into runs of one class, each cut into pieces of at most
+// 16 (a product has one pair per column).  One wave per slice, 64 positions per step.  Counting
+// pass (start == nullptr): the slice's product count into cnt[s].  Writing pass: product p (numbered
+// from off[s]) gets start[p] (its first position), cls[p] and col[16 p + c] = (source << 8 | target
+// within the group) for its column c; start[off[s + 1]] = the slice's end, so every product's
+// length is start[p + 1] - start[p].
+// The writing pass also takes the group's bound max_pairs(b_n) into gmax[g][n] (float bits, atomic
+// max: the same in any order).
+__global__ __launch_bounds__(64) void M2LProductKernel(int n_slices, int W, const M2LItem* __restrict__ items,
+                                                       const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals, M2LKeyLayout kl,
+                                                       const uint32_t* __restrict__ off, uint32_t* __restrict__ cnt,
+                                                       uint32_t* __restrict__ start, uint32_t* __restrict__ cls,
+                                                       uint32_t* __restrict__ col, int P,
+                                                       const float* __restrict__ ma,
+                                                       const float* __restrict__ dk, uint32_t* __restrict__ gmax) {
+  const int s = blockIdx.x, lane = threadIdx.x;
+  if (s >= n_slices) return;
+  int64_t sb, se;
+  const M2LItem itm = items[s / W];
+  M2LSlice(itm, s % W, W, sb, se);
+  float bmax[kM2LMfmaMaxOrder + 1];  // (indices are compile-time: the loops are unrolled)
+#pragma unroll
+  for (int n = 0; n <= kM2LMfmaMaxOrder; ++n) bmax[n] = 0.f;
+  const bool write = start != nullptr;
+  uint32_t np = write ? off[s] : 0u;  // products before the window
+  int64_t rs = sb, hp = sb;           // start of the run and of the product that the window continues
+  uint32_t prev = 0xffffffffu;        // class of the position before the window
+  const uint64_t incl = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  for (int64_t wb = sb; wb < se; wb += 64) {
+    const int64_t e = wb + lane;
+    const bool valid = e < se;
+    const uint64_t k = valid ? keys[e] : 0ull;
+    const uint32_t c = (uint32_t)((k >> kl.cls_shift) & kl.cls_mask);
+    uint32_t pc = (uint32_t)__shfl_up((int)c, 1);
+    if (lane == 0) pc = prev;
+    const uint64_t runs = __ballot(valid && (e == sb || c != pc));
+    const uint64_t rb = runs & incl;
+    const int64_t myrs = rb ? wb + 63 - __builtin_clzll(rb) : rs;
+    const bool head = valid && ((e - myrs) & 15) == 0;
+    const uint64_t heads = __ballot(head);
+    const uint64_t hb = heads & incl;
+    if (write && valid) {
+      const int64_t myhp = hb ? wb + 63 - __builtin_clzll(hb) : hp;
+      const uint32_t p = np + (uint32_t)__popcll(hb) - 1u;  // hb == 0: the product of the previous window
+      const uint32_t S = kl.packed ? (uint32_t)(k & kl.smask) : vals[e];
+      const uint32_t loc = (uint32_t)((k >> (kl.cls_shift - kM2LGroupBits)) & (kM2LGroupTargets - 1));
+      col[16 * (size_t)p + (size_t)(e - myhp)] = (S << 8) | loc;
+      const float* dkc = dk + (size_t)c * (2 * P + 1);
+      const float* mas = ma + (size_t)S * (P + 1);
+      float dv[2 * kM2LMfmaMaxOrder + 1], mv[kM2LMfmaMaxOrder + 1];
+#pragma unroll
+      for (int d = 0; d <= 2 * kM2LMfmaMaxOrder; ++d) dv[d] = d <= 2 * P ? dkc[d] : 0.f;
+#pragma unroll
+      for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) mv[a] = a <= P ? mas[a] : 0.f;
+#pragma unroll
+      for (int n = 0; n <= kM2LMfmaMaxOrder; ++n) {
+        float b = 0.f;
+#pragma unroll
+        for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) b += dv[n + a] * mv[a];
+        bmax[n] = fmaxf(bmax[n], b);
+      }
+      if (head) {
+        start[p] = (uint32_t)e;
+        cls[p] = c;
+      }
+      hp = __shfl(myhp, 63);
+    }
+    np += (uint32_t)__popcll(heads);
+    rs = (int64_t)__shfl(myrs, 63);  // (lanes past se keep the run start of the last valid lane)
+    prev = __shfl(c, 63);
+  }
+  if (lane == 0) {
+    if (write) start[off[s + 1]] = (uint32_t)se;
+    else cnt[s] = np;
+  }
+  if (write) {
+#pragma unroll
+    for (int n = 0; n <= kM2LMfmaMaxOrder; ++n) {
+      float m = bmax[n];
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if (lane == 0 && n <= P && m > 0.f) atomicMax(&gmax[(size_t)itm.g * (P + 1) + n], __float_as_uint(m));
+    }
+  }
+}
+
+// LDS bytes of M2LGroupKernel<NRB, NKS, NDL, W>: the group's locals [G][nm | 1] (doubles; an odd
+// row stride spreads the 16 targets of a product's adds over the LDS banks), the
+// per-lane fragment index table [NRB NKS][64], per wave the FP32 class tensor Df[nd + 1] (the
+// whole rounded up to 8 bytes), the grid constants [nm] (doubles).
+static size_t M2LGroupLds(int nm, int nd, int nrb, int nks, int waves) {
+  return sizeof(double) * (size_t)kM2LGroupTargets * (nm | 1) + sizeof(int) * (size_t)nrb * nks * 64 +
+         sizeof(float) * (size_t)((waves * (nd + 1) + 1) & ~1) + sizeof(double) * (size_t)nm;
+}
+
+__global__ void M2LZeroKernel(int64_t n, uint32_t* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+template <int NRB, int NKS, int NDL, int W>
+__global__ __launch_bounds__(64 * W) void M2LGroupKernel(
+    const M2LItem* __restrict__ items, const int* __restrict__ gitem, const uint32_t* __restrict__ slice_off,
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ pcls, const uint32_t* __restrict__ pcol,
+    const float* __restrict__ Dc32, FmmTab t, const float* __restrict__ M32, const uint32_t* __restrict__ gmax,
+    double* __restrict__ partial) {
   constexpr int NKP = (NKS + 3) & ~3;
+  constexpr int G = kM2LGroupTargets;
   extern __shared__ double lds[];
   const int nm = t.nm, nd = t.nd;
+  double* acc = lds;
+  const int as = nm | 1;  // row stride of the locals
+  int* s_aidx = reinterpret_cast<int*>(acc + G * as);
+  float* s_df = reinterpret_cast<float*>(s_aidx + NRB * NKS * 64);
+  double* s_grid = reinterpret_cast<double*>(s_df + ((W * (nd + 1) + 1) & ~1));  // C per coefficient
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slice = nd + (nd + 2) / 2;
-  int* s_aidx = (int*)(lds + 4 * slice);
-  int* s_rec = s_aidx + NRB * NKS * 64;
+  const int j = lane & 15, h = lane >> 4;
+  const int item = XcdItem(blockIdx.x, gridDim.x);
+  for (int e = threadIdx.x; e < G * as; e += 64 * W) acc[e] = 0.0;
+  {
+    const int grp = items[item].g;
+    const double n_prod = (double)(slice_off[gitem[grp + 1] * W] - slice_off[gitem[grp] * W]);
+    for (int g = threadIdx.x; g < nm; g += 64 * W) {
+      const int n = t.multi[3 * g] + t.multi[3 * g + 1] + t.multi[3 * g + 2];
+      const double B = 2.0 * n_prod * (double)__uint_as_float(gmax[(size_t)grp * (t.P + 1) + n]);
+      s_grid[g] = B > 0.0 ? 1.5 * ldexp(1.0, ilogb(B) + 1) : 0.0;  // 1.5 2^(eg + 52), eg = ilogb(B) + 1 - 52
+    }
+  }
   // Fragment (rb, ks) of lane (j, h) is D[m2l[16 rb + j][4 ks + h]]; outside the nm x nm matrix it
   // is Df[nd], which holds 0.
-  for (int e = threadIdx.x; e < NRB * NKS * 64; e += 256) {
+  for (int e = threadIdx.x; e < NRB * NKS * 64; e += 64 * W) {
     const int rk = e >> 6, l = e & 63, rb = rk / NKS, ks = rk - rb * NKS;
     const int g = rb * 16 + (l & 15), al = ks * 4 + (l >> 4);
     s_aidx[e] = (g < nm && al < nm) ? t.m2l[g * nm + al] : nd;
   }
-  for (int k = threadIdx.x; k < nd * 10; k += 256) s_rec[k] = t.rec[k];
-  double* Dd = lds + wave * slice;
-  float* Df = reinterpret_cast<float*>(Dd + nd);
+  float* Df = s_df + wave * (nd + 1);
   if (lane == 0) Df[nd] = 0.f;
-  __syncthreads();  // the only workgroup barrier
-  const int j = lane & 15, h = lane >> 4;
-  const int64_t n_groups = (np + 15) / 16;
-  const int64_t g0 = ((int64_t)blockIdx.x * 4 + wave) * kM2LGroupsPerWave;
-  const int64_t g1 = min(n_groups, g0 + kM2LGroupsPerWave);
-  const int nm4 = (nm + 3) & ~3;  // floats per result row
-  float afrag[NRB][NKS];
-  uint32_t cur = 0xffffffffu;
-  // the group's column j: sorted position r -> key (source cell << 32 | class) and pair q; an
-  // empty column has key ~0
-  auto column = [&](int64_t grp, uint64_t& kv, uint32_t& q) {
-    const int64_t r = grp * 16 + j;
-    const bool valid = grp < g1 && r < np;
-    kv = valid ? skeys[r] : ~0ull;
-    q = valid ? svals[r] : 0u;
+  __syncthreads();
+  // this wave's products
+  const int slice = item * W + wave;
+  const int pb = (int)slice_off[slice], pe = (int)slice_off[slice + 1];
+  // A product's columns into the locals: rounded to the grid, then LDS atomic adds, exact and so
+  // independent of the order in which the waves make them.
+  auto add_columns = [&](const f32x4v (&a)[NRB], int loc, bool present) {
+    double* row = acc + loc * as;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int g = 16 * rb + 4 * h + v;
+        if (present && g < nm) {
+          const double C = s_grid[g];
+          const double x = ((double)a[rb][v] + C) - C;
+          __hip_atomic_fetch_add(row + g, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
   };
-  // the lane's multipole values of the column's source (cell 0 for an empty column: its result
-  // column is never stored)
-  // Exactly the NKS values are loaded (16-byte loads, then single floats): a padded load would keep
-  // its unused destination registers pending, and the compiler would wait for the load before
-  // reusing them.
-  auto row = [&](uint64_t kv, float (&bv)[NKS]) {
-    const uint32_t S = kv == ~0ull ? 0u : (uint32_t)(kv >> 32);
+  // Software pipeline over three slots, the stage written out three times with the slots rotated so
+  // that no register a load may still be writing is copied.  At stage k: product k+2's table entry
+  // (start, next start, class, column word) is loaded; product k+1's multipole rows and class tensor
+  // are loaded from its entry (loaded a stage earlier); product k runs on the matrix cores; product
+  // k-1's result is added.  Every stage issues the same loads in the same order (positions past the
+  // wave's last product are clamped to it), so the compiler's in-order waits are exact.
+  struct Slot {
+    uint32_t st0, st1, cl, cw;  // table entry: start, next start, class, column word
+    float bv[NKS], dv[NDL];
+    f32x4v acc[NRB];
+  };
+  auto load_entry = [&](int k, Slot& s) {
+    const int kk = k < pe ? k : pe - 1;
+    s.st0 = pstart[kk];
+    s.st1 = pstart[kk + 1];
+    s.cl = pcls[kk];
+    s.cw = pcol[16 * (size_t)kk + j];
+  };
+  auto load_operands = [&](Slot& s) {  // the lane's multipole values of column j's source, the tensor
+    const bool present = j < (int)(s.st1 - s.st0);
+    const uint32_t S = present ? (s.cw >> 8) : 0u;  // cell 0 for an empty column (never added)
     const float* mrow = M32 + (size_t)S * (4 * NKP) + h * NKP;
 #pragma unroll
     for (int i = 0; i < NKS / 4; ++i) {
       const f32x4v v = reinterpret_cast<const f32x4v*>(mrow)[i];
-      bv[4 * i] = v[0];
-      bv[4 * i + 1] = v[1];
-      bv[4 * i + 2] = v[2];
-      bv[4 * i + 3] = v[3];
+      s.bv[4 * i] = v[0];
+      s.bv[4 * i + 1] = v[1];
+      s.bv[4 * i + 2] = v[2];
+      s.bv[4 * i + 3] = v[3];
     }
 #pragma unroll
-    for (int i = (NKS / 4) * 4; i < NKS; ++i) bv[i] = mrow[i];
+    for (int i = (NKS / 4) * 4; i < NKS; ++i) s.bv[i] = mrow[i];
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)s.cl);
+#pragma unroll
+    for (int i = 0; i < NDL; ++i) {
+      const int k = lane + 64 * i;  // values past nd are loaded from nd - 1 and never used
+      s.dv[i] = Dc32[(size_t)c * nd + (k < nd ? k : nd - 1)];
+    }
   };
-  uint64_t kv0, kv1;
-  uint32_t q0, q1;
-  float bv0[NKS];
-  column(g0, kv0, q0);
-  column(g0 + 1, kv1, q1);
-  row(kv0, bv0);
-  for (int64_t grp = g0; grp < g1; ++grp) {
-    float bv1[NKS];
-    row(kv1, bv1);  // the next group's multipoles and the column after it are in flight meanwhile
-    uint64_t kv2;
-    uint32_t q2;
-    column(grp + 2, kv2, q2);
-    const uint32_t cls = (uint32_t)kv0;
-    uint64_t pending = __ballot(kv0 != ~0ull);  // uniform
-    while (pending) {  // one class at a time (one pair at a time for unclassed pairs)
-      const int l0 = __builtin_ctzll(pending);
-      const uint32_t c = __builtin_amdgcn_readlane(cls, l0);
-      bool sel = ((pending >> lane) & 1ull) && cls == c;
-      bool rebuild = true;
-      if (c == n_cls) {  // unclassed: the pair's own tensor
-        sel = sel && j == (l0 & 15);
-        const uint32_t Sj = __builtin_amdgcn_readlane((uint32_t)(kv0 >> 32), l0);
-        const uint32_t qj = __builtin_amdgcn_readlane(q0, l0);
-        const Cell& T = cells[(int)(list[qj] >> 32)];
-        const Cell& Sc = cells[Sj];
-        const double xs[3] = {T.cx - Sc.cx, T.cy - Sc.cy, T.cz - Sc.cz};
-        DerivTensorWave(xs, t, s_rec, Dd);
-        for (int k = lane; k < nd; k += 64) Df[k] = (float)Dd[k];
-        cur = 0xffffffffu;
-      } else if (c != cur) {
-        WaveSync();
-        for (int k = lane; k < nd; k += 64) Df[k] = Dc32[(size_t)c * nd + k];
-        cur = c;
-      } else {
-        rebuild = false;
-      }
-      if (rebuild) {
-        WaveSync();
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int ks = 0; ks < NKS; ++ks) afrag[rb][ks] = Df[s_aidx[(rb * NKS + ks) * 64 + lane]];
-        WaveSync();
-      }
-      f32x4v acc[NRB];
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < NKS; ++ks)
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], bv0[ks], acc[rb], 0, 0, 0);
-      // Lane (j, h) holds the four consecutive coefficients 16 rb + 4h .. + 3 of pair j: one
-      // 16-byte write into the pair's row.  Rows are stored in the class-sorted order of the
-      // pairs, so a group's rows are contiguous.
-      if (sel) {
-        float* orow = out + (size_t)(grp * 16 + j) * nm4;
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb) {
-          const int g = rb * 16 + 4 * h;
-          if (g < nm4) *reinterpret_cast<f32x4v*>(orow + g) = acc[rb];
-        }
-      }
-      pending &= ~__ballot(sel);
-    }
-    kv0 = kv1;
-    q0 = q1;
-#pragma unroll
-    for (int i = 0; i < NKS; ++i) bv0[i] = bv1[i];
-    kv1 = kv2;
-    q1 = q2;
+  float afrag[NRB][NKS];
+  uint32_t cur = 0xffffffffu;
+  int locB = 0;
+  bool presB = false, pending = false;
+  Slot s0, s1, s2;
+  if (pb < pe) {
+    load_entry(pb, s0);
+    load_entry(pb + 1, s1);
+    load_operands(s0);
   }
-}
-
-// Where the pair at each position of the target-grouped list has its result row: pos[vals[r]] = r,
-// vals the class-sorted pairs' list positions.
-__global__ void RowPositionKernel(int64_t np, const uint32_t* __restrict__ vals, uint32_t* __restrict__ pos) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < np) pos[vals[r]] = (uint32_t)r;
-}
-
-// L of each target: its pairs' result rows summed in FP64, times 1/gamma!.  out holds a row of nm4
-// floats per pair (nm4 a multiple of 4, at most 64), the pair at list position q in row pos[q].
-// Lane l reads the 16-byte chunk l % nch (nch = nm4 / 4) of the rows of list positions
-// b + l / nch, + rpi, + 2 rpi, ... (rpi = 64 / nch rows per load instruction), and the rpi partial
-// sums of a chunk are then added in row-group order: the order is fixed, so the sum is
-// deterministic.
-__global__ __launch_bounds__(64) void M2LSumKernel(int nn, const int64_t* __restrict__ off, int nm, int nm4,
-                                                   const float* __restrict__ out, const uint32_t* __restrict__ pos,
-                                                   const double* __restrict__ inv_fact, double* __restrict__ L) {
-  __shared__ double part[64][4];
-  const int p = blockIdx.x, lane = threadIdx.x;
-  const int64_t b = off[p], e = off[p + 1];
-  const int nch = nm4 / 4, rpi = 64 / nch;
-  const int r0 = lane / nch, ch = lane - r0 * nch;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  if (r0 < rpi) {
-    int64_t q = b + r0;
-    for (; q + 3 * (int64_t)rpi < e; q += 4 * (int64_t)rpi) {
-      f32x4v v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const f32x4v*>(out + (size_t)pos[q + u * rpi] * nm4 + 4 * ch);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a0 += (double)v[u][0];
-        a1 += (double)v[u][1];
-        a2 += (double)v[u][2];
-        a3 += (double)v[u][3];
-      }
+  int k = pb;
+  auto stage = [&](Slot& c, Slot& n, Slot& f) -> bool {
+    if (k >= pe) {  // no product k: the last one (k-1, in f) is added and the wave is done
+      if (pending) add_columns(f.acc, locB, presB);
+      pending = false;
+      return false;
     }
-    for (; q < e; q += rpi) {
-      const f32x4v v = *reinterpret_cast<const f32x4v*>(out + (size_t)pos[q] * nm4 + 4 * ch);
-      a0 += (double)v[0];
-      a1 += (double)v[1];
-      a2 += (double)v[2];
-      a3 += (double)v[3];
+    load_entry(k + 2, f);
+    load_operands(n);
+    const uint32_t ccls = (uint32_t)__builtin_amdgcn_readfirstlane((int)c.cl);
+    if (ccls != cur) {  // the class's fragments
+      WaveSync();
+#pragma unroll
+      for (int i = 0; i < NDL; ++i)
+        if (lane + 64 * i < nd) Df[lane + 64 * i] = c.dv[i];
+      WaveSync();
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) afrag[rb][ks] = Df[s_aidx[(rb * NKS + ks) * 64 + lane]];
+      cur = ccls;
     }
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) c.acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+        c.acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], c.bv[ks], c.acc[rb], 0, 0, 0);
+    if (pending) add_columns(f.acc, locB, presB);  // product k-1's columns into the locals
+    locB = (int)(c.cw & (G - 1));
+    presB = j < (int)(c.st1 - c.st0);
+    pending = true;
+    ++k;
+    return true;
+  };
+  while (stage(s0, s1, s2) && stage(s1, s2, s0) && stage(s2, s0, s1)) {
   }
-  part[lane][0] = a0;
-  part[lane][1] = a1;
-  part[lane][2] = a2;
-  part[lane][3] = a3;
   __syncthreads();
-  if (lane < nm4) {
-    const int c = lane >> 2, j = lane & 3, g = lane;
-    double sum = 0.0;
-    for (int k = 0; k < rpi; ++k) sum += part[c + k * nch][j];
-    if (g < nm) L[(size_t)p * nm + g] = sum * inv_fact[g];
+  double* out = partial + (size_t)item * G * nm;
+  for (int e = threadIdx.x; e < G * nm; e += 64 * W) {
+    const int loc = e / nm;
+    out[e] = acc[loc * as + (e - loc * nm)];
   }
+}
+
+// L of each target cell: its group's item partials added in item order, times 1/gamma!.
+__global__ void M2LPartialReduceKernel(int nn, int nm, const int* __restrict__ gitem,
+                                       const double* __restrict__ partial, const double* __restrict__ inv_fact,
+                                       const uint32_t* __restrict__ bfs2pre, double* __restrict__ L) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nn * nm) return;
+  const int bfs = (int)(idx / nm), gm = (int)(idx - (int64_t)bfs * nm);
+  const int g = bfs >> kM2LGroupBits, loc = bfs & (kM2LGroupTargets - 1);
+  double s = 0.0;
+  for (int it = gitem[g]; it < gitem[g + 1]; ++it) s += partial[((size_t)it * kM2LGroupTargets + loc) * nm + gm];
+  L[(size_t)bfs2pre[bfs] * nm + gm] = s * inv_fact[gm];
 }
 
 // L2L (fast-multipole.cc:557-578) for the parents of one BFS level.
@@ -1639,41 +1834,61 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     // keep the scratch names pointing at the right buffers
     std::swap(ctx->scratch["fmm_front_a"], ctx->scratch["fmm_front_b"]);
   }
-  void *off_m2l, *off_p2p;
-  // group both lists by target cell: a stable sort of the (T << 32 | S) pairs on the cb bits of T
+  // Both lists are grouped by target cell for the kernels that walk them per target: a stable sort of
+  // the (T << 32 | S) pairs on the cb bits of T.  The MFMA M2L sorts its pairs by (target group,
+  // class) instead, straight from the traversal's order.
+  const bool mfma_plan = precision == AM_FP32 && want_forces && tab.nm <= 56;
   int cb = 1;
   while ((1 << cb) < nn) ++cb;
-  if ((rc = RadixSortKeys(ctx, (uint64_t*)m2l, n_m2l, 32, cb))) return rc;
+  void *off_m2l = nullptr, *off_p2p;
+  bool m2l_sorted = false;
+  auto sort_m2l = [&]() -> int {
+    int r;
+    if ((r = RadixSortKeys(ctx, (uint64_t*)m2l, n_m2l, 32, cb))) return r;
+    if ((r = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return r;
+    hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)m2l,
+                       n_m2l, (int64_t*)off_m2l);
+    AM_HIP(hipGetLastError());
+    m2l_sorted = true;
+    return AM_OK;
+  };
+  if (!mfma_plan && (rc = sort_m2l())) return rc;
   if ((rc = RadixSortKeys(ctx, (uint64_t*)p2p, n_p2p, 32, cb))) return rc;
-  if ((rc = Scratch(ctx, "fmm_off_m2l", (size_t)(nn + 1) * 8, &off_m2l))) return rc;
   if ((rc = Scratch(ctx, "fmm_off_p2p", (size_t)(nn + 1) * 8, &off_p2p))) return rc;
-  hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)m2l, n_m2l,
-                     (int64_t*)off_m2l);
   hipLaunchKernelGGL(PairOffsetsKernel, dim3(Blocks(nn + 1, 256)), dim3(256), 0, st, nn, (const uint64_t*)p2p, n_p2p,
                      (int64_t*)off_p2p);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
 
+  uint32_t* m2l_counts = nullptr;  // the unclassed-pair count
   TimerBegin(ctx, "fmm_m2l", &slot);
   {
     // classes of the M2L pairs (table of at most 2^22 slots, load factor <= 1/2 for up to 2^21 keys)
     uint64_t tsize = 1024;
     while (tsize < 2 * (uint64_t)std::max<int64_t>(n_m2l, 1) && tsize < ((uint64_t)1 << 22)) tsize <<= 1;
-    void *hk, *pslot, *occ, *ids;
+    void *hk, *pslot, *occ, *ids, *cnt;
     if ((rc = Scratch(ctx, "fmm_cls_keys", tsize * 8, &hk))) return rc;
     if ((rc = Scratch(ctx, "fmm_pair_slot", (size_t)std::max<int64_t>(n_m2l, 1) * 4, &pslot))) return rc;
     if ((rc = Scratch(ctx, "fmm_cls_occ", tsize * 4, &occ))) return rc;
     if ((rc = Scratch(ctx, "fmm_cls_ids", (tsize + 1) * 4, &ids))) return rc;
-    hipLaunchKernelGGL(ClassClearKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize, (uint64_t*)hk);
-    if (n_m2l > 0)
-      hipLaunchKernelGGL(M2LClassInsertKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                         (const uint64_t*)m2l, (const double4*)geo, (uint64_t*)hk, tsize - 1, (uint32_t*)pslot);
+    if ((rc = Scratch(ctx, "fmm_m2l_counts", 4, &cnt))) return rc;
+    m2l_counts = (uint32_t*)cnt;
+    hipLaunchKernelGGL(ClassClearKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize, (uint64_t*)hk,
+                       m2l_counts);
+    auto insert = [&]() {
+      if (n_m2l > 0)
+        hipLaunchKernelGGL(M2LClassInsertKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                           (const uint64_t*)m2l, (const double4*)geo, (uint64_t*)hk, tsize - 1,
+                           ctx->fmm_classes ? kClassProbes : 0, (uint32_t*)pslot, m2l_counts);
+    };
+    insert();
     hipLaunchKernelGGL(ClassOccupiedKernel, dim3(Blocks((int64_t)tsize, 256)), dim3(256), 0, st, tsize,
                        (const uint64_t*)hk, (uint32_t*)occ);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, (const uint32_t*)occ, (uint32_t*)ids, (int64_t)tsize))) return rc;
-    uint32_t n_cls = 0, sort_err = 0;  // (the traversal's list sorts are checked here too)
+    uint32_t n_cls = 0, sort_err = 0, n_uncl = 0;  // (the traversal's list sorts are checked here too)
     AM_HIP(hipMemcpyAsync(&n_cls, (uint32_t*)ids + tsize, 4, hipMemcpyDeviceToHost, st));
+    AM_HIP(hipMemcpyAsync(&n_uncl, m2l_counts, 4, hipMemcpyDeviceToHost, st));
     if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
     if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
@@ -1685,68 +1900,176 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
     if (n_cls > 0)
       hipLaunchKernelGGL(ClassTensorKernel, dim3(n_cls), dim3(64), sizeof(double) * tab.nd, st, (int)n_cls,
                          (const uint64_t*)ckey, (const Cell*)cells, tab, (double*)Dc);
-    const size_t m2l_lds =
-        sizeof(double) * ((size_t)kM2LWaves * (tab.nd + tab.nm) + tab.nm) +
-        (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
-    const bool mfma = precision == AM_FP32 && want_forces && tab.nm <= 56 && n_m2l > 0;
-    if (n_m2l > 0 && !mfma)  // the FP64 contraction reads class ids; the MFMA path reads slots
-      hipLaunchKernelGGL(PairClassKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (const uint32_t*)pslot,
-                         (const uint32_t*)ids, (uint32_t*)pslot);  // in place: slot -> class id
-    const int dpl = (tab.nd + 63) / 64;
-    auto launch = [&](auto kernel) {
-      hipLaunchKernelGGL(kernel, dim3(Blocks(nn, kM2LWaves)), dim3(64 * kM2LWaves), m2l_lds, st, nn,
-                         (const Cell*)cells, (const uint64_t*)m2l, (const int64_t*)off_m2l, (const uint32_t*)pslot,
-                         (const double*)Dc, tab, (const double*)M, (double*)L);
-    };
+    AM_HIP(hipGetLastError());
+    // The MFMA path gives each unclassed pair a tensor of its own, up to a bound; past it (or with
+    // no pairs) the FP64 contraction runs.
+    const bool mfma = mfma_plan && n_m2l > 0 && (int64_t)n_uncl <= ctx->fmm_max_unclassed;
     if (mfma) {
-      void *Dc32, *M32, *ckeys, *cvals, *outp, *rowpos;
-      if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<uint32_t>(n_cls, 1) * tab.nd * 4, &Dc32))) return rc;
       const int nrb = tab.nm <= 36 ? 3 : 4, nks = tab.nm <= 36 ? 9 : 14;  // the instance below
       const int nkp = (nks + 3) & ~3;
-      const int nm4 = (tab.nm + 3) & ~3;
+      const int64_t n_tens = (int64_t)n_cls + n_uncl;
+      void *Dc32, *M32, *keys, *vals = nullptr, *urank = nullptr;
+      if ((rc = Scratch(ctx, "fmm_cls_tensor32", (size_t)std::max<int64_t>(n_tens, 1) * tab.nd * 4, &Dc32))) return rc;
       if ((rc = Scratch(ctx, "fmm_M32", (size_t)nn * 4 * nkp * 4, &M32))) return rc;
-      if ((rc = Scratch(ctx, "fmm_cls_sortkeys", (size_t)n_m2l * 8, &ckeys))) return rc;
-      if ((rc = Scratch(ctx, "fmm_cls_sortvals", (size_t)n_m2l * 4, &cvals))) return rc;
-      if ((rc = Scratch(ctx, "fmm_m2l_out", (size_t)n_m2l * nm4 * 4, &outp))) return rc;
-      if ((rc = Scratch(ctx, "fmm_m2l_rowpos", (size_t)n_m2l * 4, &rowpos))) return rc;
       const int64_t nt = (int64_t)n_cls * tab.nd;
       if (nt > 0)
         hipLaunchKernelGGL(TensorToF32Kernel, dim3(Blocks(nt, 256)), dim3(256), 0, st, nt, (const double*)Dc,
                            (float*)Dc32);
+      if (n_uncl > 0) {
+        void *flag, *ulist;
+        if ((rc = Scratch(ctx, "fmm_uncl_flag", (size_t)n_m2l * 4, &flag))) return rc;
+        if ((rc = Scratch(ctx, "fmm_uncl_rank", (size_t)(n_m2l + 1) * 4, &urank))) return rc;
+        if ((rc = Scratch(ctx, "fmm_uncl_list", (size_t)n_uncl * 4, &ulist))) return rc;
+        hipLaunchKernelGGL(UnclassedFlagKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                           (const uint32_t*)pslot, (uint32_t*)flag);
+        AM_HIP(hipGetLastError());
+        if ((rc = ExclusiveScan(ctx, (const uint32_t*)flag, (uint32_t*)urank, n_m2l))) return rc;
+        hipLaunchKernelGGL(UnclassedListKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
+                           (const uint32_t*)pslot, (const uint32_t*)urank, (uint32_t*)ulist);
+        hipLaunchKernelGGL(UnclassedTensorKernel, dim3(n_uncl), dim3(64), sizeof(double) * tab.nd, st, (int)n_uncl,
+                           (const uint32_t*)ulist, (const uint64_t*)m2l, (const Cell*)cells, tab,
+                           (float*)Dc32 + (size_t)n_cls * tab.nd);
+        AM_HIP(hipGetLastError());
+      }
       hipLaunchKernelGGL(MultipoleToF32Kernel, dim3(Blocks((int64_t)nn * 4 * nkp, 256)), dim3(256), 0, st, nn,
                          tab.nm, nkp, (const double*)M, tab.sign, (float*)M32);
-      hipLaunchKernelGGL(PairClassKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                         (const uint32_t*)pslot, (const uint32_t*)ids, n_cls, (const uint64_t*)m2l, (uint64_t*)ckeys,
-                         (uint32_t*)cvals);
+      // sort keys: (group, class, target within the group[, source])
+      const int n_groups = (nn + kM2LGroupTargets - 1) >> kM2LGroupBits;
+      int cbits = 1, gbits = 1;
+      while (((int64_t)1 << cbits) < n_tens) ++cbits;
+      while ((1 << gbits) < n_groups) ++gbits;
+      M2LKeyLayout kl;
+      kl.packed = cb + kM2LGroupBits + cbits + gbits <= 64 && n_m2l < ((int64_t)1 << 30) - 1;
+      kl.sbits = cb;
+      kl.smask = ((uint64_t)1 << cb) - 1;
+      kl.cls_shift = (kl.packed ? cb : 0) + kM2LGroupBits;
+      kl.gshift = kl.cls_shift + cbits;
+      kl.cls_mask = ((uint64_t)1 << cbits) - 1;
+      if ((rc = Scratch(ctx, "fmm_m2l_keys", (size_t)n_m2l * 8, &keys))) return rc;
+      if (!kl.packed && (rc = Scratch(ctx, "fmm_m2l_vals", (size_t)n_m2l * 4, &vals))) return rc;
+      hipLaunchKernelGGL(M2LGroupKeyKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (const uint64_t*)m2l,
+                         (const uint32_t*)o.pre, (const uint32_t*)pslot, (const uint32_t*)ids,
+                         (const uint32_t*)urank, n_cls, kl, (uint64_t*)keys, (uint32_t*)vals);
       AM_HIP(hipGetLastError());
-      int cls_bits = 1;
-      while (((uint64_t)1 << cls_bits) <= (uint64_t)n_cls) ++cls_bits;
-      if ((rc = RadixSortPairs(ctx, (uint64_t*)ckeys, (uint32_t*)cvals, n_m2l, cls_bits))) return rc;
-      const size_t lds_mf = M2LMfmaLds(tab.nd, nrb, nks);
-      const int64_t n_groups = (n_m2l + 15) / 16;
-      const int64_t waves = (n_groups + kM2LGroupsPerWave - 1) / kM2LGroupsPerWave;
-      auto mf = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(Blocks(waves, 4)), dim3(256), lds_mf, st, n_m2l, (const uint64_t*)ckeys,
-                           (const uint32_t*)cvals, (const uint64_t*)m2l, n_cls, (const float*)Dc32, (const Cell*)cells, tab,
-                           (const float*)M32, (float*)outp);
+      // a stable sort on (group, class): within a run the traversal's order is kept
+      if (kl.packed) rc = RadixSortKeys(ctx, (uint64_t*)keys, n_m2l, kl.cls_shift, cbits + gbits);
+      else rc = RadixSortPairs(ctx, (uint64_t*)keys, (uint32_t*)vals, n_m2l, kl.gshift + gbits);
+      if (rc) return rc;
+      void* goff_dev;
+      if ((rc = Scratch(ctx, "fmm_m2l_goff", (size_t)(n_groups + 1) * 8, &goff_dev))) return rc;
+      hipLaunchKernelGGL(GroupOffsetsKernel, dim3(Blocks(n_groups + 1, 256)), dim3(256), 0, st, n_groups,
+                         (const uint64_t*)keys, n_m2l, kl.gshift, (int64_t*)goff_dev);
+      AM_HIP(hipGetLastError());
+      std::vector<int64_t> goff(n_groups + 1);
+      AM_HIP(hipMemcpyAsync(goff.data(), goff_dev, goff.size() * 8, hipMemcpyDeviceToHost, st));
+      if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
+      AM_HIP(hipStreamSynchronize(st));
+      if ((rc = CheckSortErrorAfterSync(ctx, sort_err))) return rc;
+      // Items: each group's pair list in pieces of about n_m2l / (4 CUs) pairs, so that every CU
+      // takes several workgroups; every group has at least one item (its locals are written even
+      // when it has no pairs).
+      const int64_t piece = std::max<int64_t>(4096, n_m2l / (4 * std::max(ctx->num_cus, 1)));
+      std::vector<M2LItem> items;
+      std::vector<int> gitem(n_groups + 1);
+      for (int g = 0; g < n_groups; ++g) {
+        gitem[g] = (int)items.size();
+        const int64_t b = goff[g], c = goff[g + 1] - b, k = std::max<int64_t>(1, (c + piece - 1) / piece);
+        for (int64_t i = 0; i < k; ++i) items.push_back(M2LItem{b + c * i / k, b + c * (i + 1) / k, g, 0});
+      }
+      gitem[n_groups] = (int)items.size();
+      const int n_items = (int)items.size();
+      void *items_dev, *gitem_dev, *partial;
+      if ((rc = Scratch(ctx, "fmm_m2l_items", (size_t)n_items * sizeof(M2LItem), &items_dev))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_gitem", gitem.size() * 4, &gitem_dev))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_partial", (size_t)n_items * kM2LGroupTargets * tab.nm * 8, &partial))) return rc;
+      AM_HIP(hipMemcpyAsync(items_dev, items.data(), (size_t)n_items * sizeof(M2LItem), hipMemcpyHostToDevice, st));
+      AM_HIP(hipMemcpyAsync(gitem_dev, gitem.data(), gitem.size() * 4, hipMemcpyHostToDevice, st));
+      AM_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope below
+      // Products: each wave's slice of an item cut into runs of one class of at most 16 pairs,
+      // written as a table (a counting pass, a scan, a writing pass), so the product kernel reads
+      // its operands' addresses without a chain of dependent loads.
+      const int waves = tab.nm <= 36 ? kM2LWavesP4 : kM2LWavesP5;  // of the instance below
+      const int n_slices = n_items * waves;
+      // the bounds of the exact accumulation
+      void *mnorm, *dmax, *gmax;
+      if ((rc = Scratch(ctx, "fmm_m2l_mnorm", (size_t)nn * (tab.P + 1) * 4, &mnorm))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_dmax", (size_t)std::max<int64_t>(n_tens, 1) * (2 * tab.P + 1) * 4, &dmax)))
+        return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_gmax", (size_t)n_groups * (tab.P + 1) * 4, &gmax))) return rc;
+      hipLaunchKernelGGL(CellNormKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, tab, (const double*)M,
+                         (float*)mnorm);
+      hipLaunchKernelGGL(ClassDegMaxKernel, dim3((unsigned)n_tens), dim3(64), 0, st, (int)n_tens, (const float*)Dc32,
+                         tab, (float*)dmax);
+      hipLaunchKernelGGL(M2LZeroKernel, dim3(Blocks((int64_t)n_groups * (tab.P + 1), 256)), dim3(256), 0, st,
+                         (int64_t)n_groups * (tab.P + 1), (uint32_t*)gmax);
+      AM_HIP(hipGetLastError());
+      void *scnt, *soff, *pstart, *pcls, *pcol;
+      if ((rc = Scratch(ctx, "fmm_m2l_slice_cnt", (size_t)n_slices * 4, &scnt))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_slice_off", (size_t)(n_slices + 1) * 4, &soff))) return rc;
+      hipLaunchKernelGGL(M2LProductKernel, dim3(n_slices), dim3(64), 0, st, n_slices, waves,
+                         (const M2LItem*)items_dev, (const uint64_t*)keys, (const uint32_t*)vals, kl,
+                         (const uint32_t*)nullptr, (uint32_t*)scnt, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                         (uint32_t*)nullptr, tab.P, (const float*)mnorm, (const float*)dmax, (uint32_t*)gmax);
+      AM_HIP(hipGetLastError());
+      if ((rc = ExclusiveScan(ctx, (const uint32_t*)scnt, (uint32_t*)soff, n_slices))) return rc;
+      uint32_t n_prod = 0;
+      AM_HIP(hipMemcpyAsync(&n_prod, (uint32_t*)soff + n_slices, 4, hipMemcpyDeviceToHost, st));
+      AM_HIP(hipStreamSynchronize(st));
+      if ((rc = Scratch(ctx, "fmm_m2l_pstart", (size_t)(n_prod + 1) * 4, &pstart))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_pcls", (size_t)std::max<uint32_t>(n_prod, 1) * 4, &pcls))) return rc;
+      if ((rc = Scratch(ctx, "fmm_m2l_pcol", (size_t)std::max<uint32_t>(n_prod, 1) * 64, &pcol))) return rc;
+      hipLaunchKernelGGL(M2LProductKernel, dim3(n_slices), dim3(64), 0, st, n_slices, waves,
+                         (const M2LItem*)items_dev, (const uint64_t*)keys, (const uint32_t*)vals, kl,
+                         (const uint32_t*)soff, (uint32_t*)nullptr, (uint32_t*)pstart, (uint32_t*)pcls,
+                         (uint32_t*)pcol, tab.P, (const float*)mnorm, (const float*)dmax, (uint32_t*)gmax);
+      AM_HIP(hipGetLastError());
+      const size_t lds = M2LGroupLds(tab.nm, tab.nd, nrb, nks, waves);
+      auto launch_group = [&](auto kernel) -> int {
+        AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(kernel, dim3(n_items), dim3(64 * waves), lds, st, (const M2LItem*)items_dev,
+                           (const int*)gitem_dev, (const uint32_t*)soff, (const uint32_t*)pstart,
+                           (const uint32_t*)pcls, (const uint32_t*)pcol, (const float*)Dc32, tab, (const float*)M32,
+                           (const uint32_t*)gmax, (double*)partial);
+        AM_HIP(hipGetLastError());
+        return AM_OK;
       };
       int mslot;
       TimerBegin(ctx, "fmm_m2l_mfma", &mslot);
-      if (tab.nm <= 36) mf(M2LMfmaKernel<3, 9>);
-      else mf(M2LMfmaKernel<4, 14>);
+      if (tab.nm <= 36) rc = launch_group(M2LGroupKernel<3, 9, 3, kM2LWavesP4>);
+      else rc = launch_group(M2LGroupKernel<4, 14, 5, kM2LWavesP5>);
       TimerEnd(ctx, mslot);
-      if (n_m2l > 0)
-        hipLaunchKernelGGL(RowPositionKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l,
-                           (const uint32_t*)cvals, (uint32_t*)rowpos);
-      hipLaunchKernelGGL(M2LSumKernel, dim3(nn), dim3(64), 0, st, nn, (const int64_t*)off_m2l, tab.nm, nm4,
-                         (const float*)outp, (const uint32_t*)rowpos, tab.inv_fact, (double*)L);
+      if (rc) return rc;
+      hipLaunchKernelGGL(M2LPartialReduceKernel, dim3(Blocks((int64_t)nn * tab.nm, 256)), dim3(256), 0, st, nn,
+                         tab.nm, (const int*)gitem_dev, (const double*)partial, tab.inv_fact,
+                         (const uint32_t*)o.bfs2pre, (double*)L);
       AM_HIP(hipGetLastError());
-    } else if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
-    else if (dpl <= 3) launch(M2LKernel<3, false>);
-    else if (dpl <= 5) launch(M2LKernel<5, false>);
-    else if (dpl <= 8) launch(M2LKernel<8, false>);
-    else launch(M2LKernel<16, false>);
+      s->fmm_stats.m2l_items = n_items;
+    } else {
+      if (!m2l_sorted) {  // the list in target order, and its slots again in that order
+        if ((rc = sort_m2l())) return rc;
+        insert();
+      }
+      const size_t m2l_lds =
+          sizeof(double) * ((size_t)kM2LWaves * (tab.nd + tab.nm) + tab.nm) +
+          (tab.P <= kM2LLdsOrder ? sizeof(int) * ((size_t)tab.nm * tab.nm + (size_t)tab.nd * 10) : 0);
+      if (n_m2l > 0)
+        hipLaunchKernelGGL(PairClassKernel, dim3(Blocks(n_m2l, 256)), dim3(256), 0, st, n_m2l, (const uint32_t*)pslot,
+                           (const uint32_t*)ids, (uint32_t*)pslot);  // in place: slot -> class id
+      const int dpl = (tab.nd + 63) / 64;
+      auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(Blocks(nn, kM2LWaves)), dim3(64 * kM2LWaves), m2l_lds, st, nn,
+                           (const Cell*)cells, (const uint64_t*)m2l, (const int64_t*)off_m2l, (const uint32_t*)pslot,
+                           (const double*)Dc, tab, (const double*)M, (double*)L);
+      };
+      if (tab.nm <= kRegNm - 1 && dpl <= 3) launch(M2LKernel<3, true>);
+      else if (dpl <= 3) launch(M2LKernel<3, false>);
+      else if (dpl <= 5) launch(M2LKernel<5, false>);
+      else if (dpl <= 8) launch(M2LKernel<8, false>);
+      else launch(M2LKernel<16, false>);
+    }
     s->fmm_stats.m2l_classes = n_cls;
+    s->fmm_stats.m2l_unclassed = n_uncl;
   }
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
@@ -1779,7 +2102,7 @@ int FastMultipoleRun(am_store* s, int order, double theta, bool want_forces, dou
                        (const int64_t*)off_p2p, (uint32_t*)icnt);
     AM_HIP(hipGetLastError());
     if ((rc = ExclusiveScan(ctx, (const uint32_t*)icnt, (uint32_t*)ibase, nn))) return rc;
-    uint32_t n_items = 0, sort_err = 0;  // (the M2L class sort is checked here)
+    uint32_t n_items = 0, sort_err = 0;
     AM_HIP(hipMemcpyAsync(&n_items, (uint32_t*)ibase + nn, 4, hipMemcpyDeviceToHost, st));
     if (const uint32_t* w = SortErrorWord(ctx)) AM_HIP(hipMemcpyAsync(&sort_err, w, 4, hipMemcpyDeviceToHost, st));
     AM_HIP(hipStreamSynchronize(st));
@@ -1865,6 +2188,20 @@ int am_fmm_stats(am_store* s, int64_t* n_sources, int64_t* n_cells, int64_t* m2l
   if (p2p_cell_pairs) *p2p_cell_pairs = s->fmm_stats.p2p_cell_pairs;
   if (max_depth) *max_depth = s->fmm_stats.max_depth;
   if (m2l_classes) *m2l_classes = s->fmm_stats.m2l_classes;
+  return AM_OK;
+}
+
+int am_fmm_m2l_stats(am_store* s, int64_t* unclassed, int64_t* items) {
+  if (!s) return Fail(AM_ERR_INVALID, "am_fmm_m2l_stats: NULL store");
+  if (unclassed) *unclassed = s->fmm_stats.m2l_unclassed;
+  if (items) *items = s->fmm_stats.m2l_items;
+  return AM_OK;
+}
+
+int am_debug_fmm_m2l_options(am_ctx* ctx, int classes, int64_t max_unclassed) {
+  if (!ctx) return Fail(AM_ERR_INVALID, "am_debug_fmm_m2l_options: NULL context");
+  ctx->fmm_classes = classes != 0;
+  ctx->fmm_max_unclassed = max_unclassed;
   return AM_OK;
 }
 

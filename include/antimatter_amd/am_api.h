@@ -206,6 +206,10 @@ int am_compute_fast_multipole_coulomb_forces_ex(am_store* store, int order, doub
  * number of distinct M2L displacement classes (any pointer may be NULL). */
 int am_fmm_stats(am_store* store, int64_t* n_sources, int64_t* n_cells, int64_t* m2l_pairs,
                  int64_t* p2p_cell_pairs, int32_t* max_depth, int64_t* m2l_classes);
+/* More statistics of the most recent FMM call (no reference counterpart): M2L pairs whose
+ * displacement fits no class (they get tensors of their own), and the work items of the FP32 MFMA
+ * M2L (0 when the FP64 contraction ran). */
+int am_fmm_m2l_stats(am_store* store, int64_t* unclassed, int64_t* items);
 
 /* ---- behaviours ---------------------------------------------------------------------------- */
 /* LorentzForce::Run for every alive agent (src/forces/lorentz.cc:37-71): F = qE + F_coul,
@@ -330,6 +334,10 @@ int am_debug_radix_sort(am_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, 
 /* Stable radix sort of keys alone on bits [bit0, bit0 + key_bits) on the device, in place on the host
  * array; n < 2^30. */
 int am_debug_radix_sort_keys(am_ctx* ctx, uint64_t* keys, int64_t n, int bit0, int key_bits);
+/* FMM M2L options of the context (defaults 1 and 2^20): classes = 0 gives every M2L pair a tensor of
+ * its own (no displacement classes); max_unclassed is the most unclassed pairs that the FP32 MFMA
+ * M2L takes, past which the FP64 contraction runs (negative: always the FP64 contraction). */
+int am_debug_fmm_m2l_options(am_ctx* ctx, int classes, int64_t max_unclassed);
 
 #ifdef __cplusplus
 }

@@ -119,3 +119,38 @@ def test_fmm_fp32_overfull_leaf(store):
     ref = oracle.fmm_forces(pos, q, None, 4, 0.5)
     assert rel_l2(store.forces(), ref) < 1e-6
     np.testing.assert_allclose(store.forces()[-45:], ref[-45:], rtol=1e-5, atol=1e-6 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("order,theta", [(4, 0.5), (5, 0.4), (3, 0.6)])
+def test_fmm_fp32_m2l_paths_agree(ctx, order, theta):
+    """The three M2L routes of the FP32 path on one cloud: displacement classes on matrix cores
+    (the default), every pair with a tensor of its own on matrix cores (classes switched off, so
+    every pair takes the unclassed route), and the FP64 contraction (the route past the unclassed
+    bound).  Each is within the FP32 bound of the oracle; the two matrix-core routes form the same
+    FP32 products and differ only by their exact-accumulation grids (the grid follows a group's
+    product count, 16 times larger when every pair is its own product); repeated calls are
+    bit-identical (the accumulation is exact, so its order does not matter)."""
+    pos, sp, ann = cloud(7000, seed=60 + order)
+    s = am.Store(ctx, len(sp))
+    s.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    m, q = oracle.mass_charge(sp)
+    ref = oracle.fmm_forces(pos, q, ann, order, theta)
+    live = ann == 0
+    out = {}
+    try:
+        for name, classes, cap in (("classes", True, 1 << 20), ("unclassed", False, 1 << 20), ("fp64", True, -1)):
+            ctx.fmm_m2l_options(classes, cap)
+            s.compute_fast_multipole_coulomb_forces(order, theta, am.FP32)
+            out[name] = (s.forces(), s.fmm_stats())
+            print(name, out[name][1], flush=True)
+            assert rel_l2(out[name][0][live], ref[live]) < 1e-6, name
+    finally:
+        ctx.fmm_m2l_options(True, 1 << 20)
+    st = out["classes"][1]
+    assert st["m2l_pairs"] > 0 and st["m2l_unclassed"] == 0 and st["m2l_items"] >= 1
+    assert out["unclassed"][1]["m2l_unclassed"] == st["m2l_pairs"] and out["unclassed"][1]["m2l_items"] >= 1
+    assert out["fp64"][1]["m2l_items"] == 0
+    assert rel_l2(out["unclassed"][0][live], out["classes"][0][live]) < 1e-10
+    s.compute_fast_multipole_coulomb_forces(order, theta, am.FP32)
+    np.testing.assert_array_equal(s.forces(), out["classes"][0])
+    s.close()
