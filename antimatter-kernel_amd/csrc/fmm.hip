@@ -887,7 +887,7 @@ typedef float f32x4v __attribute__((ext_vector_type(4)));
 constexpr int kM2LGroupTargets = 256;  // target cells per group (consecutive BFS ids)
 constexpr int kM2LGroupBits = 8;
 constexpr int kM2LMfmaMaxOrder = 5;  // orders of the matrix-core M2L (nm <= 56)
-constexpr int kM2LWavesP4 = 12;     // waves per workgroup of the product kernel (orders <= 4)
+constexpr int kM2LWavesP4 = 16;     // waves per workgroup of the product kernel (orders <= 4)
 constexpr int kM2LWavesP5 = 8;      // (order 5: more registers per wave)
 
 // FP32 signed multipoles in the B-operand order of the products: cell S's row of 4 nkp floats holds
@@ -1188,14 +1188,15 @@ __global__ __launch_bounds__(64 * W) void M2LGroupKernel(
   // Software pipeline over three slots, the stage written out three times with the slots rotated so
   // that no register a load may still be writing is copied.  At stage k: product k+2's table entry
   // (start, next start, class, column word) is loaded; product k+1's multipole rows and class tensor
-  // are loaded from its entry (loaded a stage earlier); product k runs on the matrix cores; product
-  // k-1's result is added.  Every stage issues the same loads in the same order (positions past the
-  // wave's last product are clamped to it), so the compiler's in-order waits are exact.
+  // are loaded from its entry (loaded a stage earlier); product k-1's result is added; product k
+  // runs on the matrix cores into the same accumulators.  Every stage issues the same loads in the
+  // same order (positions past the wave's last product are clamped to it), so the compiler's
+  // in-order waits are exact.
   struct Slot {
     uint32_t st0, st1, cl, cw;  // table entry: start, next start, class, column word
     float bv[NKS], dv[NDL];
-    f32x4v acc[NRB];
   };
+  f32x4v acc_r[NRB];
   auto load_entry = [&](int k, Slot& s) {
     const int kk = k < pe ? k : pe - 1;
     s.st0 = pstart[kk];
@@ -1229,6 +1230,8 @@ __global__ __launch_bounds__(64 * W) void M2LGroupKernel(
   int locB = 0;
   bool presB = false, pending = false;
   Slot s0, s1, s2;
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb) acc_r[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
   if (pb < pe) {
     load_entry(pb, s0);
     load_entry(pb + 1, s1);
@@ -1236,13 +1239,14 @@ __global__ __launch_bounds__(64 * W) void M2LGroupKernel(
   }
   int k = pb;
   auto stage = [&](Slot& c, Slot& n, Slot& f) -> bool {
-    if (k >= pe) {  // no product k: the last one (k-1, in f) is added and the wave is done
-      if (pending) add_columns(f.acc, locB, presB);
+    if (k >= pe) {  // no product k: the last one (k-1) is added and the wave is done
+      if (pending) add_columns(acc_r, locB, presB);
       pending = false;
       return false;
     }
     load_entry(k + 2, f);
     load_operands(n);
+    if (pending) add_columns(acc_r, locB, presB);  // product k-1's columns into the locals
     const uint32_t ccls = (uint32_t)__builtin_amdgcn_readfirstlane((int)c.cl);
     if (ccls != cur) {  // the class's fragments
       WaveSync();
@@ -1257,13 +1261,12 @@ __global__ __launch_bounds__(64 * W) void M2LGroupKernel(
       cur = ccls;
     }
 #pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) c.acc[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
+    for (int rb = 0; rb < NRB; ++rb) acc_r[rb] = f32x4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
-        c.acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], c.bv[ks], c.acc[rb], 0, 0, 0);
-    if (pending) add_columns(f.acc, locB, presB);  // product k-1's columns into the locals
+        acc_r[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[rb][ks], c.bv[ks], acc_r[rb], 0, 0, 0);
     locB = (int)(c.cw & (G - 1));
     presB = j < (int)(c.st1 - c.st0);
     pending = true;
