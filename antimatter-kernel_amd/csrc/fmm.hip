@@ -1057,48 +1057,65 @@ __global__ __launch_bounds__(64) void M2LProductKernel(int n_slices, int W, cons
   int64_t rs = sb, hp = sb;           // start of the run and of the product that the window continues
   uint32_t prev = 0xffffffffu;        // class of the position before the window
   const uint64_t incl = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  for (int64_t wb = sb; wb < se; wb += 64) {
-    const int64_t e = wb + lane;
-    const bool valid = e < se;
-    const uint64_t k = valid ? keys[e] : 0ull;
-    const uint32_t c = (uint32_t)((k >> kl.cls_shift) & kl.cls_mask);
-    uint32_t pc = (uint32_t)__shfl_up((int)c, 1);
-    if (lane == 0) pc = prev;
-    const uint64_t runs = __ballot(valid && (e == sb || c != pc));
-    const uint64_t rb = runs & incl;
-    const int64_t myrs = rb ? wb + 63 - __builtin_clzll(rb) : rs;
-    const bool head = valid && ((e - myrs) & 15) == 0;
-    const uint64_t heads = __ballot(head);
-    const uint64_t hb = heads & incl;
-    if (write && valid) {
-      const int64_t myhp = hb ? wb + 63 - __builtin_clzll(hb) : hp;
-      const uint32_t p = np + (uint32_t)__popcll(hb) - 1u;  // hb == 0: the product of the previous window
-      const uint32_t S = kl.packed ? (uint32_t)(k & kl.smask) : vals[e];
-      const uint32_t loc = (uint32_t)((k >> (kl.cls_shift - kM2LGroupBits)) & (kM2LGroupTargets - 1));
-      col[16 * (size_t)p + (size_t)(e - myhp)] = (S << 8) | loc;
-      const float* dkc = dk + (size_t)c * (2 * P + 1);
-      const float* mas = ma + (size_t)S * (P + 1);
-      float dv[2 * kM2LMfmaMaxOrder + 1], mv[kM2LMfmaMaxOrder + 1];
+  // keys come in blocks of four steps (256 positions), each block loaded while the one before it
+  // is processed
+  uint64_t kq[4], kn[4];
 #pragma unroll
-      for (int d = 0; d <= 2 * kM2LMfmaMaxOrder; ++d) dv[d] = d <= 2 * P ? dkc[d] : 0.f;
+  for (int q = 0; q < 4; ++q) kq[q] = sb + 64 * q + lane < se ? keys[sb + 64 * q + lane] : 0ull;
+  for (int64_t wb4 = sb; wb4 < se; wb4 += 256) {
 #pragma unroll
-      for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) mv[a] = a <= P ? mas[a] : 0.f;
-#pragma unroll
-      for (int n = 0; n <= kM2LMfmaMaxOrder; ++n) {
-        float b = 0.f;
-#pragma unroll
-        for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) b += dv[n + a] * mv[a];
-        bmax[n] = fmaxf(bmax[n], b);
-      }
-      if (head) {
-        start[p] = (uint32_t)e;
-        cls[p] = c;
-      }
-      hp = __shfl(myhp, 63);
+    for (int q = 0; q < 4; ++q) {
+      const int64_t en = wb4 + 256 + 64 * q + lane;
+      kn[q] = en < se ? keys[en] : 0ull;
     }
-    np += (uint32_t)__popcll(heads);
-    rs = (int64_t)__shfl(myrs, 63);  // (lanes past se keep the run start of the last valid lane)
-    prev = __shfl(c, 63);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t wb = wb4 + 64 * q;
+      if (wb >= se) break;
+      const int64_t e = wb + lane;
+      const bool valid = e < se;
+      const uint64_t k = kq[q];
+      const uint32_t c = (uint32_t)((k >> kl.cls_shift) & kl.cls_mask);
+      uint32_t pc = (uint32_t)__shfl_up((int)c, 1);
+      if (lane == 0) pc = prev;
+      const uint64_t runs = __ballot(valid && (e == sb || c != pc));
+      const uint64_t rb = runs & incl;
+      const int64_t myrs = rb ? wb + 63 - __builtin_clzll(rb) : rs;
+      const bool head = valid && ((e - myrs) & 15) == 0;
+      const uint64_t heads = __ballot(head);
+      const uint64_t hb = heads & incl;
+      const int64_t myhp = hb ? wb + 63 - __builtin_clzll(hb) : hp;
+      if (write && valid) {
+        const uint32_t p = np + (uint32_t)__popcll(hb) - 1u;  // hb == 0: the product of the previous step
+        const uint32_t S = kl.packed ? (uint32_t)(k & kl.smask) : vals[e];
+        const uint32_t loc = (uint32_t)((k >> (kl.cls_shift - kM2LGroupBits)) & (kM2LGroupTargets - 1));
+        col[16 * (size_t)p + (size_t)(e - myhp)] = (S << 8) | loc;
+        const float* dkc = dk + (size_t)c * (2 * P + 1);
+        const float* mas = ma + (size_t)S * (P + 1);
+        float dv[2 * kM2LMfmaMaxOrder + 1], mv[kM2LMfmaMaxOrder + 1];
+#pragma unroll
+        for (int d = 0; d <= 2 * kM2LMfmaMaxOrder; ++d) dv[d] = d <= 2 * P ? dkc[d] : 0.f;
+#pragma unroll
+        for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) mv[a] = a <= P ? mas[a] : 0.f;
+#pragma unroll
+        for (int n = 0; n <= kM2LMfmaMaxOrder; ++n) {
+          float bsum = 0.f;
+#pragma unroll
+          for (int a = 0; a <= kM2LMfmaMaxOrder; ++a) bsum += dv[n + a] * mv[a];
+          bmax[n] = fmaxf(bmax[n], bsum);
+        }
+        if (head) {
+          start[p] = (uint32_t)e;
+          cls[p] = c;
+        }
+      }
+      np += (uint32_t)__popcll(heads);
+      hp = (int64_t)__shfl(myhp, 63);  // (lanes past se keep the values of the last valid lane)
+      rs = (int64_t)__shfl(myrs, 63);
+      prev = __shfl(c, 63);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) kq[q] = kn[q];
   }
   if (lane == 0) {
     if (write) start[off[s + 1]] = (uint32_t)se;

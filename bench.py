@@ -93,9 +93,10 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}), flush=True)
 
 
-def read_traffic():
-    """HBM bytes per all-pairs launch from the committed rocprofv3 --pmc summary, if present."""
-    path = os.path.join(ROOT, "profiles", "allpairs_f32_n65536_pmc.json")
+def read_traffic(name="allpairs_f32_n65536_pmc.json"):
+    """HBM bytes per launch of a workload's dominant kernel from its committed rocprofv3 --pmc
+    summary under profiles/, if present."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
@@ -253,7 +254,7 @@ def tree_workload(kind, steps, warmup, with_cpu, cpu_steps):
                                  "met": bool(err <= 1e-5)}
         out["roofline"] = {"bound": "mfma", "achieved": flops / (avg_ms * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": flops / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-                           "traffic": None,
+                           "traffic": read_traffic(f"fmm_m2l_p{order}_n1m_pmc.json"),
                            "note": f"M2L MFMA kernel (FP32); {2 * nm * nm} flop ({nm} x {nm} multiply-adds) per cell pair"}
     else:
         rows = stats["model_rows"] + stats["fallback_rows"]
