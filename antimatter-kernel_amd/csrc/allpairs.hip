@@ -212,36 +212,66 @@ __global__ __launch_bounds__(kTile) void TilePrepKernel(int n, uint32_t* __restr
   TilePrepBody(blockIdx.x * kTile, n - (int)blockIdx.x * kTile, blockIdx.x, perm, x, y, z, src, shl, sloc, tiles);
 }
 
-__global__ __launch_bounds__(kAllPairsThreads) void AllPairsSortedKernel(
-    int n, int n_pad, int tiles_per_chunk, const uint32_t* __restrict__ perm, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ shl,
-    const float4* __restrict__ sloc, const TileRec* __restrict__ tiles, double* __restrict__ part) {
-  const int k = blockIdx.x * kAllPairsThreads + threadIdx.x;
-  const BlockBox bb = MakeBlockBox(tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1],
-                                    2 * blockIdx.x * kTile + kTile >= n);
-  const float4 th = shl[2 * k], tl = shl[2 * k + 1];
-  float lx = 0.f, ly = 0.f, lz = 0.f;
-  if (k < n) {
-    const uint32_t i = perm[k];
-    lx = (float)(x[i] * kUnitScale - bb.bc[0]);
-    ly = (float)(y[i] * kUnitScale - bb.bc[1]);
-    lz = (float)(z[i] * kUnitScale - bb.bc[2]);
-  }
+// Work items of the sorted kernel: (target block, source chunk), taken from a queue in chunk-major
+// order by a persistent grid of kSortedWaves waves per SIMD.  A SIMD issues from its oldest waves
+// first, so with one fixed (block, chunk) per workgroup the early workgroups finished long before the
+// late ones and the last part of the launch ran with one or two waves per SIMD (at N = 65,536 the
+// per-workgroup durations grew from 0.49 to 1.4-1.6 ms with the chunk index).  With a queue the
+// waves that run ahead take more items, and items of kTilesPerItem tiles keep the final drain short.
+// The counter pair [next item, finished workgroups] is reset by the last workgroup to finish.
+constexpr int kSortedWaves = 8;
+constexpr int kTilesPerItem = 8;
+constexpr int kSortedMaxItems = 16384;  // bounds the partial sums at chunks x 24 bytes x n
+
+__global__ __launch_bounds__(kAllPairsThreads) __attribute__((amdgpu_waves_per_eu(kSortedWaves, kSortedWaves)))
+void AllPairsSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_blocks, int n_items,
+                          unsigned* __restrict__ queue, const uint32_t* __restrict__ perm,
+                          const double* __restrict__ x, const double* __restrict__ y,
+                          const double* __restrict__ z, const float4* __restrict__ shl,
+                          const float4* __restrict__ sloc, const TileRec* __restrict__ tiles,
+                          double* __restrict__ part) {
+  __shared__ int s_item;
   const int n_tiles = n_pad / kTile;
-  const int t0 = blockIdx.y * tiles_per_chunk;
-  const int t1 = min(n_tiles, t0 + tiles_per_chunk);
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for (int T = t0; T < t1; ++T) {
-    float px = 0.f, py = 0.f, pz = 0.f;
-    TileSum(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc, px, py, pz);
-    ax += (double)px;
-    ay += (double)py;
-    az += (double)pz;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();  // every lane has read s_item before thread 0 overwrites it
+    if (item >= n_items) break;
+    const int blk = item % n_blocks, chunk = item / n_blocks;
+    const int k = blk * kAllPairsThreads + threadIdx.x;
+    const BlockBox bb = MakeBlockBox(tiles[2 * blk], tiles[2 * blk + 1], 2 * blk * kTile + kTile >= n);
+    const float4 th = shl[2 * k], tl = shl[2 * k + 1];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    if (k < n) {
+      const uint32_t i = perm[k];
+      lx = (float)(x[i] * kUnitScale - bb.bc[0]);
+      ly = (float)(y[i] * kUnitScale - bb.bc[1]);
+      lz = (float)(z[i] * kUnitScale - bb.bc[2]);
+    }
+    const int t0 = chunk * tiles_per_chunk;
+    const int t1 = min(n_tiles, t0 + tiles_per_chunk);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int T = t0; T < t1; ++T) {
+      float px = 0.f, py = 0.f, pz = 0.f;
+      TileSum(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc, px, py, pz);
+      ax += (double)px;
+      ay += (double)py;
+      az += (double)pz;
+    }
+    double* out = part + (size_t)chunk * 3 * n_pad;
+    out[k] = ax;
+    out[n_pad + k] = ay;
+    out[2 * (size_t)n_pad + k] = az;
   }
-  double* out = part + (size_t)blockIdx.y * 3 * n_pad;
-  out[k] = ax;
-  out[n_pad + k] = ay;
-  out[2 * (size_t)n_pad + k] = az;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {  // every other workgroup has taken its last item
+      queue[0] = 0u;
+      queue[1] = 0u;
+      __threadfence();
+    }
+  }
 }
 
 __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const double* __restrict__ part,
@@ -319,27 +349,48 @@ __device__ __forceinline__ float TilePotential(const TileRec& R, int tile_first,
 
 // part[chunk][k] = phi at sorted position k over the chunk's tiles (e/unit), FP32 per tile, FP64
 // across tiles.
-__global__ __launch_bounds__(kAllPairsThreads) void AllPairsPotentialSortedKernel(
-    int n, int n_pad, int tiles_per_chunk, const uint32_t* __restrict__ perm, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ shl,
-    const float4* __restrict__ sloc, const TileRec* __restrict__ tiles, double* __restrict__ part) {
-  const int k = blockIdx.x * kAllPairsThreads + threadIdx.x;
-  const BlockBox bb = MakeBlockBox(tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1],
-                                    2 * blockIdx.x * kTile + kTile >= n);
-  const float4 th = shl[2 * k], tl = shl[2 * k + 1];
-  float lx = 0.f, ly = 0.f, lz = 0.f;
-  if (k < n) {
-    const uint32_t i = perm[k];
-    lx = (float)(x[i] * kUnitScale - bb.bc[0]);
-    ly = (float)(y[i] * kUnitScale - bb.bc[1]);
-    lz = (float)(z[i] * kUnitScale - bb.bc[2]);
-  }
+// The same work queue as AllPairsSortedKernel, one partial potential per (chunk, target).
+__global__ __launch_bounds__(kAllPairsThreads) __attribute__((amdgpu_waves_per_eu(kSortedWaves, kSortedWaves)))
+void AllPairsPotentialSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_blocks, int n_items,
+                                   unsigned* __restrict__ queue, const uint32_t* __restrict__ perm,
+                                   const double* __restrict__ x, const double* __restrict__ y,
+                                   const double* __restrict__ z, const float4* __restrict__ shl,
+                                   const float4* __restrict__ sloc, const TileRec* __restrict__ tiles,
+                                   double* __restrict__ part) {
+  __shared__ int s_item;
   const int n_tiles = n_pad / kTile;
-  const int t0 = blockIdx.y * tiles_per_chunk;
-  const int t1 = min(n_tiles, t0 + tiles_per_chunk);
-  double acc = 0.0;
-  for (int T = t0; T < t1; ++T) acc += (double)TilePotential(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc);
-  part[(size_t)blockIdx.y * n_pad + k] = acc;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= n_items) break;
+    const int blk = item % n_blocks, chunk = item / n_blocks;
+    const int k = blk * kAllPairsThreads + threadIdx.x;
+    const BlockBox bb = MakeBlockBox(tiles[2 * blk], tiles[2 * blk + 1], 2 * blk * kTile + kTile >= n);
+    const float4 th = shl[2 * k], tl = shl[2 * k + 1];
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    if (k < n) {
+      const uint32_t i = perm[k];
+      lx = (float)(x[i] * kUnitScale - bb.bc[0]);
+      ly = (float)(y[i] * kUnitScale - bb.bc[1]);
+      lz = (float)(z[i] * kUnitScale - bb.bc[2]);
+    }
+    const int t0 = chunk * tiles_per_chunk;
+    const int t1 = min(n_tiles, t0 + tiles_per_chunk);
+    double acc = 0.0;
+    for (int T = t0; T < t1; ++T)
+      acc += (double)TilePotential(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc);
+    part[(size_t)chunk * n_pad + k] = acc;
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      queue[0] = 0u;
+      queue[1] = 0u;
+      __threadfence();
+    }
+  }
 }
 
 // out[i] = (q_i / e) phi_i in e^2/unit, zero for annihilated, neutral or non-finite particles.
@@ -365,6 +416,34 @@ struct SortedTiles {
   const TileRec* tiles = nullptr;
   int n_tiles = 0;
 };
+
+__global__ void QueueZeroKernel(unsigned* q) {
+  if (threadIdx.x < 2) q[threadIdx.x] = 0u;
+}
+
+// Source chunks of the sorted kernels: kTilesPerItem tiles each, or more where that would give more
+// than kSortedMaxItems (block, chunk) items.
+static void SortedChunks(int n_tiles, int tblocks, int* tpc, int* nch) {
+  const int max_chunks = std::max(1, kSortedMaxItems / tblocks);
+  *tpc = std::max(kTilesPerItem, (n_tiles + max_chunks - 1) / max_chunks);
+  *nch = (n_tiles + *tpc - 1) / *tpc;
+}
+
+// The sorted kernel's work-queue counters: zeroed by a kernel when first allocated (so no memset node
+// enters a captured graph); each launch leaves them at zero.
+static int SortedQueue(am_ctx* ctx, unsigned** out) {
+  auto it = ctx->scratch.find("ap_queue");
+  const bool fresh = it == ctx->scratch.end() || it->second.ptr == nullptr;
+  void* q;
+  int rc = Scratch(ctx, "ap_queue", 2 * sizeof(unsigned), &q);
+  if (rc) return rc;
+  if (fresh) {
+    hipLaunchKernelGGL(QueueZeroKernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned*)q);
+    AM_HIP(hipGetLastError());
+  }
+  *out = (unsigned*)q;
+  return AM_OK;
+}
 
 static int PrepareSortedTiles(am_store* s, SortedTiles* out) {
   am_ctx* ctx = s->ctx;
@@ -416,14 +495,17 @@ int AllPairsForcesF32Sorted(am_store* s) {
   const int n_tiles = T.n_tiles;
   void* part;
   const int tblocks = n_pad / kAllPairsThreads;
-  const int chunks = ChunkCount(ctx->num_cus, tblocks, n_pad);
-  const int tpc = (n_tiles + chunks - 1) / chunks;
-  const int nch = (n_tiles + tpc - 1) / tpc;
+  int tpc, nch;
+  SortedChunks(n_tiles, tblocks, &tpc, &nch);
   if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
+  unsigned* queue;
+  if ((rc = SortedQueue(ctx, &queue))) return rc;
+  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
   int slot;
   TimerBegin(ctx, "allpairs_f32", &slot);
-  hipLaunchKernelGGL(AllPairsSortedKernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, st, n, n_pad, tpc,
-                     (const uint32_t*)s->perm, s->x, s->y, s->z, T.shl, T.sloc, T.tiles, (double*)part);
+  hipLaunchKernelGGL(AllPairsSortedKernel, dim3(cus * kSortedWaves), dim3(kAllPairsThreads), 0, st, n, n_pad, tpc,
+                     tblocks, tblocks * nch, queue, (const uint32_t*)s->perm, s->x, s->y, s->z, T.shl, T.sloc,
+                     T.tiles, (double*)part);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   hipLaunchKernelGGL(AllPairsSortedReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, st, n, n_pad, nch,
@@ -508,15 +590,18 @@ int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_s
     SortedTiles T;
     int rc = PrepareSortedTiles(s, &T);
     if (rc) return rc;
-    const int tpc = (T.n_tiles + chunks - 1) / chunks;
-    const int nch = (T.n_tiles + tpc - 1) / tpc;
+    int tpc, nch;
+    SortedChunks(T.n_tiles, tblocks, &tpc, &nch);
     void* part = nullptr;
     if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
+    unsigned* queue;
+    if ((rc = SortedQueue(ctx, &queue))) return rc;
+    const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
     int slot;
     TimerBegin(ctx, "allpairs_potential_f32", &slot);
-    hipLaunchKernelGGL(AllPairsPotentialSortedKernel, dim3(tblocks, nch), dim3(kAllPairsThreads), 0, ctx->stream, n,
-                       n_pad, tpc, (const uint32_t*)s->perm, s->x, s->y, s->z, T.shl, T.sloc, T.tiles,
-                       (double*)part);
+    hipLaunchKernelGGL(AllPairsPotentialSortedKernel, dim3(cus * kSortedWaves), dim3(kAllPairsThreads), 0,
+                       ctx->stream, n, n_pad, tpc, tblocks, tblocks * nch, queue, (const uint32_t*)s->perm, s->x,
+                       s->y, s->z, T.shl, T.sloc, T.tiles, (double*)part);
     TimerEnd(ctx, slot);
     AM_HIP(hipGetLastError());
     hipLaunchKernelGGL(AllPairsPotentialSortedReduceKernel, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, n,

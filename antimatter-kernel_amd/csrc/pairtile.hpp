@@ -226,27 +226,29 @@ __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const 
   const int gc = gb + (nq > gb ? ((nq - gb) & ~7) : 0);          // [gb,gc) -e
   const int gd = nq > gc ? (nq + 7) & ~7 : gc;                   // [gc,gd) mixed, then q = 0
   if (near) {
+    // Hi/lo sources in groups of 4 (32 SGPRs per group): groups of 8 need 64 SGPRs, which at
+    // 8 waves per SIMD spill to VGPR lanes.
     const float4* sp = shl + 2 * tile_first;
     if (unit) {
-      for (int j0 = 0; j0 < ga; j0 += 8) {
-        const SrcHL8 g = LoadSrcHL8(sp, j0);
+      for (int j0 = 0; j0 < ga; j0 += 4) {
+        const Src8 g = LoadSrc8(sp, 2 * j0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) PairF32Unit<true>(th, tl, g.hi(u), g.lo(u), px, py, pz);
+        for (int u = 0; u < 4; ++u) PairF32Unit<true>(th, tl, g[2 * u], g[2 * u + 1], px, py, pz);
       }
-      for (int j0 = ga; j0 < gb; j0 += 8) {
-        const SrcHL8 g = LoadSrcHL8(sp, j0);
+      for (int j0 = ga; j0 < gb; j0 += 4) {
+        const Src8 g = LoadSrc8(sp, 2 * j0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) PairF32(th, tl, g.hi(u), g.lo(u), px, py, pz);
+        for (int u = 0; u < 4; ++u) PairF32(th, tl, g[2 * u], g[2 * u + 1], px, py, pz);
       }
-      for (int j0 = gb; j0 < gc; j0 += 8) {
-        const SrcHL8 g = LoadSrcHL8(sp, j0);
+      for (int j0 = gb; j0 < gc; j0 += 4) {
+        const Src8 g = LoadSrc8(sp, 2 * j0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) PairF32Unit<false>(th, tl, g.hi(u), g.lo(u), px, py, pz);
+        for (int u = 0; u < 4; ++u) PairF32Unit<false>(th, tl, g[2 * u], g[2 * u + 1], px, py, pz);
       }
-      for (int j0 = gc; j0 < gd; j0 += 8) {
-        const SrcHL8 g = LoadSrcHL8(sp, j0);
+      for (int j0 = gc; j0 < gd; j0 += 4) {
+        const Src8 g = LoadSrc8(sp, 2 * j0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) PairF32(th, tl, g.hi(u), g.lo(u), px, py, pz);
+        for (int u = 0; u < 4; ++u) PairF32(th, tl, g[2 * u], g[2 * u + 1], px, py, pz);
       }
     } else {
 #pragma unroll 8
