@@ -417,32 +417,12 @@ struct SortedTiles {
   int n_tiles = 0;
 };
 
-__global__ void QueueZeroKernel(unsigned* q) {
-  if (threadIdx.x < 2) q[threadIdx.x] = 0u;
-}
-
 // Source chunks of the sorted kernels: kTilesPerItem tiles each, or more where that would give more
 // than kSortedMaxItems (block, chunk) items.
 static void SortedChunks(int n_tiles, int tblocks, int* tpc, int* nch) {
   const int max_chunks = std::max(1, kSortedMaxItems / tblocks);
   *tpc = std::max(kTilesPerItem, (n_tiles + max_chunks - 1) / max_chunks);
   *nch = (n_tiles + *tpc - 1) / *tpc;
-}
-
-// The sorted kernel's work-queue counters: zeroed by a kernel when first allocated (so no memset node
-// enters a captured graph); each launch leaves them at zero.
-static int SortedQueue(am_ctx* ctx, unsigned** out) {
-  auto it = ctx->scratch.find("ap_queue");
-  const bool fresh = it == ctx->scratch.end() || it->second.ptr == nullptr;
-  void* q;
-  int rc = Scratch(ctx, "ap_queue", 2 * sizeof(unsigned), &q);
-  if (rc) return rc;
-  if (fresh) {
-    hipLaunchKernelGGL(QueueZeroKernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned*)q);
-    AM_HIP(hipGetLastError());
-  }
-  *out = (unsigned*)q;
-  return AM_OK;
 }
 
 static int PrepareSortedTiles(am_store* s, SortedTiles* out) {
@@ -499,7 +479,7 @@ int AllPairsForcesF32Sorted(am_store* s) {
   SortedChunks(n_tiles, tblocks, &tpc, &nch);
   if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
   unsigned* queue;
-  if ((rc = SortedQueue(ctx, &queue))) return rc;
+  if ((rc = QueueCounters(ctx, "ap_queue", &queue))) return rc;
   const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
   int slot;
   TimerBegin(ctx, "allpairs_f32", &slot);
@@ -595,7 +575,7 @@ int AllPairsPotentialTerms(am_store* s, int precision, double* out, double* si_s
     void* part = nullptr;
     if ((rc = Scratch(ctx, "allpairs_part", (size_t)nch * 3 * n_pad * sizeof(double), &part))) return rc;
     unsigned* queue;
-    if ((rc = SortedQueue(ctx, &queue))) return rc;
+    if ((rc = QueueCounters(ctx, "ap_queue", &queue))) return rc;
     const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
     int slot;
     TimerBegin(ctx, "allpairs_potential_f32", &slot);

@@ -118,6 +118,12 @@ struct am_store {
   // Bumped by every am_store_upload: objects built over the store (ensembles) record it at creation
   // and refuse to run once the particle set they were sized for has been replaced.
   uint64_t generation = 0;
+  // Barnes-Hut walk schedule: node visits of each 64-target chunk in the previous walk (walk_cost),
+  // and the chunks in descending cost (walk_order) for the next one.  Only the speed depends on it.
+  uint32_t* walk_cost = nullptr;
+  uint32_t* walk_order = nullptr;
+  int64_t walk_chunks = 0;     // chunks walk_cost was measured for (0: none yet)
+  int64_t walk_capacity = 0;   // chunks the two buffers hold
   am::FmmRunStats fmm_stats;
   am::BHBuildInfo bh_info;
 };
@@ -126,6 +132,9 @@ namespace am {
 
 // Scratch buffer owned by the context, grown on demand (contents not preserved on growth).
 int Scratch(am_ctx* ctx, const char* name, size_t bytes, void** out);
+// Two work-queue counters [next item, finished workers] in a named scratch buffer, zeroed when first
+// allocated.  The kernels that use them leave them at zero.
+int QueueCounters(am_ctx* ctx, const char* name, unsigned** out);
 
 // Kernel timing: Begin/End bracket a launch family on the context stream when timing is on.
 void TimerBegin(am_ctx* ctx, const char* name, int* slot);

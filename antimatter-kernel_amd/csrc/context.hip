@@ -142,6 +142,20 @@ __global__ void ZeroU64Kernel(unsigned long long* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
+int QueueCounters(am_ctx* ctx, const char* name, unsigned** out) {
+  auto it = ctx->scratch.find(name);
+  const bool fresh = it == ctx->scratch.end() || it->second.ptr == nullptr;
+  void* q;
+  int rc = Scratch(ctx, name, sizeof(unsigned long long), &q);
+  if (rc) return rc;
+  if (fresh) {  // a kernel, not a memset, so no memset node enters a captured graph
+    hipLaunchKernelGGL(ZeroU64Kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long*)q, (int64_t)1);
+    AM_HIP(hipGetLastError());
+  }
+  *out = (unsigned*)q;
+  return AM_OK;
+}
+
 int64_t CountActive(am_store* s, int* status) {
   *status = AM_OK;
   if (s->n == 0) return 0;
@@ -308,8 +322,9 @@ int am_store_destroy(am_store* s) {
   if (!s) return AM_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  void* ptrs[] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->q, s->m,
-                  s->species, s->flags, s->trap, s->src, s->perm};
+  void* ptrs[] = {s->x,     s->y,     s->z,    s->vx,  s->vy,   s->vz,        s->fx,
+                  s->fy,    s->fz,    s->q,    s->m,   s->species, s->flags,  s->trap,
+                  s->src,   s->perm,  s->walk_cost, s->walk_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
