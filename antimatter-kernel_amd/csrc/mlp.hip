@@ -498,6 +498,7 @@ __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, 
 // 2's outputs come back to the lanes of their rows the same way.  Adds the predicted residual (or,
 // past the bound, the exact one) to the force.  Each lane sums its rows' corrections in FP64 and
 // the wave adds the 64 partial sums once per target in a fixed order: deterministic.
+constexpr int kCorrectTake = 8;  // targets per queue take
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void MlpCorrectKernel(int m, const uint32_t* __restrict__ rec, int rec_cap,
                                                        const uint32_t* __restrict__ rec_cnt,
                                                        const WalkNode* __restrict__ nodes,
@@ -507,78 +508,88 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void Ml
                                                        const double* __restrict__ q, const float4* __restrict__ src,
                                                        MlpDev net, float ybound, float amax,
                                                        double* __restrict__ fx, double* __restrict__ fy,
-                                                       double* __restrict__ fz, unsigned long long* __restrict__ stats) {
+                                                       double* __restrict__ fz, unsigned long long* __restrict__ stats,
+                                                       unsigned* __restrict__ queue) {
   __shared__ MlpLds lds;
   MlpWave ws;
   MlpWaveLoad(net, lds, ws);
   const int l = threadIdx.x;
   const bool upper = l >= 32;
   unsigned long long n_model = 0, n_fallback = 0, n_over = 0;
-  // Persistent over targets: the weight fragments stay in registers for the whole launch.
-  for (int k = blockIdx.x; k < m; k += gridDim.x) {
-    const uint32_t i = order[k];
-    const double qi = q[i];
-    const float4 th = src[2 * i], tl = src[2 * i + 1];
-    const int cnt = min((int)rec_cnt[k], rec_cap);
-    n_over += (int)rec_cnt[k] > rec_cap ? 1 : 0;
-    const int nrows = 2 * cnt;
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int base = 0; base < nrows; base += 64) {
-      const int row = base + l;
-      bool ok = false;
-      float R = 1.f, Qs = 0.f, hh = 0.f;
-      float yb[3] = {0.f, 0.f, 0.f};
-      float f[kIn];
-      for (int a = 0; a < kIn; ++a) f[a] = 0.f;
-      int c = 0;
-      if (row < nrows) {
-        c = (int)rec[(size_t)k * rec_cap + row / 2];
-        ok = RowFeatures(th, tl, nodes32[c], feat[c], row & 1, f, R, Qs, hh);
-        if (ok && net.baseline) QuadBaseline(f, yb);
-      }
-      bf16x8 lo, hi;
-      MlpInputHalves(net.norm, f, ok, lo, hi);
-      // batch 1 = rows base .. base+31, batch 2 = rows base+32 .. base+63: lane n < 32 feeds
-      // features 0..7 of row n of its batch, lane n + 32 features 8..15 of row n
-      const bf16x8 got = OtherHalf(upper ? lo : hi);
-      float y1[kOut], y2[kOut] = {0.f, 0.f, 0.f};
-      MlpForwardRegs(ws, lds, upper ? got : lo, y1);
-      if (base + 32 < nrows) MlpForwardRegs(ws, lds, upper ? hi : got, y2);
-      float y[kOut];
-      for (int o = 0; o < kOut; ++o) {
-        const float moved = __shfl_xor(y2[o], 32);  // batch 2's row n + 32 is output in lane n
-        y[o] = upper ? moved : y1[o];
-      }
-      if (ok) {
-        float corr[3];
-        const float p0 = y[0] + yb[0], p1 = y[1] + yb[1], p2 = y[2] + yb[2];
-        const float a = hh / R;
-        const float mag = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-        if (mag <= ybound && a <= amax && mag == mag) {
-          const float sc = Qs * hh * hh / (R * R * R * R);
-          corr[0] = p0 * sc;
-          corr[1] = p1 * sc;
-          corr[2] = p2 * sc;
-          ++n_model;
-        } else {
-          ExactResidual(i, th, tl, nodes[c], nodes32[c], row & 1, order, src, corr);
-          ++n_fallback;
+  // Persistent over targets: the weight fragments stay in registers for the whole launch, and each wave
+  // takes kCorrectTake targets at a time from a queue, so the waves that run ahead (a SIMD issues from
+  // its oldest waves first) take more; the grid is the number of waves that fit at once.
+  for (;;) {
+    unsigned k0 = 0;
+    if (l == 0) k0 = atomicAdd(&queue[0], (unsigned)kCorrectTake);
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    if ((int)k0 >= m) break;
+    const int k1 = min(m, (int)k0 + kCorrectTake);
+    for (int k = (int)k0; k < k1; ++k) {
+      const uint32_t i = order[k];
+      const double qi = q[i];
+      const float4 th = src[2 * i], tl = src[2 * i + 1];
+      const int cnt = min((int)rec_cnt[k], rec_cap);
+      n_over += (int)rec_cnt[k] > rec_cap ? 1 : 0;
+      const int nrows = 2 * cnt;
+      double cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int base = 0; base < nrows; base += 64) {
+        const int row = base + l;
+        bool ok = false;
+        float R = 1.f, Qs = 0.f, hh = 0.f;
+        float yb[3] = {0.f, 0.f, 0.f};
+        float f[kIn];
+        for (int a = 0; a < kIn; ++a) f[a] = 0.f;
+        int c = 0;
+        if (row < nrows) {
+          c = (int)rec[(size_t)k * rec_cap + row / 2];
+          ok = RowFeatures(th, tl, nodes32[c], feat[c], row & 1, f, R, Qs, hh);
+          if (ok && net.baseline) QuadBaseline(f, yb);
         }
-        cx += corr[0];
-        cy += corr[1];
-        cz += corr[2];
+        bf16x8 lo, hi;
+        MlpInputHalves(net.norm, f, ok, lo, hi);
+        // batch 1 = rows base .. base+31, batch 2 = rows base+32 .. base+63: lane n < 32 feeds
+        // features 0..7 of row n of its batch, lane n + 32 features 8..15 of row n
+        const bf16x8 got = OtherHalf(upper ? lo : hi);
+        float y1[kOut], y2[kOut] = {0.f, 0.f, 0.f};
+        MlpForwardRegs(ws, lds, upper ? got : lo, y1);
+        if (base + 32 < nrows) MlpForwardRegs(ws, lds, upper ? hi : got, y2);
+        float y[kOut];
+        for (int o = 0; o < kOut; ++o) {
+          const float moved = __shfl_xor(y2[o], 32);  // batch 2's row n + 32 is output in lane n
+          y[o] = upper ? moved : y1[o];
+        }
+        if (ok) {
+          float corr[3];
+          const float p0 = y[0] + yb[0], p1 = y[1] + yb[1], p2 = y[2] + yb[2];
+          const float a = hh / R;
+          const float mag = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+          if (mag <= ybound && a <= amax && mag == mag) {
+            const float sc = Qs * hh * hh / (R * R * R * R);
+            corr[0] = p0 * sc;
+            corr[1] = p1 * sc;
+            corr[2] = p2 * sc;
+            ++n_model;
+          } else {
+            ExactResidual(i, th, tl, nodes[c], nodes32[c], row & 1, order, src, corr);
+            ++n_fallback;
+          }
+          cx += corr[0];
+          cy += corr[1];
+          cz += corr[2];
+        }
       }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      cx += __shfl_xor(cx, off);
-      cy += __shfl_xor(cy, off);
-      cz += __shfl_xor(cz, off);
-    }
-    if (l == 0 && qi != 0.0) {
-      const double scale = kCoulomb * qi * kElementaryCharge * kInvUnit2;
-      fx[i] += scale * cx;
-      fy[i] += scale * cy;
-      fz[i] += scale * cz;
+      for (int off = 32; off > 0; off >>= 1) {
+        cx += __shfl_xor(cx, off);
+        cy += __shfl_xor(cy, off);
+        cz += __shfl_xor(cz, off);
+      }
+      if (l == 0 && qi != 0.0) {
+        const double scale = kCoulomb * qi * kElementaryCharge * kInvUnit2;
+        fx[i] += scale * cx;
+        fy[i] += scale * cy;
+        fz[i] += scale * cz;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -589,6 +600,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void Ml
     atomicAdd(&stats[0], n_model);
     atomicAdd(&stats[1], n_fallback);
     atomicAdd(&stats[2], n_over);
+  }
+  if (l == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {  // one wave per workgroup
+      queue[0] = 0u;
+      queue[1] = 0u;
+      __threadfence();
+    }
   }
 }
 
@@ -975,11 +994,14 @@ int am_compute_barnes_hut_learned_forces(am_store* s, double theta, am_mlp* net,
     return rc;
   int slot;
   TimerBegin(ctx, "bh_learned_correct", &slot);
-  const int grid = std::min(t.m, std::max(1, ctx->num_cus) * 12);
+  unsigned* queue;
+  if ((rc = QueueCounters(ctx, "bhl_queue", &queue))) return rc;
+  // 2 waves per SIMD fit (the weight fragments take about 230 VGPRs): 8 one-wave workgroups per CU.
+  const int grid = std::min((t.m + kCorrectTake - 1) / kCorrectTake, std::max(1, ctx->num_cus) * 8);
   hipLaunchKernelGGL(MlpCorrectKernel, dim3(grid), dim3(64), 0, st, t.m, (const uint32_t*)rec, rec_cap,
                      (const uint32_t*)cnt, (const WalkNode*)t.nodes, (const WalkNodeF32*)t.nodes32,
                      (const FeatNode*)t.feat, (const uint32_t*)t.order, s->q, (const float4*)s->src, MlpView(net),
-                     ybound, amax, s->fx, s->fy, s->fz, (unsigned long long*)dstats);
+                     ybound, amax, s->fx, s->fy, s->fz, (unsigned long long*)dstats, queue);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   if (stats) {
