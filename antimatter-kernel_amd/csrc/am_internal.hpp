@@ -118,12 +118,6 @@ struct am_store {
   // Bumped by every am_store_upload: objects built over the store (ensembles) record it at creation
   // and refuse to run once the particle set they were sized for has been replaced.
   uint64_t generation = 0;
-  // Barnes-Hut walk schedule: node visits of each 64-target chunk in the previous walk (walk_cost),
-  // and the chunks in descending cost (walk_order) for the next one.  Only the speed depends on it.
-  uint32_t* walk_cost = nullptr;
-  uint32_t* walk_order = nullptr;
-  int64_t walk_chunks = 0;     // chunks walk_cost was measured for (0: none yet)
-  int64_t walk_capacity = 0;   // chunks the two buffers hold
   am::FmmRunStats fmm_stats;
   am::BHBuildInfo bh_info;
 };

@@ -313,19 +313,20 @@ struct ExtTargets {
   const uint32_t* self = nullptr;
 };
 
-// One wave's walk for targets k = chunk * 64 + lane (k >= m: no target); returns the number of nodes
-// the wave visited.
-template <int MODE, bool COUNT, bool EXT>
-__device__ __forceinline__ uint32_t BHWalkTargets(int k, int m, int nn, const WalkNode* __restrict__ nodes,
-                                                  const WalkNodeF32* __restrict__ nodes32,
-                                                  const uint32_t* __restrict__ order, const double* __restrict__ x,
-                                                  const double* __restrict__ y, const double* __restrict__ z,
-                                                  const double* __restrict__ q, const float4* __restrict__ src,
-                                                  const float4* __restrict__ ssrc, double theta2,
-                                                  double* __restrict__ out_x, double* __restrict__ out_y,
-                                                  double* __restrict__ out_z, int64_t* __restrict__ counts,
-                                                  const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec,
-                                                  int rec_cap, uint32_t* __restrict__ rec_cnt, const ExtTargets& ext) {
+template <int MODE, bool COUNT, bool EXT = false>
+__global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes,
+                                                     const WalkNodeF32* __restrict__ nodes32,
+                                                     const uint32_t* __restrict__ order,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const double* __restrict__ q,
+                                                     const float4* __restrict__ src,
+                                                     const float4* __restrict__ ssrc, double theta2,
+                                                     double* __restrict__ out_x, double* __restrict__ out_y,
+                                                     double* __restrict__ out_z, int64_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec,
+                                                     int rec_cap, uint32_t* __restrict__ rec_cnt,
+                                                     ExtTargets ext = ExtTargets()) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < m;
   const int kc = valid ? k : m - 1;
   uint32_t i;
@@ -364,9 +365,7 @@ __device__ __forceinline__ uint32_t BHWalkTargets(int k, int m, int nn, const Wa
   double ax = 0.0, ay = 0.0, az = 0.0;
   int64_t cnt = 0;
   int c = __builtin_amdgcn_readfirstlane(WaveMin(next));  // wave-uniform: node records come in by scalar loads
-  uint32_t visits = 0;
   while (c < nn) {
-    ++visits;
     const WalkNode nd = nodes[c];
     const bool mine = next == c;
     bool opened = false;
@@ -464,7 +463,7 @@ __device__ __forceinline__ uint32_t BHWalkTargets(int k, int m, int nn, const Wa
     // skip(c): no reduction over the lanes is needed.
     c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : nd.skip);
   }
-  if (!valid) return visits;
+  if (!valid) return;
   if (EXT) {  // per point: the force on charge q, or phi (C/m for MODE 2, e/unit for MODE 3)
     if (MODE == 0) {
       const double scale = want ? kCoulomb * qi * kElementaryCharge * kInvUnit2 : 0.0;
@@ -478,7 +477,7 @@ __device__ __forceinline__ uint32_t BHWalkTargets(int k, int m, int nn, const Wa
     } else {
       out_x[k] = ax;
     }
-    return visits;
+    return;
   }
   if (rec_cnt) rec_cnt[k] = nrec;
   if (MODE == 0) {
@@ -496,126 +495,6 @@ __device__ __forceinline__ uint32_t BHWalkTargets(int k, int m, int nn, const Wa
     out_x[i] = (double)th.w * ax;
   }
   if (COUNT) counts[i] = cnt;
-  return visits;
-}
-
-// Schedule of a walk launch.  Without a queue every wave of the grid walks one 64-target chunk.  With
-// a queue a persistent grid of waves takes chunks in the given order (ascending without one), so the
-// waves that run ahead take more chunks (a SIMD issues from its oldest waves first, and with a fixed
-// grid the last chunks ran with few waves per SIMD); each chunk's node visits go to cost.  The
-// counter pair [next item, finished waves] is reset by the last wave to finish.
-struct WalkSched {
-  unsigned* queue = nullptr;
-  const uint32_t* order = nullptr;
-  uint32_t* cost = nullptr;
-  int n_chunks = 0;
-};
-
-template <int MODE, bool COUNT, bool EXT = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes, const WalkNodeF32* __restrict__ nodes32,
-                  const uint32_t* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
-                  const double* __restrict__ z, const double* __restrict__ q, const float4* __restrict__ src,
-                  const float4* __restrict__ ssrc, double theta2, double* __restrict__ out_x,
-                  double* __restrict__ out_y, double* __restrict__ out_z, int64_t* __restrict__ counts,
-                  const uint32_t* __restrict__ tsel, uint32_t* __restrict__ rec, int rec_cap,
-                  uint32_t* __restrict__ rec_cnt, ExtTargets ext, WalkSched sched) {
-  const int lane = threadIdx.x & 63;
-  const bool queued = sched.queue != nullptr;
-  auto take = [&]() {
-    unsigned it = 0;
-    if (lane == 0) it = atomicAdd(&sched.queue[0], 1u);
-    return (int)__builtin_amdgcn_readfirstlane(it);
-  };
-  // Without a queue: one chunk per wave of the grid, the whole grid covering the targets.
-  int item = queued ? take() : (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int n_items = queued ? sched.n_chunks : (int)((gridDim.x * blockDim.x) >> 6);
-  while (item < n_items) {
-    const int chunk = sched.order ? (int)sched.order[item] : item;
-    const uint32_t visits =
-        BHWalkTargets<MODE, COUNT, EXT>(chunk * 64 + lane, m, nn, nodes, nodes32, order, x, y, z, q, src, ssrc,
-                                        theta2, out_x, out_y, out_z, counts, tsel, rec, rec_cap, rec_cnt, ext);
-    if (sched.cost && lane == 0) sched.cost[chunk] = visits;
-    if (!queued) break;
-    item = take();
-  }
-  if (queued && lane == 0) {
-    __threadfence();
-    const unsigned waves = gridDim.x * (blockDim.x >> 6);
-    if (atomicAdd(&sched.queue[1], 1u) == waves - 1) {
-      sched.queue[0] = 0u;
-      sched.queue[1] = 0u;
-      __threadfence();
-    }
-  }
-}
-
-// Chunks in descending node visits of the previous walk, for the next walk's queue: a counting sort over
-// kOrderBins cost bins in one workgroup.  The order within a bin follows the atomics; any order is a
-// valid schedule, and only the speed depends on it.
-constexpr int kOrderBins = 1024;
-__global__ __launch_bounds__(kOrderBins) void WalkOrderKernel(int n, const uint32_t* __restrict__ cost,
-                                                              uint32_t* __restrict__ order) {
-  __shared__ unsigned hist[kOrderBins];
-  __shared__ unsigned wsum[kOrderBins / 64];
-  __shared__ unsigned cmax;
-  const int t = threadIdx.x;
-  hist[t] = 0u;
-  if (t == 0) cmax = 1u;
-  __syncthreads();
-  unsigned m = 1u;
-  for (int i = t; i < n; i += kOrderBins) m = max(m, cost[i]);
-  atomicMax(&cmax, m);
-  __syncthreads();
-  const unsigned long long cm = cmax;
-  auto bin = [&](int i) {
-    const unsigned long long c = min((unsigned long long)cost[i], cm);
-    return kOrderBins - 1 - (int)(c * (kOrderBins - 1) / cm);
-  };
-  for (int i = t; i < n; i += kOrderBins) atomicAdd(&hist[bin(i)], 1u);
-  __syncthreads();
-  const unsigned h = hist[t];
-  unsigned v = h;  // inclusive scan within the wave, then across the waves' totals
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned y = __shfl_up(v, off, 64);
-    if ((t & 63) >= off) v += y;
-  }
-  if ((t & 63) == 63) wsum[t >> 6] = v;
-  __syncthreads();
-  unsigned base = 0u;
-  for (int w = 0; w < (t >> 6); ++w) base += wsum[w];
-  hist[t] = base + v - h;  // first position of bin t
-  __syncthreads();
-  for (int i = t; i < n; i += kOrderBins) order[atomicAdd(&hist[bin(i)], 1u)] = (uint32_t)i;
-}
-
-// Walk schedule of a store: the queue counters, the cost buffer the walk fills, and the order from the
-// previous walk's costs when it covered the same number of chunks.
-static int WalkSchedule(am_store* s, int64_t n_chunks, WalkSched* out) {
-  am_ctx* ctx = s->ctx;
-  if (s->walk_capacity < n_chunks) {
-    if (s->walk_cost) AM_HIP(hipFree(s->walk_cost));
-    if (s->walk_order) AM_HIP(hipFree(s->walk_order));
-    s->walk_cost = s->walk_order = nullptr;
-    s->walk_capacity = 0;
-    s->walk_chunks = 0;
-    const int64_t cap = n_chunks + n_chunks / 4 + 64;
-    AM_HIP(hipMalloc((void**)&s->walk_cost, (size_t)cap * sizeof(uint32_t)));
-    AM_HIP(hipMalloc((void**)&s->walk_order, (size_t)cap * sizeof(uint32_t)));
-    s->walk_capacity = cap;
-  }
-  int rc = QueueCounters(ctx, "bh_walk_queue", &out->queue);
-  if (rc) return rc;
-  out->cost = s->walk_cost;
-  out->n_chunks = (int)n_chunks;
-  out->order = nullptr;
-  if (s->walk_chunks == n_chunks) {
-    hipLaunchKernelGGL(WalkOrderKernel, dim3(1), dim3(kOrderBins), 0, ctx->stream, (int)n_chunks,
-                       (const uint32_t*)s->walk_cost, s->walk_order);
-    AM_HIP(hipGetLastError());
-    out->order = s->walk_order;
-  }
-  return AM_OK;
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
@@ -729,16 +608,11 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   int slot;
   TimerBegin(ctx, mode == 0 ? "bh_walk_f32" : (mode == 1 ? "bh_walk_f64" : (mode == 2 ? "bh_potential" : "bh_potential_f32")),
              &slot);
-  // A persistent grid of 8 waves per SIMD takes 64-target chunks from a queue, longest first.
-  const int64_t n_chunks = ((int64_t)t.m + 63) / 64;
-  WalkSched sched;
-  if ((rc = WalkSchedule(s, n_chunks, &sched))) return rc;
-  const int64_t slots = (int64_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 8;
-  const dim3 grid((unsigned)std::min<int64_t>(Blocks(t.m, 256), slots)), block(256);
+  const dim3 grid(Blocks(t.m, 256)), block(256);
   auto launch = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z,
                        s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr, nullptr, 0, nullptr,
-                       ExtTargets(), sched);
+                       ExtTargets());
   };
   if (mode == 0) counts ? launch(BHWalkKernel<0, true>) : launch(BHWalkKernel<0, false>);
   else if (mode == 1) counts ? launch(BHWalkKernel<1, true>) : launch(BHWalkKernel<1, false>);
@@ -746,7 +620,6 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
   else counts ? launch(BHWalkKernel<3, true>) : launch(BHWalkKernel<3, false>);
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
-  s->walk_chunks = n_chunks;
   return AM_OK;
 }
 
@@ -756,12 +629,10 @@ int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* 
   am_ctx* ctx = s->ctx;
   int slot;
   TimerBegin(ctx, "bh_walk_f32", &slot);
-  // The recording walk keeps the fixed grid: on the queue, in cost order, its record writes lost their
-  // locality and it ran slower (3.4 -> 4.3 ms at N = 1M, theta = 0.9).
-  hipLaunchKernelGGL((BHWalkKernel<0, false>), dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets,
-                     t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc,
-                     theta * theta, ox, oy, oz, (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt, ExtTargets(),
-                     WalkSched());
+  hipLaunchKernelGGL((BHWalkKernel<0, false>), dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
+                     t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
+                     ox, oy, oz,
+                     (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt, ExtTargets());
   TimerEnd(ctx, slot);
   AM_HIP(hipGetLastError());
   return AM_OK;
@@ -845,8 +716,7 @@ int am_barnes_hut_evaluate(am_store* s, double theta, int precision, int64_t n_p
   auto launch = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, t.nn, t.nodes, t.nodes32, t.order, s->x,
                        s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, out, out + m, out + 2 * (size_t)m,
-                       (int64_t*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0, (uint32_t*)nullptr, ext,
-                       WalkSched());
+                       (int64_t*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0, (uint32_t*)nullptr, ext);
   };
   std::vector<double> h(3 * (size_t)m);
   if (force) {

@@ -322,9 +322,8 @@ int am_store_destroy(am_store* s) {
   if (!s) return AM_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  void* ptrs[] = {s->x,     s->y,     s->z,    s->vx,  s->vy,   s->vz,        s->fx,
-                  s->fy,    s->fz,    s->q,    s->m,   s->species, s->flags,  s->trap,
-                  s->src,   s->perm,  s->walk_cost, s->walk_order};
+  void* ptrs[] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->q, s->m,
+                  s->species, s->flags, s->trap, s->src, s->perm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
