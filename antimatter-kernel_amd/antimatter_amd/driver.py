@@ -47,12 +47,16 @@ def initial_conditions(populations: dict, temperatures: dict, radii, seed: int):
     return pos, vel, species
 
 
-def simulate(params, rng_seed: int = 42, precision: int = FP64, device: int = 0, csv_path: str = None):
+def simulate(params, rng_seed: int = 42, precision: int = FP64, device: int = 0, csv_path: str = None,
+             export_dir: str = None, export_interval: int = 0):
     """Run the reference driver loop on the GPU (the library's am_simulate).
 
     ``params``: a dict of params.json keys, a path to a params.json, or JSON text; it is read with the
     reference's conversions and must pass the reference's Validate checks.  Returns rows[k, 8]:
     step, KE, PE_trap, PE_coul, total, n_active, fractional drift (of e*|U0|), absolute drift.
+    With ``export_dir`` and ``export_interval`` > 0, the ParaView files of every export_interval-th step
+    (the reference's BioDynaMo export, bdm.toml:6-13) go to that directory.
     """
     from .params import simulate as native
-    return native(params, rng_seed=rng_seed, precision=precision, device=device, csv_path=csv_path)
+    return native(params, rng_seed=rng_seed, precision=precision, device=device, csv_path=csv_path,
+                  export_dir=export_dir, export_interval=export_interval)

@@ -36,7 +36,7 @@ class Params(ctypes.Structure):
 
 class RunOptions(ctypes.Structure):
     _fields_ = [("device", _i32), ("precision", _i32), ("rng_seed", _u64), ("print", _i32),
-                ("print_final_positions", _i32)]
+                ("print_final_positions", _i32), ("export_interval", _i32), ("export_dir", ctypes.c_char_p)]
 
 
 class EnergyRow(ctypes.Structure):
@@ -51,6 +51,8 @@ SIGNATURES = {
     "am_params_validate": (ctypes.c_int, [_vp]),
     "am_params_method": (ctypes.c_int, [ctypes.c_char_p, _vp]),
     "am_simulate": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _i64, _vp]),
+    "am_export_vtu": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "am_export_pvd": (ctypes.c_int, [ctypes.c_char_p, _vp, _vp, _i64]),
 }
 
 
@@ -96,18 +98,21 @@ def method_code(name: str) -> int:
 
 
 def simulate(config, rng_seed: int = 42, precision: int = FP64, device: int = 0, csv_path: str = None,
-             echo: bool = False):
+             echo: bool = False, export_dir: str = None, export_interval: int = 0):
     """The reference driver loop on the device for a params.json path, JSON text, dict or Params.
 
     Returns the logged energy rows as an array [k, 8]: step, KE, PE_trap, PE_coul, total, n_active,
-    fractional drift (of e |U0|), absolute drift.  csv_path receives the reference's energy CSV."""
+    fractional drift (of e |U0|), absolute drift.  csv_path receives the reference's energy CSV;
+    with export_dir and export_interval > 0 the ParaView files of every export_interval-th step (and
+    step 0) go to export_dir (an existing directory)."""
     if isinstance(config, Params):
         p = config
     elif isinstance(config, str) and not config.lstrip().startswith("{"):
         p = load_params(config)
     else:
         p = parse_params(config)
-    opt = RunOptions(device, int(precision), int(rng_seed), int(echo), 0)
+    opt = RunOptions(device, int(precision), int(rng_seed), int(echo), 0, int(export_interval),
+                     None if export_dir is None else export_dir.encode())
     cap = int(min(p.num_steps, 1 << 22)) + 2
     rows = (EnergyRow * cap)()
     n = ctypes.c_int64()
@@ -119,3 +124,8 @@ def simulate(config, rng_seed: int = 42, precision: int = FP64, device: int = 0,
         out[k] = (r.step, r.kinetic, r.trap_potential, r.coulomb_potential, r.total, r.num_active_particles,
                   r.fractional_drift, r.absolute_drift)
     return out
+
+
+def export_vtu(store, path: str):
+    """The store's agents still in the simulation as a ParaView .vtu file (am_export_vtu)."""
+    _check(_bind().am_export_vtu(store.h, path.encode()))

@@ -1,9 +1,11 @@
 // am_simulate: the reference executable's main (src/antimatter-kernel.cc:42-220) on the MI355X
 // library.  Reads params.json (or --params FILE), validates it, prints the configuration, runs the
-// simulation on the device and writes the energy CSV into the output directory.
+// simulation on the device and writes the energy CSV into the output directory; with
+// --export-interval K (the reference's bdm.toml exports every step) also the ParaView files of every
+// K-th step into DIR/visualization.
 //
 //   am_simulate [--params FILE] [--output-dir DIR] [--fp32] [--rng-seed N] [--device D]
-//               [--final-positions]
+//               [--final-positions] [--export-interval K]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -46,7 +48,7 @@ void PrintParams(const am_params& p) {
 
 int Usage() {
   std::cerr << "usage: am_simulate [--params FILE] [--output-dir DIR] [--fp32] [--rng-seed N] [--device D] "
-               "[--final-positions]"
+               "[--final-positions] [--export-interval K]"
             << std::endl;
   return 2;
 }
@@ -56,7 +58,8 @@ int Usage() {
 int main(int argc, char** argv) {
   const auto start = std::chrono::high_resolution_clock::now();
   std::string params_path = "params.json", out_dir = "output";
-  am_run_options opt{0, AM_FP64, 42, 1, 0};
+  am_run_options opt{0, AM_FP64, 42, 1, 0, 0, nullptr};
+  std::string export_dir;
   for (int a = 1; a < argc; ++a) {
     const std::string s = argv[a];
     auto next = [&]() -> const char* { return a + 1 < argc ? argv[++a] : nullptr; };
@@ -66,6 +69,7 @@ int main(int argc, char** argv) {
     else if (s == "--rng-seed") { const char* v = next(); if (!v) return Usage(); opt.rng_seed = std::strtoull(v, nullptr, 10); }
     else if (s == "--device") { const char* v = next(); if (!v) return Usage(); opt.device = std::atoi(v); }
     else if (s == "--final-positions") opt.print_final_positions = 1;
+    else if (s == "--export-interval") { const char* v = next(); if (!v) return Usage(); opt.export_interval = std::atoi(v); }
     else return Usage();
   }
   am_params p;
@@ -81,6 +85,11 @@ int main(int argc, char** argv) {
   std::cout << std::endl;
   PrintParams(p);
   mkdir(out_dir.c_str(), 0755);
+  if (opt.export_interval > 0) {  // the ParaView files beside the energy CSV
+    export_dir = out_dir + "/visualization";
+    mkdir(export_dir.c_str(), 0755);
+    opt.export_dir = export_dir.c_str();
+  }
   const std::string csv = out_dir + "/energy_analysis_data.csv";
   int64_t rows = 0;
   if (am_simulate(&p, &opt, csv.c_str(), nullptr, 0, &rows) != AM_OK) {

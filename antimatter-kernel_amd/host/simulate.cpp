@@ -39,7 +39,7 @@ extern "C" int am_simulate(const am_params* p, const am_run_options* opt_in, con
   if (n_rows) *n_rows = 0;
   int rc = am_params_validate(p);
   if (rc) return rc;
-  am_run_options opt{0, AM_FP64, 42, 0, 0};
+  am_run_options opt{0, AM_FP64, 42, 0, 0, 0, nullptr};
   if (opt_in) opt = *opt_in;
   if (opt.precision != AM_FP32 && opt.precision != AM_FP64) return am::Fail(AM_ERR_INVALID, "am_simulate: precision");
   int32_t coulomb_method = 0, energy_method = 0;
@@ -110,6 +110,20 @@ extern "C" int am_simulate(const am_params* p, const am_run_options* opt_in, con
               << " PE_coulomb=" << e0.energy_potential << " TOTAL=" << total0 << " J" << std::endl;
   record(0, e0, total0, 0.0, 0.0);
 
+  // ParaView export (bdm.toml:6-13): the agents after every export_interval-th step, and a collection.
+  std::vector<std::string> vtu_names;
+  std::vector<double> vtu_times;
+  auto export_step = [&](uint64_t step) -> int {
+    const std::string name = "ChargedAntiparticle-" + std::to_string(step) + ".vtu";
+    const int r = am_export_vtu(run.store, (std::string(opt.export_dir) + "/" + name).c_str());
+    if (r) return r;
+    vtu_names.push_back(name);
+    vtu_times.push_back((double)step * p->dt);
+    return AM_OK;
+  };
+  const bool exporting = opt.export_interval > 0 && opt.export_dir;
+  if (exporting && (rc = export_step(0))) return rc;
+
   const int fmm_order = (int)p->fast_multipole_order;  // real -> int (antimatter-kernel.cc:158)
   for (uint64_t step = 1; step <= p->num_steps; ++step) {
     if (p->coulomb_enabled) {
@@ -127,6 +141,7 @@ extern "C" int am_simulate(const am_params* p, const am_run_options* opt_in, con
         (rc = am_annihilation(run.store, p->contact_radius, p->annihilation_probability, opt.rng_seed, step,
                               nullptr)))
       return rc;
+    if (exporting && step % (uint64_t)opt.export_interval == 0 && (rc = export_step(step))) return rc;
     const bool log_step = (step % interval == 0) || (step == p->num_steps);
     const bool need = log_step || ((step + 1) % interval == 0) || (step + 1 == p->num_steps);
     if (!need) continue;
@@ -141,6 +156,13 @@ extern "C" int am_simulate(const am_params* p, const am_run_options* opt_in, con
                 << " PE_coulomb=" << e.energy_potential << " TOTAL=" << total << " J"
                 << "  (drift: " << absd << " J, " << frac << " of trap depth)" << std::endl;
     record((int64_t)step, e, total, frac, absd);
+  }
+  if (exporting) {
+    std::vector<const char*> names;
+    for (const std::string& v : vtu_names) names.push_back(v.c_str());
+    if ((rc = am_export_pvd((std::string(opt.export_dir) + "/ChargedAntiparticle.pvd").c_str(), names.data(),
+                            vtu_times.data(), (int64_t)names.size())))
+      return rc;
   }
   if (csv.is_open()) {
     csv.close();

@@ -74,6 +74,11 @@ typedef struct {
   uint64_t rng_seed;     /* key of the counter-based draws (annihilation, cooling noise) */
   int32_t print;         /* the reference's stdout lines (step energies, drift) */
   int32_t print_final_positions; /* "final position: (x, y, z)" per particle (antimatter-kernel.cc:204-210) */
+  /* ParaView export (bdm.toml:6-13 turns BioDynaMo's on every step): every export_interval steps
+   * (0: off) the agents go to export_dir/ChargedAntiparticle-<step>.vtu, step 0 included, and
+   * export_dir/ChargedAntiparticle.pvd lists them with their times. */
+  int32_t export_interval;
+  const char* export_dir;
 } am_run_options;
 
 /* One row of the energy log: the 8 CSV columns of antimatter-kernel.cc:128. */
@@ -92,6 +97,14 @@ typedef struct {
  * am_params_validate. */
 int am_simulate(const am_params* p, const am_run_options* opt, const char* csv_path, am_energy_row* rows,
                 int64_t rows_cap, int64_t* n_rows);
+
+/* The agents still in the simulation (not annihilated) as a VTK XML UnstructuredGrid of vertex cells:
+ * the points are the positions (m), with the point data species_id (charged_antiparticle.h:73, the
+ * data member bdm.toml:13 adds to the export) and diameter_ (1e-6 m, constants.h:38), the content of
+ * BioDynaMo's ParaView export of ChargedAntiparticle.  ASCII, positions to 17 significant digits. */
+int am_export_vtu(am_store* store, const char* path);
+/* A .pvd collection of count exported files (names relative to the .pvd) with their times (s). */
+int am_export_pvd(const char* path, const char* const* files, const double* times, int64_t count);
 
 #ifdef __cplusplus
 }

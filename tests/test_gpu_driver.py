@@ -70,3 +70,54 @@ def test_executable_writes_the_same_csv(tmp_path):
     res = subprocess.run([exe, "--params", str(bad), "--output-dir", str(outdir)], capture_output=True, text=True,
                          timeout=60)
     assert res.returncode == 1 and "barnes_hut" in res.stderr
+
+
+def read_vtu(path):
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(path).getroot().find("UnstructuredGrid/Piece")
+    arrays = {a.get("Name"): a for a in piece.iter("DataArray")}
+    npts = int(piece.get("NumberOfPoints"))
+    pts = np.array(piece.find("Points/DataArray").text.split(), dtype=np.float64).reshape(npts, 3)
+    sid = np.array(arrays["species_id"].text.split(), dtype=np.int64)
+    diam = np.array(arrays["diameter_"].text.split(), dtype=np.float64)
+    conn = np.array(arrays["connectivity"].text.split(), dtype=np.int64)
+    types = np.array(arrays["types"].text.split(), dtype=np.int64)
+    return pts, sid, diam, conn, types
+
+
+def test_paraview_export(tmp_path):
+    """The agents of every 10th step as VTK XML (the reference's BioDynaMo export, bdm.toml:6-13, with
+    species_id and the 1e-6 m diameter), and a .pvd collection of them."""
+    import xml.etree.ElementTree as ET
+    from antimatter_amd import params as P
+    rows = am.simulate(PARAMS, export_dir=str(tmp_path), export_interval=10)
+    cfg = P.load_params(PARAMS)
+    coll = ET.parse(tmp_path / "ChargedAntiparticle.pvd").getroot().find("Collection")
+    sets = coll.findall("DataSet")
+    assert [d.get("file") for d in sets] == [f"ChargedAntiparticle-{k}.vtu" for k in range(0, 101, 10)]
+    np.testing.assert_allclose([float(d.get("timestep")) for d in sets], np.arange(0, 101, 10) * cfg.dt, rtol=1e-15)
+    # step 0: the reference's initial conditions, 100 positrons (species 1) and 100 electrons (3)
+    pos0, _, sp0 = am.initial_conditions({"positron": 100, "electron": 100}, {"positron": 1000.0, "electron": 1000.0},
+                                         (1e-4, 1e-4, 1e-4), 42)
+    pts, sid, diam, conn, types = read_vtu(tmp_path / "ChargedAntiparticle-0.vtu")
+    np.testing.assert_array_equal(pts, pos0)  # 17 significant digits round-trip exactly
+    np.testing.assert_array_equal(sid, sp0)
+    assert np.all(diam == 1e-6)
+    np.testing.assert_array_equal(conn, np.arange(200))
+    assert np.all(types == 1)  # VTK_VERTEX
+    # later steps hold the agents still in the simulation: the logged active counts, pairs removed
+    for k in range(10, 101, 10):
+        pts, sid, _, _, _ = read_vtu(tmp_path / f"ChargedAntiparticle-{k}.vtu")
+        assert len(pts) == int(rows[k, 5]) == len(sid)
+        assert (sid == 1).sum() == (sid == 3).sum()  # annihilation removes e+ e- pairs
+        assert np.all(np.isfinite(pts)) and np.all(np.abs(pts) < 1e-2)
+    assert len(read_vtu(tmp_path / "ChargedAntiparticle-100.vtu")[0]) == 170
+    # the executable's --export-interval writes the same files into <output-dir>/visualization
+    exe = os.path.join(ROOT, "antimatter-kernel_amd", "am_simulate")
+    outdir = tmp_path / "out"
+    res = subprocess.run([exe, "--params", PARAMS, "--output-dir", str(outdir), "--export-interval", "50"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    for k in (0, 50, 100):
+        name = f"ChargedAntiparticle-{k}.vtu"
+        assert open(outdir / "visualization" / name).read() == open(tmp_path / name).read()
