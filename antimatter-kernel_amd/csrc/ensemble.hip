@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kEnsTile) void EnsTilePrepKernel(const int* __restr
   TilePrepBody(T * kEnsTile, nvalid, T, perm, x, y, z, src, shl, sloc, tiles);
 }
 
-__global__ __launch_bounds__(kEnsThreads) void EnsembleForceKernel(const EnsItem* __restrict__ items, int n_items,
+__global__ __launch_bounds__(kEnsThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void EnsembleForceKernel(const EnsItem* __restrict__ items, int n_items,
                                                                    unsigned* __restrict__ counter,
                                                                    const TrapRec* __restrict__ traps,
                                                                    int tiles_per_chunk,

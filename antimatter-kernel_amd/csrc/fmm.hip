@@ -394,12 +394,12 @@ __global__ __launch_bounds__(64) void M2MKernel(int n0, NodeArrays nd, const uin
   }
 }
 
-// One expansion step of the dual-tree traversal (fast-multipole.cc:494-525).  Pair i of the
-// frontier is accepted for M2L, kept for P2P, or split; cnt[i], cnt[np + i] and cnt[2 np + i] count
-// its M2L pair, its P2P pair and its child pairs, so one scan over cnt places all three outputs.
-// The decision reads the traversal records trv[c] = (centre, radius) and tinfo[c] = leaf | nchild << 1
-// (72 bytes per pair instead of two whole cell records); children come from the cell record only
-// for split pairs.
+// One expansion step of the dual-tree traversal (fast-multipole.cc:494-525): pair i of the frontier
+// is accepted for M2L, kept for P2P, or split.  The tile kernels below classify each 1024-pair tile's
+// pairs and count its three outputs, scan the tile counts, and write every output at its tile's
+// offset plus a workgroup scan.  The decision reads the traversal records trv[c] = (centre, radius)
+// and tinfo[c] = leaf | nchild << 1 (72 bytes per pair instead of two whole cell records); children
+// come from the cell record only for split pairs.
 __device__ inline int TraverseKind(const double4 T, const double4 S, uint32_t ti, uint32_t si, double theta2) {
   const double dx = T.x - S.x, dy = T.y - S.y, dz = T.z - S.z;
   const double dist2 = dx * dx + dy * dy + dz * dz;
