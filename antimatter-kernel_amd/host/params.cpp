@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -33,6 +34,30 @@ int SpeciesOfName(const std::string& name) {
   throw std::invalid_argument("params.json: unknown species '" + name + "'");
 }
 
+// nlohmann's get<size_t>() / get<int>() on a floating-point value is a static_cast, which is
+// undefined when the target cannot hold the value's integer part (1e300 steps, a seed of 1e20, a
+// temperature of -2 K through size_t).  Such a value is rejected here; an in-range value converts
+// exactly as the reference's get<T>() does (truncation toward zero), and integer values keep
+// nlohmann's integer conversion.
+template <typename T>
+T IntegerOf(const nlohmann::json& v, const char* key) {
+  if (v.is_number_float()) {
+    const double d = v.get<double>();
+    const double lo = (double)std::numeric_limits<T>::min() - 1.0;
+    const double hi = (double)std::numeric_limits<T>::max() + 1.0;  // 2^64 or 2^31, exact in FP64
+    if (!(d > lo && d < hi))
+      throw std::invalid_argument(std::string("params.json: ") + key + " = " + v.dump() +
+                                  " is outside the range of its integer type");
+  }
+  return v.get<T>();
+}
+
+template <typename T>
+T IntegerValue(const nlohmann::json& j, const char* key, T fallback) {
+  if (!j.is_object()) throw std::invalid_argument("params.json: the top level is not an object");
+  return j.count(key) ? IntegerOf<T>(j.at(key), key) : fallback;
+}
+
 void CopyName(char* dst, const std::string& s, const char* key) {
   if (s.size() >= AM_PARAMS_METHOD_LEN)
     throw std::invalid_argument(std::string("params.json: ") + key + " '" + s + "' is not a known method");
@@ -40,7 +65,7 @@ void CopyName(char* dst, const std::string& s, const char* key) {
 }
 
 void ReadPhysics(const nlohmann::json& j, am_params* p) {
-  p->seed = j.value("seed", p->seed);
+  p->seed = IntegerValue<int>(j, "seed", p->seed);
   p->b0 = j.value("b0", p->b0);
   p->trap_voltage = j.value("trap_voltage", p->trap_voltage);
   p->z0 = j.value("z0", p->z0);
@@ -56,7 +81,7 @@ void ReadPhysics(const nlohmann::json& j, am_params* p) {
     const nlohmann::json& pops = j.at("particle_populations");
     for (auto it = pops.begin(); it != pops.end(); ++it) {
       const int s = SpeciesOfName(it.key());
-      p->populations[s] = (uint64_t)it.value().get<size_t>();
+      p->populations[s] = (uint64_t)IntegerOf<size_t>(it.value(), "particle_populations");
       p->population_listed[s] = 1;
       double m, q;
       am_species_properties(s, &m, &q);
@@ -72,11 +97,11 @@ void ReadPhysics(const nlohmann::json& j, am_params* p) {
     const nlohmann::json& temps = j.at("particle_temperatures");
     for (auto it = temps.begin(); it != temps.end(); ++it) {
       const int s = SpeciesOfName(it.key());
-      p->temperatures[s] = (double)it.value().get<size_t>();
+      p->temperatures[s] = (double)IntegerOf<size_t>(it.value(), "particle_temperatures");
       p->temperature_listed[s] = 1;
     }
   }
-  p->num_steps = (uint64_t)j.value("num_steps", (size_t)p->num_steps);
+  p->num_steps = (uint64_t)IntegerValue<size_t>(j, "num_steps", (size_t)p->num_steps);
   if (j.count("initial_cloud_radii")) {
     const nlohmann::json& r = j.at("initial_cloud_radii");
     const char* keys[3] = {"radius_x", "radius_y", "radius_z"};

@@ -62,6 +62,18 @@ def test_temperatures_and_counts_go_through_size_t():
     assert p.temperatures[1] == 1000.0 and p.temperatures[3] == 3.0 and p.num_steps == 250
 
 
+@pytest.mark.parametrize("key, value", [("num_steps", 1e300), ("num_steps", -1.5), ("seed", 1e20),
+                                        ("particle_temperatures", {"electron": -2.5}),
+                                        ("particle_populations", {"positron": 1e30})])
+def test_out_of_range_integers_are_rejected(key, value):
+    # nlohmann's get<size_t>/get<int> would static_cast these (undefined behaviour); the loader
+    # rejects them instead
+    cfg = shipped()
+    cfg[key] = value
+    with pytest.raises(AmError, match="outside the range"):
+        P.parse_params(cfg)
+
+
 def test_dt_rule():
     cfg = shipped()
     cfg["particle_populations"] = {"antiproton": 10, "antihydrogen": 5}
