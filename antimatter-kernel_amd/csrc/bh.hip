@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "bh.hpp"
+#include "pairtile.hpp"
 
 namespace am {
 
@@ -214,6 +215,22 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
   w.cy = nd.cy[node];
   w.cz = nd.cz[node];
   w.half = nd.half[node];
+  {  // FP32 form of the acceptance test's operands (units of 2^-20 m, exact power-of-two scaling)
+    const double c[3] = {w.cx * kUnitScale, w.cy * kUnitScale, w.cz * kUnitScale};
+    float hi[3], lo[3];
+    for (int d = 0; d < 3; ++d) {
+      hi[d] = (float)c[d];
+      lo[d] = (float)(c[d] - (double)hi[d]);
+    }
+    w.chx = hi[0];
+    w.chy = hi[1];
+    w.chz = hi[2];
+    w.clx = lo[0];
+    w.cly = lo[1];
+    w.clz = lo[2];
+    const double wu = 2.0 * w.half * kUnitScale;
+    w.w2 = (float)(wu * wu);
+  }
   w.pqx = mo.cpx[node];
   w.pqy = mo.cpy[node];
   w.pqz = mo.cpz[node];
@@ -225,6 +242,15 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
   w.b = nd.b[node];
   w.e = nd.e[node];
   w.leaf = nd.leaf[node];
+  // A twig is an internal node whose children are all leaves: a lane that opens it sums the
+  // children's particles directly, in the children's order, which is what visiting them would do.
+  w.twig = 0;
+  if (!w.leaf) {
+    const int c0 = nd.first_child[node], c1 = c0 + nd.nchild[node];
+    int all = c1 > c0 ? 1 : 0;
+    for (int ch = c0; ch < c1; ++ch) all &= nd.leaf[ch] ? 1 : 0;
+    w.twig = all;
+  }
   // skip = first preorder position whose key >= (e << 5)
   const uint64_t target = (uint64_t)w.e << 5;
   int lo = p + 1, hi = nn;
@@ -246,6 +272,8 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.p_hi = make_float4(xh, yh, zh, (float)(w.qp * qs));
     f.p_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
+    f.p_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
+                          (float)(zs - w.cz * kUnitScale), (float)(w.qp * qs));
   }
   {
     const bool has = w.qn > 0.0;
@@ -254,6 +282,8 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.n_hi = make_float4(xh, yh, zh, (float)(-w.qn * qs));
     f.n_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
+    f.n_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
+                          (float)(zs - w.cz * kUnitScale), (float)(-w.qn * qs));
   }
   out32[p] = f;
 }
@@ -300,6 +330,33 @@ __device__ __forceinline__ float PotF32(const float4 th, const float4 tl, const 
   return w * __builtin_amdgcn_rsqf(r2);
 }
 
+// FP32 potential of a source given by its offset s (units) from a cube centre, for a target at
+// offset (dx, dy, dz) from the same centre; the same softening and self floor as PotF32.
+__device__ __forceinline__ float PotLocal(float dx, float dy, float dz, const float4 s) {
+  const float rx = dx - s.x, ry = dy - s.y, rz = dz - s.z;
+  const float r2 = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, kEps2Units)));
+  const float w = r2 < 2.f * kEps2Units ? 0.f : s.w;
+  return w * __builtin_amdgcn_rsqf(r2);
+}
+
+// The acceptance test (2h)^2 < theta^2 |x - c|^2 (barnes-hut.cc:201-204) in FP32.  With the target
+// and the cube centre as hi/lo pairs in units of 2^-20 m, each FP32 difference is within
+// 2^-23 |d| + delta of the exact one, delta = 2^-44 R (R bounds |coordinate| in units), and
+// theta^2 |d|^2 then lies within 2^-19 relative of the FP64 side wherever |d| >= w_min / max(1, theta)
+// (w_min = the smallest internal cube width, 2^-19 root half widths at depth <= 20) and
+// 3.5 delta max(1, theta) / w_min <= 2^-20, i.e. R max(1, theta) <= 8 root half widths.  Outside a
+// relative band of 2^-16 the FP32 verdict is therefore the FP64 one; inside it the walk evaluates the
+// FP64 test.  FastMacRange checks that bound (and keeps the units^2 values in FP32's normal range).
+constexpr float kMacSureBelow = 1.0f - 0x1p-16f;
+constexpr float kMacSureAbove = 1.0f + 0x1p-16f;
+
+__device__ inline bool FastMacRange(const RootBox* __restrict__ root, double theta2) {
+  const RootBox r = *root;
+  const double h = r.half * kUnitScale;
+  const double R = (fmax(fmax(fabs(r.cx), fabs(r.cy)), fabs(r.cz)) + r.half) * kUnitScale;
+  return h >= 0x1p-30 && h <= 0x1p60 && R * R * fmax(1.0, theta2) <= 64.0 * h * h;
+}
+
 // MODE 0: FP32 force; MODE 1: FP64 force; MODE 2: FP64 potential (q_i phi_i into out_x); MODE 3:
 // FP32 potential ((q_i / e) phi_i in e^2/unit into out_x, FP32 per leaf, FP64 across nodes).  COUNT:
 // interactions are counted into counts (a compile-time switch, so the plain walk carries no counting
@@ -316,6 +373,7 @@ struct ExtTargets {
 template <int MODE, bool COUNT, bool EXT = false>
 __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes,
                                                      const WalkNodeF32* __restrict__ nodes32,
+                                                     const RootBox* __restrict__ root,
                                                      const uint32_t* __restrict__ order,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, const double* __restrict__ q,
@@ -364,6 +422,50 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
   int next = want ? 0 : nn;
   double ax = 0.0, ay = 0.0, az = 0.0;
   int64_t cnt = 0;
+  // Store targets decide the acceptance test in FP32 where the root cube allows it (FastMacRange);
+  // external points may lie anywhere and always take the FP64 test.
+  const bool fast = !EXT && FastMacRange(root, theta2);
+  const float theta2f = (float)theta2;
+  // Direct sum over sorted positions [b, e): a leaf's particles, or a twig's (its leaf children's,
+  // in their order).
+  auto direct = [&](int b, int e) {
+    if (MODE == 3) {
+      float ph = 0.f;
+      for (int kk = b; kk < e; ++kk)
+        if (!EXT || order[kk] != i) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
+      ax += ph;
+      if (COUNT) for (int kk = b; kk < e; ++kk) cnt += (ssrc[2 * kk].w != 0.f && kk != ks) ? 1 : 0;
+    } else if (MODE == 0) {
+      // The range's sources are contiguous in the tree-ordered copy of the FP32 mirror.  The target
+      // itself and charge-0 sources add exactly 0 (softened pair, q = 0), so they are not tested; the
+      // interaction count leaves them out.
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      for (int kk = b; kk < e; ++kk) {
+        const float4 sh = ssrc[2 * kk];
+        if (EXT && order[kk] == i) continue;  // the point's self source, at any position
+        PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
+        if (COUNT) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
+      }
+      ax += fx;
+      ay += fy;
+      az += fz;
+    } else {
+      for (int kk = b; kk < e; ++kk) {
+        const uint32_t j = order[kk];
+        if (j == i) continue;
+        const double qj = q[j];
+        if (qj == 0.0) continue;
+        if (MODE == 1) {
+          PairF64Acc(px, py, pz, qi, x[j], y[j], z[j], qj, ax, ay, az);
+        } else {
+          const double rx = __dsub_rn(px, x[j]), ry = __dsub_rn(py, y[j]), rz = __dsub_rn(pz, z[j]);
+          const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
+          if (d2 >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(qj, sqrt(d2)));
+        }
+        if (COUNT) ++cnt;
+      }
+    }
+  };
   int c = __builtin_amdgcn_readfirstlane(WaveMin(next));  // wave-uniform: node records come in by scalar loads
   while (c < nn) {
     const WalkNode nd = nodes[c];
@@ -373,57 +475,35 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
       if (nd.qp == 0.0 && nd.qn == 0.0) {
         next = nd.skip;
       } else if (nd.leaf) {
-        if (MODE == 3) {
-          float ph = 0.f;
-          for (int kk = nd.b; kk < nd.e; ++kk)
-            if (!EXT || order[kk] != i) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
-          ax += ph;
-          if (COUNT) for (int kk = nd.b; kk < nd.e; ++kk) cnt += (ssrc[2 * kk].w != 0.f && kk != ks) ? 1 : 0;
-        } else if (MODE == 0) {
-          // The leaf's sources are contiguous in the tree-ordered copy of the FP32 mirror.  The target
-          // itself and charge-0 sources add exactly 0 (softened pair, q = 0), so they are not
-          // tested; the interaction count leaves them out.
-          float fx = 0.f, fy = 0.f, fz = 0.f;
-          for (int kk = nd.b; kk < nd.e; ++kk) {
-            const float4 sh = ssrc[2 * kk];
-            if (EXT && order[kk] == i) continue;  // the point's self source, at any position
-            PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
-            if (COUNT) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
-          }
-          ax += fx;
-          ay += fy;
-          az += fz;
-        } else {
-          for (int kk = nd.b; kk < nd.e; ++kk) {
-            const uint32_t j = order[kk];
-            if (j == i) continue;
-            const double qj = q[j];
-            if (qj == 0.0) continue;
-            if (MODE == 1) {
-              PairF64Acc(px, py, pz, qi, x[j], y[j], z[j], qj, ax, ay, az);
-            } else {
-              const double rx = __dsub_rn(px, x[j]), ry = __dsub_rn(py, y[j]), rz = __dsub_rn(pz, z[j]);
-              const double d2 = Dot3(rx, ry, rz, rx, ry, rz);
-              if (d2 >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(qj, sqrt(d2)));
-            }
-            if (COUNT) ++cnt;
-          }
-        }
+        direct(nd.b, nd.e);
         next = nd.skip;
       } else {
-        const double dx = __dsub_rn(px, nd.cx), dy = __dsub_rn(py, nd.cy), dz = __dsub_rn(pz, nd.cz);
-        const double d2 = Dot3(dx, dy, dz, dx, dy, dz);
-        const double w = __dmul_rn(2.0, nd.half);
-        if (__dmul_rn(w, w) < __dmul_rn(theta2, d2)) {  // accepted (barnes-hut.cc:201-215)
+        // The target's offset from the cube centre in FP32 (hi/lo): the fast acceptance test's
+        // distance and the accepted monopoles' differences.
+        const float dxc = (th.x - nd.chx) + (tl.x - nd.clx);
+        const float dyc = (th.y - nd.chy) + (tl.y - nd.cly);
+        const float dzc = (th.z - nd.chz) + (tl.z - nd.clz);
+        int verdict = -1;
+        if (fast) {
+          const float r = theta2f * fmaf(dxc, dxc, fmaf(dyc, dyc, dzc * dzc));
+          verdict = nd.w2 < r * kMacSureBelow ? 1 : (nd.w2 > r * kMacSureAbove ? 0 : -1);
+        }
+        if (verdict < 0) {  // the reference's test (barnes-hut.cc:201-204) in FP64 without contraction
+          const double dx = __dsub_rn(px, nd.cx), dy = __dsub_rn(py, nd.cy), dz = __dsub_rn(pz, nd.cz);
+          const double d2 = Dot3(dx, dy, dz, dx, dy, dz);
+          const double w = __dmul_rn(2.0, nd.half);
+          verdict = __dmul_rn(w, w) < __dmul_rn(theta2, d2) ? 1 : 0;
+        }
+        if (verdict) {  // accepted (barnes-hut.cc:201-215)
           if (MODE == 3) {
             const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
-            ax += PotF32(th, tl, f.p_hi, f.p_lo) + PotF32(th, tl, f.n_hi, f.n_lo);
+            ax += PotLocal(dxc, dyc, dzc, f.p_rel) + PotLocal(dxc, dyc, dzc, f.n_rel);
             if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
           } else if (MODE == 0) {
             const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            PairF32(th, tl, f.p_hi, f.p_lo, fx, fy, fz);
-            PairF32(th, tl, f.n_hi, f.n_lo, fx, fy, fz);
+            PairLocal(dxc, dyc, dzc, f.p_rel, fx, fy, fz);
+            PairLocal(dxc, dyc, dzc, f.n_rel, fx, fy, fz);
             if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
             ax += fx;
             ay += fy;
@@ -450,6 +530,9 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
             }
           }
           next = nd.skip;
+        } else if (nd.twig) {
+          direct(nd.b, nd.e);  // opened; its children are leaves, summed here in their order
+          next = nd.skip;
         } else {
           next = c + 1;
           opened = true;
@@ -460,7 +543,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
     // only by opening c, and the wave visits c once (its node only increases), so every lane
     // that was not at c already points at skip(c) or beyond, and every lane at c now points at
     // c + 1 (opened) or skip(c).  The minimum is therefore c + 1 if any lane opened c, else
-    // skip(c): no reduction over the lanes is needed.
+    // skip(c): no reduction over the lanes is needed.  (A lane that opens a twig sums its
+    // children at once and moves to skip(c).)
     c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : nd.skip);
   }
   if (!valid) return;
@@ -585,6 +669,7 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   t->order = o.order;
   t->nodes = (WalkNode*)wn;
   t->nodes32 = (WalkNodeF32*)wn32;
+  t->root = (const RootBox*)root;
   t->max_depth = nlevels - 1;
   t->deep_overflow = o.deep_overflow;
   return AM_OK;
@@ -610,7 +695,7 @@ int BarnesHutWalk(am_store* s, double theta, int mode, double* ox, double* oy, d
              &slot);
   const dim3 grid(Blocks(t.m, 256)), block(256);
   auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.order, s->x, s->y, s->z,
+    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, t.m, t.nn, t.nodes, t.nodes32, t.root, t.order, s->x, s->y, s->z,
                        s->q, s->src, (const float4*)t.ssrc, theta2, ox, oy, oz, counts, nullptr, nullptr, 0, nullptr,
                        ExtTargets());
   };
@@ -630,7 +715,7 @@ int LaunchBHWalkF32(am_store* s, const BHTree& t, double theta, const uint32_t* 
   int slot;
   TimerBegin(ctx, "bh_walk_f32", &slot);
   hipLaunchKernelGGL((BHWalkKernel<0, false>), dim3(Blocks(n_targets, 256)), dim3(256), 0, ctx->stream, n_targets, t.nn,
-                     t.nodes, t.nodes32, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
+                     t.nodes, t.nodes32, t.root, t.order, s->x, s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta * theta,
                      ox, oy, oz,
                      (int64_t*)nullptr, tsel, rec, rec_cap, rec_cnt, ExtTargets());
   TimerEnd(ctx, slot);
@@ -714,7 +799,7 @@ int am_barnes_hut_evaluate(am_store* s, double theta, int precision, int64_t n_p
   ext.self = dself;
   const double theta2 = theta * theta;
   auto launch = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, t.nn, t.nodes, t.nodes32, t.order, s->x,
+    hipLaunchKernelGGL(kernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, t.nn, t.nodes, t.nodes32, t.root, t.order, s->x,
                        s->y, s->z, s->q, s->src, (const float4*)t.ssrc, theta2, out, out + m, out + 2 * (size_t)m,
                        (int64_t*)nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, 0, (uint32_t*)nullptr, ext);
   };

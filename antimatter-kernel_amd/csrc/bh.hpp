@@ -16,16 +16,22 @@ struct FeatNode {
 
 struct WalkNode {
   // The first 64 bytes are what the walk reads at every visit (one scalar fetch).
-  double cx, cy, cz, half;     // geometric cube
-  double qp, qn;               // positive charge and |negative charge|
-  int b, e, skip, leaf;        // particle range in the sorted order, subtree skip pointer
-  double pqx, pqy, pqz;        // positive charge centre
-  double nqx, nqy, nqz;        // negative charge centre
+  float chx, chy, chz;  // cube centre in units of 2^-20 m, FP32 hi parts
+  float w2;             // (2 half)^2 in units^2, rounded to FP32: the acceptance test's left side
+  float clx, cly, clz;  // lo parts: centre = hi + lo to ~2^-48 relative
+  int twig;             // internal node whose children are all leaves
+  double qp, qn;        // positive charge and |negative charge|
+  int b, e, skip, leaf; // particle range in the sorted order, subtree skip pointer
+  double pqx, pqy, pqz; // positive charge centre
+  double nqx, nqy, nqz; // negative charge centre
+  double cx, cy, cz, half;  // geometric cube in FP64 (the exact acceptance test)
 };
-struct WalkNodeF32 {  // FP32 hi/lo mirror of the monopoles in units (charge in e)
-  float4 p_hi, p_lo, n_hi, n_lo;
+struct WalkNodeF32 {  // FP32 monopoles, charge in e
+  // Charge centre minus cube centre in units of 2^-20 m (|offset| <= half width, so FP32 keeps it to
+  // 2^-24 of the half width); an empty class is (0, 0, 0, charge 0).  The walk reads these 32 bytes.
+  float4 p_rel, n_rel;
+  float4 p_hi, p_lo, n_hi, n_lo;  // hi/lo charge centres (learned-correction residuals)
 };
-
 
 struct BHTree {
   Octree tree;
@@ -34,6 +40,7 @@ struct BHTree {
   uint32_t* order = nullptr;  // sorted position -> particle index
   WalkNode* nodes = nullptr;
   WalkNodeF32* nodes32 = nullptr;
+  const RootBox* root = nullptr;  // device: the root cube (the FP32 acceptance test's range check)
   FeatNode* feat = nullptr;  // only when built with features
   float4* ssrc = nullptr;    // FP32 hi/lo mirror in tree order (2 per sorted position)
   int max_depth = 0;

@@ -85,6 +85,48 @@ def test_fp64_forces_and_counts_bit_identical(store, theta):
     np.testing.assert_array_equal(f[live], ref[live])
 
 
+def lattice_cloud(shift=(0.0, 0.0, 0.0)):
+    # 17^3 particles on a dyadic lattice (spacing 2^-14 m, centred on the origin), alternating e+/e-.
+    # The root cube is the bounding box widened by (1 + 1e-9), so cube centres and widths are lattice
+    # points up to that factor: 500-1200 (target, internal cube) pairs per theta lie within 1e-8
+    # relative of the acceptance threshold, inside the FP32 test's undecided band of 2^-16, where
+    # the FP64 test decides.
+    g = np.arange(-8, 9) * 2.0 ** -14
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + np.asarray(shift)
+    sp = np.where(np.arange(len(pos)) % 2 == 0, 1, 3).astype(np.uint8)
+    return pos, sp
+
+
+@pytest.mark.parametrize("theta", [0.25, 0.5, 1.0])
+def test_acceptance_near_ties_on_a_lattice(store, theta):
+    pos, sp = lattice_cloud()
+    store.upload(pos, np.zeros_like(pos), sp)
+    m, q = oracle.mass_charge(sp)
+    ref, rcnt = oracle.bh_forces(pos, q, None, theta, 8, want_counts=True)
+    cnt64 = store.barnes_hut_forces_counted(theta, am.FP64)
+    np.testing.assert_array_equal(cnt64, rcnt)
+    np.testing.assert_array_equal(store.forces(), ref)
+    cnt32 = store.barnes_hut_forces_counted(theta, am.FP32)
+    np.testing.assert_array_equal(cnt32, rcnt)
+    assert rel_l2(store.forces(), ref) < 1e-5
+
+
+def test_cloud_far_from_the_origin_takes_the_fp64_test(store):
+    # 1 m away from the origin the FP32 test's range check fails (the coordinates exceed 8 root
+    # half widths), so every acceptance test is the FP64 one; the interaction sets stay the oracle's.
+    pos, sp, ann = cloud(4000, seed=24)
+    pos = pos + np.array([1.0, -0.5, 0.25])
+    store.upload(pos, np.zeros_like(pos), sp, annihilated=ann)
+    m, q = oracle.mass_charge(sp)
+    ref, rcnt = oracle.bh_forces(pos, q, ann, 0.6, 8, want_counts=True)
+    live = ann == 0
+    cnt = store.barnes_hut_forces_counted(0.6, am.FP64)
+    np.testing.assert_array_equal(cnt[live], rcnt[live])
+    np.testing.assert_array_equal(store.forces()[live], ref[live])
+    cnt = store.barnes_hut_forces_counted(0.6, am.FP32)
+    np.testing.assert_array_equal(cnt[live], rcnt[live])
+
+
 def test_fp32_theta0_is_allpairs_and_theta_error(store):
     pos, sp, ann = cloud(8000, seed=23, frac_neutral=0.0, frac_dead=0.0)
     store.upload(pos, np.zeros_like(pos), sp)
