@@ -187,7 +187,7 @@ __global__ void SecondMomentKernel(int n0, int n1, NodeArrays nd, Moments mo, Se
 
 __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const uint64_t* __restrict__ skeys,
                                      const uint32_t* __restrict__ pre, WalkNode* __restrict__ out,
-                                     WalkNodeF32* __restrict__ out32, SecondMoments sm,
+                                     WalkNodeAux* __restrict__ out32, SecondMoments sm,
                                      FeatNode* __restrict__ feat) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nn) return;
@@ -211,6 +211,7 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     feat[p] = f;
   }
   WalkNode w;
+  WalkNodeAux f;
   w.cx = nd.cx[node];
   w.cy = nd.cy[node];
   w.cz = nd.cz[node];
@@ -231,26 +232,29 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     const double wu = 2.0 * w.half * kUnitScale;
     w.w2 = (float)(wu * wu);
   }
-  w.pqx = mo.cpx[node];
-  w.pqy = mo.cpy[node];
-  w.pqz = mo.cpz[node];
+  f.pqx = mo.cpx[node];
+  f.pqy = mo.cpy[node];
+  f.pqz = mo.cpz[node];
   w.qp = mo.qp[node];
-  w.nqx = mo.cnx[node];
-  w.nqy = mo.cny[node];
-  w.nqz = mo.cnz[node];
+  f.nqx = mo.cnx[node];
+  f.nqy = mo.cny[node];
+  f.nqz = mo.cnz[node];
   w.qn = mo.qn[node];
   w.b = nd.b[node];
   w.e = nd.e[node];
   w.leaf = nd.leaf[node];
   // A twig is an internal node whose children are all leaves: a lane that opens it sums the
   // children's particles directly, in the children's order, which is what visiting them would do.
-  w.twig = 0;
+  uint32_t flags = (w.qp == 0.0 && w.qn == 0.0) ? kNodeEmpty : 0u;
+  if (w.qp > 0.0) flags |= kNodeHasPos;
+  if (w.qn > 0.0) flags |= kNodeHasNeg;
   if (!w.leaf) {
     const int c0 = nd.first_child[node], c1 = c0 + nd.nchild[node];
     int all = c1 > c0 ? 1 : 0;
     for (int ch = c0; ch < c1; ++ch) all &= nd.leaf[ch] ? 1 : 0;
-    w.twig = all;
+    if (all) flags |= kNodeTwig;
   }
+  w.flags = flags;
   // skip = first preorder position whose key >= (e << 5)
   const uint64_t target = (uint64_t)w.e << 5;
   int lo = p + 1, hi = nn;
@@ -260,31 +264,30 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     else hi = mid;
   }
   w.skip = lo;
-  out[p] = w;
   // FP32 monopoles; an empty charge class sits at the cell centre with charge 0, so the walk can
   // sum both monopoles of an accepted node without testing (a charge-0 pair adds exactly 0).
-  WalkNodeF32 f;
   const double qs = 1.0 / kElementaryCharge;
   {
     const bool has = w.qp > 0.0;
-    const double xs = (has ? w.pqx : w.cx) * kUnitScale, ys = (has ? w.pqy : w.cy) * kUnitScale,
-                 zs = (has ? w.pqz : w.cz) * kUnitScale;
+    const double xs = (has ? f.pqx : w.cx) * kUnitScale, ys = (has ? f.pqy : w.cy) * kUnitScale,
+                 zs = (has ? f.pqz : w.cz) * kUnitScale;
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.p_hi = make_float4(xh, yh, zh, (float)(w.qp * qs));
     f.p_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
-    f.p_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
+    w.p_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
                           (float)(zs - w.cz * kUnitScale), (float)(w.qp * qs));
   }
   {
     const bool has = w.qn > 0.0;
-    const double xs = (has ? w.nqx : w.cx) * kUnitScale, ys = (has ? w.nqy : w.cy) * kUnitScale,
-                 zs = (has ? w.nqz : w.cz) * kUnitScale;
+    const double xs = (has ? f.nqx : w.cx) * kUnitScale, ys = (has ? f.nqy : w.cy) * kUnitScale,
+                 zs = (has ? f.nqz : w.cz) * kUnitScale;
     const float xh = (float)xs, yh = (float)ys, zh = (float)zs;
     f.n_hi = make_float4(xh, yh, zh, (float)(-w.qn * qs));
     f.n_lo = make_float4((float)(xs - xh), (float)(ys - yh), (float)(zs - zh), 0.f);
-    f.n_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
+    w.n_rel = make_float4((float)(xs - w.cx * kUnitScale), (float)(ys - w.cy * kUnitScale),
                           (float)(zs - w.cz * kUnitScale), (float)(-w.qn * qs));
   }
+  out[p] = w;
   out32[p] = f;
 }
 
@@ -347,6 +350,29 @@ __device__ __forceinline__ float PotLocal(float dx, float dy, float dz, const fl
 // 3.5 delta max(1, theta) / w_min <= 2^-20, i.e. R max(1, theta) <= 8 root half widths.  Outside a
 // relative band of 2^-16 the FP32 verdict is therefore the FP64 one; inside it the walk evaluates the
 // FP64 test.  FastMacRange checks that bound (and keeps the units^2 values in FP32's normal range).
+// A walk record's hot part (two wide scalar loads: the node index is wave-uniform) and its FP64 part.
+typedef uint32_t u32x16v __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x8v __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ WalkHot LoadHot(const WalkNode* __restrict__ nodes, int c) {
+  const char* base = reinterpret_cast<const char*>(nodes + c);
+  const u32x16v a = *reinterpret_cast<const u32x16v*>(__builtin_assume_aligned(base, 64));
+  const u32x4v b = *reinterpret_cast<const u32x4v*>(__builtin_assume_aligned(base + 64, 16));
+  WalkHot h;
+  __builtin_memcpy(&h, &a, 64);
+  __builtin_memcpy(reinterpret_cast<char*>(&h) + 64, &b, 16);
+  return h;
+}
+__device__ __forceinline__ WalkCold LoadCold(const WalkNode* __restrict__ nodes, int c) {
+  const char* base = reinterpret_cast<const char*>(nodes + c) + sizeof(WalkHot);
+  const u32x8v a = *reinterpret_cast<const u32x8v*>(__builtin_assume_aligned(base, 16));
+  const u32x4v b = *reinterpret_cast<const u32x4v*>(__builtin_assume_aligned(base + 32, 16));
+  WalkCold w;
+  __builtin_memcpy(&w, &a, 32);
+  __builtin_memcpy(reinterpret_cast<char*>(&w) + 32, &b, 16);
+  return w;
+}
+
 constexpr float kMacSureBelow = 1.0f - 0x1p-16f;
 constexpr float kMacSureAbove = 1.0f + 0x1p-16f;
 
@@ -372,7 +398,7 @@ struct ExtTargets {
 
 template <int MODE, bool COUNT, bool EXT = false>
 __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNode* __restrict__ nodes,
-                                                     const WalkNodeF32* __restrict__ nodes32,
+                                                     const WalkNodeAux* __restrict__ nodes32,
                                                      const RootBox* __restrict__ root,
                                                      const uint32_t* __restrict__ order,
                                                      const double* __restrict__ x, const double* __restrict__ y,
@@ -466,13 +492,16 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
       }
     }
   };
-  int c = __builtin_amdgcn_readfirstlane(WaveMin(next));  // wave-uniform: node records come in by scalar loads
-  while (c < nn) {
-    const WalkNode nd = nodes[c];
-    const bool mine = next == c;
+  // One visit of node c (record nd) by the wave; returns whether any lane opened c, so that the
+  // wave descends to c + 1 (else it moves to skip(c)).  A lane reaches a node inside c's subtree
+  // only by opening c, and the wave visits c once (its node only increases), so every lane that was
+  // not at c already points at skip(c) or beyond, and every lane at c now points at c + 1 (opened)
+  // or skip(c): the wave's next node needs no reduction over the lanes.  A lane that opens a twig
+  // sums its children at once and moves to skip(c).
+  auto visit = [&](const int c, const WalkHot& nd) -> bool {
     bool opened = false;
-    if (mine) {
-      if (nd.qp == 0.0 && nd.qn == 0.0) {
+    if (next == c) {
+      if (nd.flags & kNodeEmpty) {
         next = nd.skip;
       } else if (nd.leaf) {
         direct(nd.b, nd.e);
@@ -489,22 +518,21 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
           verdict = nd.w2 < r * kMacSureBelow ? 1 : (nd.w2 > r * kMacSureAbove ? 0 : -1);
         }
         if (verdict < 0) {  // the reference's test (barnes-hut.cc:201-204) in FP64 without contraction
-          const double dx = __dsub_rn(px, nd.cx), dy = __dsub_rn(py, nd.cy), dz = __dsub_rn(pz, nd.cz);
+          const WalkCold cb = LoadCold(nodes, c);
+          const double dx = __dsub_rn(px, cb.cx), dy = __dsub_rn(py, cb.cy), dz = __dsub_rn(pz, cb.cz);
           const double d2 = Dot3(dx, dy, dz, dx, dy, dz);
-          const double w = __dmul_rn(2.0, nd.half);
+          const double w = __dmul_rn(2.0, cb.half);
           verdict = __dmul_rn(w, w) < __dmul_rn(theta2, d2) ? 1 : 0;
         }
-        if (verdict) {  // accepted (barnes-hut.cc:201-215)
+        if (verdict) {  // accepted (barnes-hut.cc:201-215); an empty class has charge 0 and adds 0
           if (MODE == 3) {
-            const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
-            ax += PotLocal(dxc, dyc, dzc, f.p_rel) + PotLocal(dxc, dyc, dzc, f.n_rel);
-            if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
+            ax += PotLocal(dxc, dyc, dzc, nd.p_rel) + PotLocal(dxc, dyc, dzc, nd.n_rel);
+            if (COUNT) cnt += ((nd.flags & kNodeHasPos) ? 1 : 0) + ((nd.flags & kNodeHasNeg) ? 1 : 0);
           } else if (MODE == 0) {
-            const WalkNodeF32 f = nodes32[c];  // an empty class has charge 0 and adds exactly 0
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            PairLocal(dxc, dyc, dzc, f.p_rel, fx, fy, fz);
-            PairLocal(dxc, dyc, dzc, f.n_rel, fx, fy, fz);
-            if (COUNT) cnt += (nd.qp > 0.0 ? 1 : 0) + (nd.qn > 0.0 ? 1 : 0);
+            PairLocal(dxc, dyc, dzc, nd.p_rel, fx, fy, fz);
+            PairLocal(dxc, dyc, dzc, nd.n_rel, fx, fy, fz);
+            if (COUNT) cnt += ((nd.flags & kNodeHasPos) ? 1 : 0) + ((nd.flags & kNodeHasNeg) ? 1 : 0);
             ax += fx;
             ay += fy;
             az += fz;
@@ -512,25 +540,29 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
               if (nrec < (uint32_t)rec_cap) rec[(size_t)k * rec_cap + nrec] = (uint32_t)c;
               ++nrec;
             }
-          } else if (MODE == 1) {
-            if (nd.qp > 0.0) { PairF64Acc(px, py, pz, qi, nd.pqx, nd.pqy, nd.pqz, nd.qp, ax, ay, az); if (COUNT) ++cnt; }
-            if (nd.qn > 0.0) { PairF64Acc(px, py, pz, qi, nd.nqx, nd.nqy, nd.nqz, -nd.qn, ax, ay, az); if (COUNT) ++cnt; }
           } else {
-            if (nd.qp > 0.0) {
-              const double rx = __dsub_rn(px, nd.pqx), ry = __dsub_rn(py, nd.pqy), rz = __dsub_rn(pz, nd.pqz);
-              const double d2p = Dot3(rx, ry, rz, rx, ry, rz);
-              if (d2p >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(nd.qp, sqrt(d2p)));
-              if (COUNT) ++cnt;
-            }
-            if (nd.qn > 0.0) {
-              const double rx = __dsub_rn(px, nd.nqx), ry = __dsub_rn(py, nd.nqy), rz = __dsub_rn(pz, nd.nqz);
-              const double d2n = Dot3(rx, ry, rz, rx, ry, rz);
-              if (d2n >= kMinSquaredDistance) ax = __dsub_rn(ax, __ddiv_rn(nd.qn, sqrt(d2n)));
-              if (COUNT) ++cnt;
+            const WalkNodeAux au = nodes32[c];
+            const WalkCold cq = LoadCold(nodes, c);
+            if (MODE == 1) {
+              if (cq.qp > 0.0) { PairF64Acc(px, py, pz, qi, au.pqx, au.pqy, au.pqz, cq.qp, ax, ay, az); if (COUNT) ++cnt; }
+              if (cq.qn > 0.0) { PairF64Acc(px, py, pz, qi, au.nqx, au.nqy, au.nqz, -cq.qn, ax, ay, az); if (COUNT) ++cnt; }
+            } else {
+              if (cq.qp > 0.0) {
+                const double rx = __dsub_rn(px, au.pqx), ry = __dsub_rn(py, au.pqy), rz = __dsub_rn(pz, au.pqz);
+                const double d2p = Dot3(rx, ry, rz, rx, ry, rz);
+                if (d2p >= kMinSquaredDistance) ax = __dadd_rn(ax, __ddiv_rn(cq.qp, sqrt(d2p)));
+                if (COUNT) ++cnt;
+              }
+              if (cq.qn > 0.0) {
+                const double rx = __dsub_rn(px, au.nqx), ry = __dsub_rn(py, au.nqy), rz = __dsub_rn(pz, au.nqz);
+                const double d2n = Dot3(rx, ry, rz, rx, ry, rz);
+                if (d2n >= kMinSquaredDistance) ax = __dsub_rn(ax, __ddiv_rn(cq.qn, sqrt(d2n)));
+                if (COUNT) ++cnt;
+              }
             }
           }
           next = nd.skip;
-        } else if (nd.twig) {
+        } else if (nd.flags & kNodeTwig) {
           direct(nd.b, nd.e);  // opened; its children are leaves, summed here in their order
           next = nd.skip;
         } else {
@@ -539,13 +571,30 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         }
       }
     }
-    // The wave's next node is the smallest lane next.  A lane reaches a node inside c's subtree
-    // only by opening c, and the wave visits c once (its node only increases), so every lane
-    // that was not at c already points at skip(c) or beyond, and every lane at c now points at
-    // c + 1 (opened) or skip(c).  The minimum is therefore c + 1 if any lane opened c, else
-    // skip(c): no reduction over the lanes is needed.  (A lane that opens a twig sums its
-    // children at once and moves to skip(c).)
-    c = __builtin_amdgcn_readfirstlane(__ballot(opened) ? c + 1 : nd.skip);
+    return __ballot(opened) != 0;
+  };
+  // Node records come in by scalar loads (the node is wave-uniform).  The loop is unrolled by two
+  // with the records in alternating registers: while node c is visited, the record of skip(c), the
+  // wave's next node unless some lane opens c, is already being fetched.  An opened node is internal
+  // and not a twig, so c + 1 < nn then.
+  int c = __builtin_amdgcn_readfirstlane(WaveMin(next));
+  WalkHot n0 = LoadHot(nodes, min(c, nn - 1)), n1;
+  while (c < nn) {
+    n1 = LoadHot(nodes, min(n0.skip, nn - 1));
+    if (visit(c, n0)) {
+      ++c;
+      n1 = LoadHot(nodes, c);
+    } else {
+      c = n0.skip;
+    }
+    if (c >= nn) break;
+    n0 = LoadHot(nodes, min(n1.skip, nn - 1));
+    if (visit(c, n1)) {
+      ++c;
+      n0 = LoadHot(nodes, c);
+    } else {
+      c = n1.skip;
+    }
   }
   if (!valid) return;
   if (EXT) {  // per point: the force on charge q, or phi (C/m for MODE 2, e/unit for MODE 3)
@@ -653,9 +702,9 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   }
   void *wn, *wn32;
   if ((rc = Scratch(ctx, "bh_walknodes", (size_t)nn * sizeof(WalkNode), &wn))) return rc;
-  if ((rc = Scratch(ctx, "bh_walknodes32", (size_t)nn * sizeof(WalkNodeF32), &wn32))) return rc;
+  if ((rc = Scratch(ctx, "bh_walknodes32", (size_t)nn * sizeof(WalkNodeAux), &wn32))) return rc;
   hipLaunchKernelGGL(GatherPreorderKernel, dim3(Blocks(nn, 256)), dim3(256), 0, st, nn, o.nd, mo,
-                     (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeF32*)wn32, sm,
+                     (const uint64_t*)o.pkeys, (const uint32_t*)o.pre, (WalkNode*)wn, (WalkNodeAux*)wn32, sm,
                      (FeatNode*)featp);
   AM_HIP(hipGetLastError());
   void* ssrc;
@@ -668,7 +717,7 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
   t->nn = nn;
   t->order = o.order;
   t->nodes = (WalkNode*)wn;
-  t->nodes32 = (WalkNodeF32*)wn32;
+  t->nodes32 = (WalkNodeAux*)wn32;
   t->root = (const RootBox*)root;
   t->max_depth = nlevels - 1;
   t->deep_overflow = o.deep_overflow;

@@ -14,23 +14,44 @@ struct FeatNode {
   float pad;
 };
 
+// Flags of a walk record.
+constexpr uint32_t kNodeTwig = 1;    // internal node whose children are all leaves
+constexpr uint32_t kNodeEmpty = 2;   // no charge of either sign: the walk skips it
+constexpr uint32_t kNodeHasPos = 4;  // qp > 0
+constexpr uint32_t kNodeHasNeg = 8;  // qn > 0
+
+// A node as the walk reads it, 128 bytes in DFS preorder.  The first 80 bytes (WalkHot) are what
+// an FP32 visit needs; the walk fetches them with two wide scalar loads ahead of the visit (the
+// successor skip(c) while c is being visited).  The rest (WalkCold) holds the FP64 charges, read by
+// the FP64 walks, and the FP64 cube, read by the exact acceptance test near its threshold.
 struct WalkNode {
-  // The first 64 bytes are what the walk reads at every visit (one scalar fetch).
-  float chx, chy, chz;  // cube centre in units of 2^-20 m, FP32 hi parts
-  float w2;             // (2 half)^2 in units^2, rounded to FP32: the acceptance test's left side
-  float clx, cly, clz;  // lo parts: centre = hi + lo to ~2^-48 relative
-  int twig;             // internal node whose children are all leaves
-  double qp, qn;        // positive charge and |negative charge|
-  int b, e, skip, leaf; // particle range in the sorted order, subtree skip pointer
-  double pqx, pqy, pqz; // positive charge centre
-  double nqx, nqy, nqz; // negative charge centre
-  double cx, cy, cz, half;  // geometric cube in FP64 (the exact acceptance test)
-};
-struct WalkNodeF32 {  // FP32 monopoles, charge in e
-  // Charge centre minus cube centre in units of 2^-20 m (|offset| <= half width, so FP32 keeps it to
-  // 2^-24 of the half width); an empty class is (0, 0, 0, charge 0).  The walk reads these 32 bytes.
+  float chx, chy, chz;      // cube centre in units of 2^-20 m, FP32 hi parts
+  float w2;                 // (2 half)^2 in units^2, rounded to FP32: the acceptance test's left side
+  float clx, cly, clz;      // lo parts: centre = hi + lo to ~2^-48 relative
+  uint32_t flags;           // kNodeTwig | kNodeEmpty | kNodeHasPos | kNodeHasNeg
+  int b, e, skip, leaf;     // particle range in the sorted order, subtree skip pointer, leaf (0/1)
+  // Charge centre minus cube centre in units (|offset| <= half width, so FP32 keeps it to 2^-24 of
+  // the half width), charge in e; an empty class is (0, 0, 0, charge 0).
   float4 p_rel, n_rel;
-  float4 p_hi, p_lo, n_hi, n_lo;  // hi/lo charge centres (learned-correction residuals)
+  double qp, qn;            // positive charge and |negative charge|
+  double cx, cy, cz, half;  // geometric cube in FP64
+};
+struct WalkHot {  // the first 80 bytes of a WalkNode
+  float chx, chy, chz, w2, clx, cly, clz;
+  uint32_t flags;
+  int b, e, skip, leaf;
+  float4 p_rel, n_rel;
+};
+struct WalkCold {  // the last 48 bytes of a WalkNode
+  double qp, qn;
+  double cx, cy, cz, half;
+};
+static_assert(sizeof(WalkNode) == 128 && sizeof(WalkHot) == 80 && sizeof(WalkCold) == 48, "walk record layout");
+
+struct WalkNodeAux {  // read by the FP64 walks' accepted nodes and by the learned correction
+  float4 p_hi, p_lo, n_hi, n_lo;  // FP32 hi/lo charge centres in units, charge in e
+  double pqx, pqy, pqz;           // FP64 charge centres
+  double nqx, nqy, nqz;
 };
 
 struct BHTree {
@@ -39,7 +60,7 @@ struct BHTree {
   int nn = 0;           // nodes
   uint32_t* order = nullptr;  // sorted position -> particle index
   WalkNode* nodes = nullptr;
-  WalkNodeF32* nodes32 = nullptr;
+  WalkNodeAux* nodes32 = nullptr;
   const RootBox* root = nullptr;  // device: the root cube (the FP32 acceptance test's range check)
   FeatNode* feat = nullptr;  // only when built with features
   float4* ssrc = nullptr;    // FP32 hi/lo mirror in tree order (2 per sorted position)

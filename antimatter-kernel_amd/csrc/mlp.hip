@@ -386,7 +386,7 @@ __global__ __launch_bounds__(64) void MlpInferKernel(int rows, const float* __re
 // ---- features, labels and the correction ------------------------------------------------------------
 // Raw features of (target t, node c, sign s) and the monopole geometry; returns false if the node
 // has no sign-s charge.  Units: 2^-20 m and e.
-__device__ inline bool RowFeatures(const float4 th, const float4 tl, const WalkNodeF32& nf, const FeatNode& fe, int sgn,
+__device__ inline bool RowFeatures(const float4 th, const float4 tl, const WalkNodeAux& nf, const FeatNode& fe, int sgn,
                                    float feat[kIn], float& R, float& Qs_signed, float& h) {
   const float* f = sgn == 0 ? fe.fp : fe.fn;
   const float Q = f[9];
@@ -408,7 +408,7 @@ __device__ inline bool RowFeatures(const float4 th, const float4 tl, const WalkN
 
 // Exact residual (e/unit^2) of node c's sign-s particles on target i, FP32 pair math.
 __device__ inline void ExactResidual(uint32_t i, const float4 th, const float4 tl, const WalkNode& nd,
-                                     const WalkNodeF32& nf, int sgn, const uint32_t* __restrict__ order,
+                                     const WalkNodeAux& nf, int sgn, const uint32_t* __restrict__ order,
                                      const float4* __restrict__ src, float out[3]) {
   float fx = 0.f, fy = 0.f, fz = 0.f;
   for (int k = nd.b; k < nd.e; ++k) {
@@ -429,7 +429,7 @@ __device__ inline void ExactResidual(uint32_t i, const float4 th, const float4 t
 // both signs: raw features [16] and the normalised label y (FP64 exact residual).
 __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, const uint32_t* __restrict__ rec,
                               int rec_cap, const uint32_t* __restrict__ rec_cnt, const uint32_t* __restrict__ row_off,
-                              const WalkNode* __restrict__ nodes, const WalkNodeF32* __restrict__ nodes32,
+                              const WalkNode* __restrict__ nodes, const WalkNodeAux* __restrict__ nodes32,
                               const FeatNode* __restrict__ feat, const uint32_t* __restrict__ order,
                               const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                               const double* __restrict__ q, const float4* __restrict__ src, float* __restrict__ X,
@@ -443,7 +443,7 @@ __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, 
   const uint32_t i = order[tsel[k]];
   const int c = (int)rec[(size_t)k * rec_cap + slot];
   const WalkNode nd = nodes[c];
-  const WalkNodeF32 nf = nodes32[c];
+  const WalkNodeAux nf = nodes32[c];
   const FeatNode fe = feat[c];
   const float4 th = src[2 * i], tl = src[2 * i + 1];
   const double tx = x[i] * kUnitScale, ty = y[i] * kUnitScale, tz = z[i] * kUnitScale;
@@ -456,8 +456,8 @@ __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, 
     for (int a = 0; a < kIn; ++a) X[row * kIn + a] = ok ? f[a] : 0.f;
     double dx = 0.0, dy = 0.0, dz = 0.0, lscale = 0.0;
     if (ok) {
-      const double cs[3] = {(sgn == 0 ? nd.pqx : nd.nqx) * kUnitScale, (sgn == 0 ? nd.pqy : nd.nqy) * kUnitScale,
-                            (sgn == 0 ? nd.pqz : nd.nqz) * kUnitScale};
+      const double cs[3] = {(sgn == 0 ? nf.pqx : nf.nqx) * kUnitScale, (sgn == 0 ? nf.pqy : nf.nqy) * kUnitScale,
+                            (sgn == 0 ? nf.pqz : nf.nqz) * kUnitScale};
       for (int kk = nd.b; kk < nd.e; ++kk) {
         const uint32_t j = order[kk];
         const double qj = q[j] / kElementaryCharge;
@@ -502,7 +502,7 @@ constexpr int kCorrectTake = 8;  // targets per queue take
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void MlpCorrectKernel(int m, const uint32_t* __restrict__ rec, int rec_cap,
                                                        const uint32_t* __restrict__ rec_cnt,
                                                        const WalkNode* __restrict__ nodes,
-                                                       const WalkNodeF32* __restrict__ nodes32,
+                                                       const WalkNodeAux* __restrict__ nodes32,
                                                        const FeatNode* __restrict__ feat,
                                                        const uint32_t* __restrict__ order,
                                                        const double* __restrict__ q, const float4* __restrict__ src,
@@ -932,7 +932,7 @@ int am_bh_learned_training_data(am_store* s, double theta, int n_targets, int st
   AM_HIP(hipMemcpyAsync(off, ho.data(), (size_t)(nt + 1) * 4, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(MlpDataKernel, dim3(Blocks(maxc, 64), nt), dim3(64), 0, st, nt, (const uint32_t*)dsel,
                      (const uint32_t*)rec, rec_cap, (const uint32_t*)cnt, (const uint32_t*)off,
-                     (const WalkNode*)t.nodes, (const WalkNodeF32*)t.nodes32, (const FeatNode*)t.feat,
+                     (const WalkNode*)t.nodes, (const WalkNodeAux*)t.nodes32, (const FeatNode*)t.feat,
                      (const uint32_t*)t.order, s->x, s->y, s->z, s->q, (const float4*)s->src, (float*)dX,
                      (float*)dY, (uint8_t*)dv, (int32_t*)dt, (int32_t*)dn,
                      (float*)ds);
@@ -999,7 +999,7 @@ int am_compute_barnes_hut_learned_forces(am_store* s, double theta, am_mlp* net,
   // 2 waves per SIMD fit (the weight fragments take about 230 VGPRs): 8 one-wave workgroups per CU.
   const int grid = std::min((t.m + kCorrectTake - 1) / kCorrectTake, std::max(1, ctx->num_cus) * 8);
   hipLaunchKernelGGL(MlpCorrectKernel, dim3(grid), dim3(64), 0, st, t.m, (const uint32_t*)rec, rec_cap,
-                     (const uint32_t*)cnt, (const WalkNode*)t.nodes, (const WalkNodeF32*)t.nodes32,
+                     (const uint32_t*)cnt, (const WalkNode*)t.nodes, (const WalkNodeAux*)t.nodes32,
                      (const FeatNode*)t.feat, (const uint32_t*)t.order, s->q, (const float4*)s->src, MlpView(net),
                      ybound, amax, s->fx, s->fy, s->fz, (unsigned long long*)dstats, queue);
   TimerEnd(ctx, slot);
