@@ -373,6 +373,24 @@ __device__ __forceinline__ WalkCold LoadCold(const WalkNode* __restrict__ nodes,
   return w;
 }
 
+// kDirectGroup consecutive sources' hi/lo pairs from the tree-ordered FP32 mirror, starting at
+// sorted position k0 (32-byte aligned): one wide scalar load per two sources.  The mirror has
+// kSsrcPad zero entries past its last position, so a group never reads outside it.
+constexpr int kDirectGroup = 2;  // sources per group of a walk's direct sum
+constexpr int kSsrcPad = 8;
+static_assert(kDirectGroup % 2 == 0 && kDirectGroup <= kSsrcPad, "direct-sum groups");
+__device__ __forceinline__ void LoadSourceGroup(const float4* __restrict__ ssrc, int k0, float4* sh, float4* sl) {
+#pragma unroll
+  for (int g = 0; g < kDirectGroup / 2; ++g) {
+    const u32x16v v = *reinterpret_cast<const u32x16v*>(__builtin_assume_aligned(ssrc + 2 * (k0 + 2 * g), 32));
+    const char* w = reinterpret_cast<const char*>(&v);
+    __builtin_memcpy(&sh[2 * g], w, 16);
+    __builtin_memcpy(&sl[2 * g], w + 16, 16);
+    __builtin_memcpy(&sh[2 * g + 1], w + 32, 16);
+    __builtin_memcpy(&sl[2 * g + 1], w + 48, 16);
+  }
+}
+
 constexpr float kMacSureBelow = 1.0f - 0x1p-16f;
 constexpr float kMacSureAbove = 1.0f + 0x1p-16f;
 
@@ -455,22 +473,45 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
   // Direct sum over sorted positions [b, e): a leaf's particles, or a twig's (its leaf children's,
   // in their order).
   auto direct = [&](int b, int e) {
+    // FP32 modes: sources in groups of kDirectGroup whose loads are issued together (one round trip
+    // per group).  A group's entries past e are loaded clamped with their charge set to 0, so every
+    // entry runs the same arithmetic (a charge-0 pair adds exactly 0) and no branch separates the
+    // loads from their uses.
     if (MODE == 3) {
       float ph = 0.f;
-      for (int kk = b; kk < e; ++kk)
-        if (!EXT || order[kk] != i) ph += PotF32(th, tl, ssrc[2 * kk], ssrc[2 * kk + 1]);
+      for (int k0 = b; k0 < e; k0 += kDirectGroup) {
+        float4 sh[kDirectGroup], sl[kDirectGroup];
+        LoadSourceGroup(ssrc, k0, sh, sl);
+#pragma unroll
+        for (int u = 0; u < kDirectGroup; ++u) {
+          const bool self = EXT && order[min(k0 + u, e - 1)] == i;
+          if (k0 + u >= e || self) sh[u].w = 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kDirectGroup; ++u) {
+          ph += PotF32(th, tl, sh[u], sl[u]);
+          if (COUNT) cnt += (sh[u].w != 0.f && k0 + u != ks) ? 1 : 0;
+        }
+      }
       ax += ph;
-      if (COUNT) for (int kk = b; kk < e; ++kk) cnt += (ssrc[2 * kk].w != 0.f && kk != ks) ? 1 : 0;
     } else if (MODE == 0) {
       // The range's sources are contiguous in the tree-ordered copy of the FP32 mirror.  The target
       // itself and charge-0 sources add exactly 0 (softened pair, q = 0), so they are not tested; the
       // interaction count leaves them out.
       float fx = 0.f, fy = 0.f, fz = 0.f;
-      for (int kk = b; kk < e; ++kk) {
-        const float4 sh = ssrc[2 * kk];
-        if (EXT && order[kk] == i) continue;  // the point's self source, at any position
-        PairF32(th, tl, sh, ssrc[2 * kk + 1], fx, fy, fz);
-        if (COUNT) cnt += (sh.w != 0.f && kk != ks) ? 1 : 0;
+      for (int k0 = b; k0 < e; k0 += kDirectGroup) {
+        float4 sh[kDirectGroup], sl[kDirectGroup];
+        LoadSourceGroup(ssrc, k0, sh, sl);
+#pragma unroll
+        for (int u = 0; u < kDirectGroup; ++u) {
+          const bool self = EXT && order[min(k0 + u, e - 1)] == i;  // the point's self source, anywhere
+          if (k0 + u >= e || self) sh[u].w = 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kDirectGroup; ++u) {
+          PairF32(th, tl, sh[u], sl[u], fx, fy, fz);
+          if (COUNT) cnt += (sh[u].w != 0.f && k0 + u != ks) ? 1 : 0;
+        }
       }
       ax += fx;
       ay += fy;
@@ -631,11 +672,18 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
 }
 
 // ---- orchestration ------------------------------------------------------------------------------
-// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position).
+// Tree-ordered copy of the FP32 source mirror (hi/lo per sorted position), followed by
+// kSsrcPad zero entries: the walk's direct sums fetch sources in groups that may run past the last
+// position (those entries get charge 0).
 __global__ void GatherTreeSrcKernel(int m, const uint32_t* __restrict__ order, const float4* __restrict__ src,
                                     float4* __restrict__ ssrc) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
+  if (k >= m + kSsrcPad) return;
+  if (k >= m) {
+    ssrc[2 * k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ssrc[2 * k + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   const uint32_t i = order[k];
   ssrc[2 * k] = src[2 * i];
   ssrc[2 * k + 1] = src[2 * i + 1];
@@ -708,8 +756,8 @@ int BuildBHTree(am_store* s, int leaf_cap, BHTree* t, bool with_features) {
                      (FeatNode*)featp);
   AM_HIP(hipGetLastError());
   void* ssrc;
-  if ((rc = Scratch(ctx, "bh_ssrc", (size_t)m * 2 * sizeof(float4), &ssrc))) return rc;
-  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
+  if ((rc = Scratch(ctx, "bh_ssrc", (size_t)(m + kSsrcPad) * 2 * sizeof(float4), &ssrc))) return rc;
+  hipLaunchKernelGGL(GatherTreeSrcKernel, dim3(Blocks(m + kSsrcPad, 256)), dim3(256), 0, st, m, (const uint32_t*)o.order,
                      (const float4*)s->src, (float4*)ssrc);
   AM_HIP(hipGetLastError());
   t->ssrc = (float4*)ssrc;
