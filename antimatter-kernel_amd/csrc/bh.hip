@@ -229,8 +229,9 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
     w.clx = lo[0];
     w.cly = lo[1];
     w.clz = lo[2];
-    const double wu = 2.0 * w.half * kUnitScale;
-    w.w2 = (float)(wu * wu);
+    const double wu = 2.0 * w.half * kUnitScale, W = wu * wu;
+    w.w2hi = (float)(W * (1.0 + 0x1p-15));
+    w.w2lo = (float)(W * (1.0 - 0x1p-15));
   }
   f.pqx = mo.cpx[node];
   f.pqy = mo.cpy[node];
@@ -242,13 +243,14 @@ __global__ void GatherPreorderKernel(int nn, NodeArrays nd, Moments mo, const ui
   w.qn = mo.qn[node];
   w.b = nd.b[node];
   w.e = nd.e[node];
-  w.leaf = nd.leaf[node];
   // A twig is an internal node whose children are all leaves: a lane that opens it sums the
   // children's particles directly, in the children's order, which is what visiting them would do.
+  const bool leaf = nd.leaf[node] != 0;
   uint32_t flags = (w.qp == 0.0 && w.qn == 0.0) ? kNodeEmpty : 0u;
+  if (leaf) flags |= kNodeLeaf;
   if (w.qp > 0.0) flags |= kNodeHasPos;
   if (w.qn > 0.0) flags |= kNodeHasNeg;
-  if (!w.leaf) {
+  if (!leaf) {
     const int c0 = nd.first_child[node], c1 = c0 + nd.nchild[node];
     int all = c1 > c0 ? 1 : 0;
     for (int ch = c0; ch < c1; ++ch) all &= nd.leaf[ch] ? 1 : 0;
@@ -342,14 +344,6 @@ __device__ __forceinline__ float PotLocal(float dx, float dy, float dz, const fl
   return w * __builtin_amdgcn_rsqf(r2);
 }
 
-// The acceptance test (2h)^2 < theta^2 |x - c|^2 (barnes-hut.cc:201-204) in FP32.  With the target
-// and the cube centre as hi/lo pairs in units of 2^-20 m, each FP32 difference is within
-// 2^-23 |d| + delta of the exact one, delta = 2^-44 R (R bounds |coordinate| in units), and
-// theta^2 |d|^2 then lies within 2^-19 relative of the FP64 side wherever |d| >= w_min / max(1, theta)
-// (w_min = the smallest internal cube width, 2^-19 root half widths at depth <= 20) and
-// 3.5 delta max(1, theta) / w_min <= 2^-20, i.e. R max(1, theta) <= 8 root half widths.  Outside a
-// relative band of 2^-16 the FP32 verdict is therefore the FP64 one; inside it the walk evaluates the
-// FP64 test.  FastMacRange checks that bound (and keeps the units^2 values in FP32's normal range).
 // A walk record's hot part (two wide scalar loads: the node index is wave-uniform) and its FP64 part.
 typedef uint32_t u32x16v __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x8v __attribute__((ext_vector_type(8)));
@@ -391,9 +385,28 @@ __device__ __forceinline__ void LoadSourceGroup(const float4* __restrict__ ssrc,
   }
 }
 
-constexpr float kMacSureBelow = 1.0f - 0x1p-16f;
-constexpr float kMacSureAbove = 1.0f + 0x1p-16f;
 
+// PairLocal into fresh accumulators (products instead of fused adds to zero).
+__device__ __forceinline__ void PairLocalFirst(const float tx, const float ty, const float tz, const float4 s,
+                                               float& px, float& py, float& pz) {
+  const float dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, kEps2Units)));
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float c = (s.w * ri) * (ri * ri);
+  px = c * dx;
+  py = c * dy;
+  pz = c * dz;
+}
+
+// The acceptance test (2h)^2 < theta^2 |x - c|^2 (barnes-hut.cc:201-204) in FP32.  With the target
+// and the cube centre as hi/lo pairs in units of 2^-20 m, each FP32 difference is within
+// 2^-23 |d| + delta of the exact one, delta = 2^-44 R (R bounds |coordinate| in units), and
+// theta^2 |d|^2 then lies within 2^-19 relative of the FP64 side wherever |d| >= w_min / max(1, theta)
+// (w_min = the smallest internal cube width, 2^-19 root half widths at depth <= 20) and
+// 3.5 delta max(1, theta) / w_min <= 2^-20, i.e. R max(1, theta) <= 8 root half widths.  The node
+// brackets W = (2h)^2 by w2hi >= W (1 + 2^-16) and w2lo <= W (1 - 2^-16), so theta^2 |d|^2 above
+// w2hi accepts and below w2lo opens exactly as the FP64 test does; in between the walk evaluates the
+// FP64 test.  FastMacRange checks that bound (and keeps the units^2 values in FP32's normal range).
 __device__ inline bool FastMacRange(const RootBox* __restrict__ root, double theta2) {
   const RootBox r = *root;
   const double h = r.half * kUnitScale;
@@ -544,7 +557,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
     if (next == c) {
       if (nd.flags & kNodeEmpty) {
         next = nd.skip;
-      } else if (nd.leaf) {
+      } else if (nd.flags & kNodeLeaf) {
         direct(nd.b, nd.e);
         next = nd.skip;
       } else {
@@ -556,7 +569,7 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
         int verdict = -1;
         if (fast) {
           const float r = theta2f * fmaf(dxc, dxc, fmaf(dyc, dyc, dzc * dzc));
-          verdict = nd.w2 < r * kMacSureBelow ? 1 : (nd.w2 > r * kMacSureAbove ? 0 : -1);
+          verdict = r > nd.w2hi ? 1 : (r < nd.w2lo ? 0 : -1);
         }
         if (verdict < 0) {  // the reference's test (barnes-hut.cc:201-204) in FP64 without contraction
           const WalkCold cb = LoadCold(nodes, c);
@@ -570,8 +583,8 @@ __global__ __launch_bounds__(256) void BHWalkKernel(int m, int nn, const WalkNod
             ax += PotLocal(dxc, dyc, dzc, nd.p_rel) + PotLocal(dxc, dyc, dzc, nd.n_rel);
             if (COUNT) cnt += ((nd.flags & kNodeHasPos) ? 1 : 0) + ((nd.flags & kNodeHasNeg) ? 1 : 0);
           } else if (MODE == 0) {
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-            PairLocal(dxc, dyc, dzc, nd.p_rel, fx, fy, fz);
+            float fx, fy, fz;
+            PairLocalFirst(dxc, dyc, dzc, nd.p_rel, fx, fy, fz);
             PairLocal(dxc, dyc, dzc, nd.n_rel, fx, fy, fz);
             if (COUNT) cnt += ((nd.flags & kNodeHasPos) ? 1 : 0) + ((nd.flags & kNodeHasNeg) ? 1 : 0);
             ax += fx;
@@ -966,7 +979,7 @@ int am_bh_tree_download(am_store* s, uint32_t* order, int32_t* begin, int32_t* e
     for (int k = 0; k < nn; ++k) {
       if (begin) begin[k] = w[k].b;
       if (end) end[k] = w[k].e;
-      if (leaf) leaf[k] = (uint8_t)w[k].leaf;
+      if (leaf) leaf[k] = (w[k].flags & kNodeLeaf) ? 1 : 0;
       if (qp) qp[k] = w[k].qp;
       if (qn) qn[k] = w[k].qn;
     }

@@ -19,6 +19,7 @@ constexpr uint32_t kNodeTwig = 1;    // internal node whose children are all lea
 constexpr uint32_t kNodeEmpty = 2;   // no charge of either sign: the walk skips it
 constexpr uint32_t kNodeHasPos = 4;  // qp > 0
 constexpr uint32_t kNodeHasNeg = 8;  // qn > 0
+constexpr uint32_t kNodeLeaf = 16;
 
 // A node as the walk reads it, 128 bytes in DFS preorder.  The first 80 bytes (WalkHot) are what
 // an FP32 visit needs; the walk fetches them with two wide scalar loads ahead of the visit (the
@@ -26,10 +27,13 @@ constexpr uint32_t kNodeHasNeg = 8;  // qn > 0
 // the FP64 walks, and the FP64 cube, read by the exact acceptance test near its threshold.
 struct WalkNode {
   float chx, chy, chz;      // cube centre in units of 2^-20 m, FP32 hi parts
-  float w2;                 // (2 half)^2 in units^2, rounded to FP32: the acceptance test's left side
+  // The acceptance test's left side W = (2 half)^2 in units^2 bracketed for the FP32 test:
+  // w2hi >= W (1 + 2^-16) and w2lo <= W (1 - 2^-16) (FP32 roundings of W (1 +- 2^-15)).
+  float w2hi;
   float clx, cly, clz;      // lo parts: centre = hi + lo to ~2^-48 relative
-  uint32_t flags;           // kNodeTwig | kNodeEmpty | kNodeHasPos | kNodeHasNeg
-  int b, e, skip, leaf;     // particle range in the sorted order, subtree skip pointer, leaf (0/1)
+  uint32_t flags;           // kNodeTwig | kNodeEmpty | kNodeHasPos | kNodeHasNeg | kNodeLeaf
+  int b, e, skip;           // particle range in the sorted order, subtree skip pointer
+  float w2lo;
   // Charge centre minus cube centre in units (|offset| <= half width, so FP32 keeps it to 2^-24 of
   // the half width), charge in e; an empty class is (0, 0, 0, charge 0).
   float4 p_rel, n_rel;
@@ -37,9 +41,10 @@ struct WalkNode {
   double cx, cy, cz, half;  // geometric cube in FP64
 };
 struct WalkHot {  // the first 80 bytes of a WalkNode
-  float chx, chy, chz, w2, clx, cly, clz;
+  float chx, chy, chz, w2hi, clx, cly, clz;
   uint32_t flags;
-  int b, e, skip, leaf;
+  int b, e, skip;
+  float w2lo;
   float4 p_rel, n_rel;
 };
 struct WalkCold {  // the last 48 bytes of a WalkNode
