@@ -176,6 +176,7 @@ __global__ __launch_bounds__(256) void MlpTrainTileKernel(int rows, const float*
   }
 }
 
+// Sums the tiles' gradients in tile order; the batch's summed squared error goes to *loss.
 __global__ void GradReduceKernel(int tiles, const float* __restrict__ part, const float* __restrict__ loss_part,
                                  float* __restrict__ grad, float* __restrict__ loss) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -191,16 +192,54 @@ __global__ void GradReduceKernel(int tiles, const float* __restrict__ part, cons
   }
 }
 
-__global__ void AdamKernel(float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m,
-                           float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+// The training loop's update in one launch: each thread sums one parameter's tile gradients in tile
+// order (as GradReduceKernel does) and applies the Adam step to it.
+__global__ void GradReduceAdamKernel(int tiles, const float* __restrict__ part, const float* __restrict__ loss_part,
+                                     double* __restrict__ epoch_loss, float inv_count, float* __restrict__ w,
+                                     float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2,
+                                     float eps, float c1, float c2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kParams) return;
-  const float g = grad[p];
-  const float mm = b1 * m[p] + (1.f - b1) * g;
-  const float vv = b2 * v[p] + (1.f - b2) * g * g;
-  m[p] = mm;
-  v[p] = vv;
-  w[p] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+  if (p < kParams) {
+    float g = 0.f;
+    for (int b = 0; b < tiles; ++b) g += part[(size_t)b * kParams + p];
+    const float mm = b1 * m[p] + (1.f - b1) * g;
+    const float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    w[p] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+  }
+  if (p == kParams) {
+    float acc = 0.f;
+    for (int b = 0; b < tiles; ++b) acc += loss_part[b];
+    *epoch_loss += (double)(acc * inv_count);
+  }
+}
+
+// Epoch shuffle on the device: a Philox draw per row (counter (row, epoch), key = seed), then a
+// stable sort of (draw, row), so ties keep row order and the order depends only on the seed.
+__global__ void ShuffleKeysKernel(int64_t rows, uint64_t seed, int ep, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  uint32_t c[4] = {(uint32_t)r, (uint32_t)((uint64_t)r >> 32), (uint32_t)ep, 11u};
+  Philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  keys[r] = c[0];
+  vals[r] = (uint32_t)r;
+}
+
+__global__ void ZeroEpochLossKernel(int n, double* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// Rows of the normalised dataset in the epoch's order.
+__global__ void GatherRowsKernel(int64_t rows, const uint32_t* __restrict__ perm, const float* __restrict__ xn,
+                                 const float* __restrict__ yn, float* __restrict__ bx, float* __restrict__ by) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t q = perm[r];
+  for (int k = 0; k < kIn; ++k) bx[r * kIn + k] = xn[q * kIn + k];
+  for (int k = 0; k < kOut; ++k) by[r * kOut + k] = yn[q * kOut + k];
 }
 
 // ---- MFMA inference --------------------------------------------------------------------------------
@@ -793,13 +832,18 @@ int am_mlp_gradients(am_mlp* net, const float* X, const float* Y, int rows, floa
 }
 
 // Adam training on a dataset (raw features [rows][16], labels [rows][3], host): features are
-// standardised and labels divided by the output scale with the network's normalisation, then
-// `epochs` passes of minibatches (in a seeded shuffled order).  losses (optional, epochs).
+// standardised and labels divided by the output scale with the network's normalisation (once, on
+// the host), the normalised set is uploaded once, and each of the `epochs` passes runs on the
+// device without host round trips: a seeded shuffle (ShuffleKeysKernel and a radix sort), the
+// gather of the shuffled rows, and per minibatch the gradient tiles, their reduction and the Adam
+// update.  losses (optional, epochs): mean minibatch loss per epoch, summed on the device.
 int am_mlp_train(am_mlp* net, const float* X, const float* Y, int64_t rows, int epochs, int batch, float lr,
                  uint64_t seed, float* losses) {
   if (!net || !X || !Y || rows <= 0 || epochs <= 0 || batch <= 0)
     return Fail(AM_ERR_INVALID, "am_mlp_train: bad arguments");
+  if (rows > (int64_t)UINT32_MAX) return Fail(AM_ERR_INVALID, "am_mlp_train: more than 2^32 rows");
   am_ctx* ctx = net->ctx;
+  hipStream_t st = ctx->stream;
   std::vector<float> xn((size_t)rows * kIn), yn((size_t)rows * kOut);
   for (int64_t r = 0; r < rows; ++r) {
     for (int k = 0; k < kIn; ++k)
@@ -808,49 +852,55 @@ int am_mlp_train(am_mlp* net, const float* X, const float* Y, int64_t rows, int 
     if (net->baseline) QuadBaseline(X + r * kIn, yb);  // the network learns what the quadrupole leaves
     for (int k = 0; k < kOut; ++k) yn[r * kOut + k] = (Y[r * kOut + k] - yb[k]) / net->norm.out_scale[k];
   }
-  void *bX, *bY;
+  void *dxn, *dyn, *bX, *bY, *keys, *perm, *eloss;
   int rc;
-  // Shuffled copies per epoch (gather on the host order, uploaded once per epoch).
-  std::vector<int64_t> perm(rows);
-  for (int64_t r = 0; r < rows; ++r) perm[r] = r;
+  if ((rc = Scratch(ctx, "mlp_xn", (size_t)rows * kIn * 4, &dxn))) return rc;
+  if ((rc = Scratch(ctx, "mlp_yn", (size_t)rows * kOut * 4, &dyn))) return rc;
   if ((rc = Scratch(ctx, "mlp_bx", (size_t)rows * kIn * 4, &bX))) return rc;
   if ((rc = Scratch(ctx, "mlp_by", (size_t)rows * kOut * 4, &bY))) return rc;
-  std::vector<float> sx((size_t)rows * kIn), sy((size_t)rows * kOut);
+  if ((rc = Scratch(ctx, "mlp_shuffle_keys", (size_t)rows * 8, &keys))) return rc;
+  if ((rc = Scratch(ctx, "mlp_shuffle_perm", (size_t)rows * 4, &perm))) return rc;
+  if ((rc = Scratch(ctx, "mlp_epoch_loss", (size_t)epochs * 8, &eloss))) return rc;
+  if ((rc = UploadSync(ctx, dxn, xn.data(), (size_t)rows * kIn * 4))) return rc;
+  if ((rc = UploadSync(ctx, dyn, yn.data(), (size_t)rows * kOut * 4))) return rc;
+  hipLaunchKernelGGL(ZeroEpochLossKernel, dim3(Blocks(epochs, 256)), dim3(256), 0, st, epochs, (double*)eloss);
+  AM_HIP(hipGetLastError());
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  std::vector<int> nbatches(epochs, 0);
   for (int ep = 0; ep < epochs; ++ep) {
-    for (int64_t r = rows - 1; r > 0; --r) {  // Fisher-Yates with Philox draws
-      uint32_t c[4] = {(uint32_t)r, (uint32_t)ep, 11u, 5u};
-      Philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-      const int64_t j = (int64_t)(((uint64_t)c[0] << 32 | c[1]) % (uint64_t)(r + 1));
-      std::swap(perm[r], perm[j]);
-    }
-    for (int64_t r = 0; r < rows; ++r) {
-      std::memcpy(&sx[r * kIn], &xn[perm[r] * kIn], kIn * 4);
-      std::memcpy(&sy[r * kOut], &yn[perm[r] * kOut], kOut * 4);
-    }
-    AM_HIP(hipStreamSynchronize(ctx->stream));  // the previous epoch's kernels read bX / bY
-    if ((rc = UploadSync(ctx, bX, sx.data(), (size_t)rows * kIn * 4))) return rc;
-    if ((rc = UploadSync(ctx, bY, sy.data(), (size_t)rows * kOut * 4))) return rc;
-    double lsum = 0.0;
-    int nb = 0;
+    hipLaunchKernelGGL(ShuffleKeysKernel, dim3(Blocks(rows, 256)), dim3(256), 0, st, rows, seed, ep, (uint64_t*)keys,
+                       (uint32_t*)perm);
+    AM_HIP(hipGetLastError());
+    if ((rc = RadixSortPairs(ctx, (uint64_t*)keys, (uint32_t*)perm, rows, 32))) return rc;
+    hipLaunchKernelGGL(GatherRowsKernel, dim3(Blocks(rows, 256)), dim3(256), 0, st, rows, (const uint32_t*)perm,
+                       (const float*)dxn, (const float*)dyn, (float*)bX, (float*)bY);
+    AM_HIP(hipGetLastError());
     for (int64_t r0 = 0; r0 < rows; r0 += batch) {
       const int nrow = (int)std::min<int64_t>(batch, rows - r0);
-      float l = 0.f;
-      if ((rc = MlpGradients(net, (const float*)bX + r0 * kIn, (const float*)bY + r0 * kOut, nrow,
-                             losses ? &l : nullptr)))
-        return rc;
-      lsum += l;
-      ++nb;
+      const int tiles = (nrow + kTrainRows - 1) / kTrainRows;
+      void *part, *lpart;
+      if ((rc = Scratch(ctx, "mlp_part", (size_t)tiles * kParams * 4, &part))) return rc;
+      if ((rc = Scratch(ctx, "mlp_lpart", (size_t)tiles * 4, &lpart))) return rc;
+      const float inv_count = 1.f / ((float)nrow * kOut);
+      hipLaunchKernelGGL(MlpTrainTileKernel, dim3(tiles), dim3(256), 0, st, nrow, (const float*)bX + r0 * kIn,
+                         (const float*)bY + r0 * kOut, (const float*)net->w, inv_count, (float*)part, (float*)lpart);
+      ++nbatches[ep];
       ++net->adam_step;
       const float c1 = 1.f / (1.f - std::pow(b1, (float)net->adam_step));
       const float c2 = 1.f / (1.f - std::pow(b2, (float)net->adam_step));
-      hipLaunchKernelGGL(AdamKernel, dim3(Blocks(kParams, 256)), dim3(256), 0, ctx->stream, net->w,
-                         (const float*)net->grad, net->adam_m, net->adam_v, lr, b1, b2, eps, c1, c2);
+      hipLaunchKernelGGL(GradReduceAdamKernel, dim3(Blocks(kParams + 1, 256)), dim3(256), 0, st, tiles,
+                         (const float*)part, (const float*)lpart, (double*)eloss + ep, inv_count, net->w,
+                         net->adam_m, net->adam_v, lr, b1, b2, eps, c1, c2);
       AM_HIP(hipGetLastError());
     }
-    if (losses) losses[ep] = (float)(lsum / std::max(nb, 1));
   }
-  AM_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> el(epochs);
+  AM_HIP(hipMemcpyAsync(el.data(), eloss, (size_t)epochs * 8, hipMemcpyDeviceToHost, st));
+  AM_HIP(hipStreamSynchronize(st));
+  // the shuffles' sorts report a look-back give-up here (their permutation would be wrong)
+  if ((rc = CheckSortError(ctx))) return rc;
+  if (losses)
+    for (int ep = 0; ep < epochs; ++ep) losses[ep] = (float)(el[ep] / std::max(nbatches[ep], 1));
   net->frag_valid = false;
   return AM_OK;
 }
