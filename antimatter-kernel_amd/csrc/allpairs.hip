@@ -231,6 +231,7 @@ void AllPairsSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_blocks, i
                           const float4* __restrict__ sloc, const TileRec* __restrict__ tiles,
                           double* __restrict__ part) {
   __shared__ int s_item;
+  __shared__ float4 s_stage[kTile];  // the expanded far-tile sums' source records
   const int n_tiles = n_pad / kTile;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
@@ -254,7 +255,7 @@ void AllPairsSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_blocks, i
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int T = t0; T < t1; ++T) {
       float px = 0.f, py = 0.f, pz = 0.f;
-      TileSum(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc, px, py, pz);
+      TileSum(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc, s_stage, px, py, pz);
       ax += (double)px;
       ay += (double)py;
       az += (double)pz;

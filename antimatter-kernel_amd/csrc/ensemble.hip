@@ -161,6 +161,7 @@ __global__ __launch_bounds__(kEnsThreads) __attribute__((amdgpu_waves_per_eu(8, 
                                                                    const TileRec* __restrict__ tiles,
                                                                    double* __restrict__ part) {
   __shared__ int s_item;
+  __shared__ float4 s_stage[kTile];  // the expanded far-tile sums' source records
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kEnsThreads) __attribute__((amdgpu_waves_per_eu(8, 
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int T = t0; T < t1; ++T) {
       float px = 0.f, py = 0.f, pz = 0.f;
-      TileSum(tiles[T], T * kEnsTile, bb, th, tl, lx, ly, lz, shl, sloc, px, py, pz);
+      TileSum(tiles[T], T * kEnsTile, bb, th, tl, lx, ly, lz, shl, sloc, s_stage, px, py, pz);
       ax += (double)px;
       ay += (double)py;
       az += (double)pz;

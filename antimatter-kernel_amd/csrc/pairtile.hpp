@@ -7,6 +7,14 @@
 // coordinate is shifted once per tile by the FP32-rounded centre offset, then dx = tx'' - sx'
 // costs one instruction instead of three.  Its rounding error is ~2^-24 (|dx| + box size), i.e.
 // <~1e-6 relative for a far pair, below the FP32 accumulation error of the tile sums.
+//
+// Unit-charge tiles at least kExpandGap x the larger half extent away (and at a positive gap) use the
+// expanded form r^2 = |t|^2 + (|s|^2 + eps^2) - 2 t.s: the workgroup stages the tile's records
+// (-2s, |s|^2 + eps^2) in LDS, each wave reads them back as broadcasts (VGPR operands), and sums c and
+// c(-2s) per charge sign; t sum(c) - sum(c s) is formed once per tile.  That is 10 VALU instructions
+// and one rsq per pair, none reading an SGPR (a VALU instruction with an SGPR operand issues at half
+// rate on gfx950).  The cancellation error of r^2 is ~2^-24 (|t| + |s|)^2 / r^2 with t and s local to
+// the source tile, at most ~20 x 2^-24 at that separation.
 #pragma once
 
 #include "am_internal.hpp"
@@ -15,6 +23,7 @@ namespace am {
 
 constexpr int kTile = 128;        // sources per tile (and per FP32 partial sum)
 constexpr double kNearGap = 0.1;  // near if box gap < kNearGap * (larger half extent) * 2
+constexpr double kExpandGap = 1.0;  // expanded form if box gap >= kExpandGap * (larger half extent)
 
 struct TileRec {  // sorted source tile, units of 2^-20 m
   double cx, cy, cz;
@@ -185,6 +194,20 @@ __device__ __forceinline__ void PairLocal(const float tx, const float ty, const 
   pz = fmaf(c, dz, pz);
 }
 
+// Expanded-form pair of a unit source of one sign: s = (-2 s_x, -2 s_y, -2 s_z, |s|^2 + eps^2) local
+// to the tile centre (a staged record, read from LDS), t local to it too with tt = |t|^2; adds
+// c = 1/r^3 to sc and c (-2s) to (sx, sy, sz).
+__device__ __forceinline__ void PairExpanded(const float tx, const float ty, const float tz, const float tt,
+                                             const float4 s, float& sc, float& sx, float& sy, float& sz) {
+  const float r2 = fmaf(s.x, tx, fmaf(s.y, ty, fmaf(s.z, tz, tt + s.w)));
+  const float ri = __builtin_amdgcn_rsqf(r2);
+  const float c = ri * (ri * ri);
+  sc += c;
+  sx = fmaf(c, s.x, sx);
+  sy = fmaf(c, s.y, sy);
+  sz = fmaf(c, s.z, sz);
+}
+
 // Box of a 256-target block: the union of its two tiles (an empty second tile is left out).
 struct BlockBox {
   double bc[3], bh[3], bhmax;
@@ -210,13 +233,20 @@ __device__ inline BlockBox MakeBlockBox(const TileRec& A, const TileRec& B, bool
 // block: hi/lo coordinates th, tl and the coordinates local to the block centre (lx, ly, lz).
 __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const BlockBox& bb, const float4 th,
                                         const float4 tl, float lx, float ly, float lz, const float4* __restrict__ shl,
-                                        const float4* __restrict__ sloc, float& px, float& py, float& pz) {
+                                        const float4* __restrict__ sloc, float4* __restrict__ stage, float& px,
+                                        float& py, float& pz) {
   const double gx = fmax(0.0, fabs(R.cx - bb.bc[0]) - R.hx - bb.bh[0]);
   const double gy = fmax(0.0, fabs(R.cy - bb.bc[1]) - R.hy - bb.bh[1]);
   const double gz = fmax(0.0, fabs(R.cz - bb.bc[2]) - R.hz - bb.bh[2]);
   const double size = fmax(fmax(fmax(R.hx, R.hy), R.hz), bb.bhmax);
   const double lim = 2.0 * kNearGap * size;
-  const int near = __builtin_amdgcn_readfirstlane((gx * gx + gy * gy + gz * gz) < lim * lim ? 1 : 0);
+  const double gap2 = gx * gx + gy * gy + gz * gz;
+  const int near = __builtin_amdgcn_readfirstlane(gap2 < lim * lim ? 1 : 0);
+  // Expanded form only across a positive gap: then r >= gap > 0 for every pair (no self or coincident
+  // pair, whose r^2 would cancel to rounding noise); degenerate (zero-size) boxes at zero gap keep the
+  // local form.
+  const int expand =
+      __builtin_amdgcn_readfirstlane(gap2 > 0.0 && gap2 >= (kExpandGap * size) * (kExpandGap * size) ? 1 : 0);
   // Unit tiles: groups of 8 sources; whole groups of one sign fold the sign into the difference,
   // the (at most two) groups straddling a sign boundary use the charge-multiplied form.
   const int unit = __builtin_amdgcn_readfirstlane(R.unit);
@@ -259,7 +289,36 @@ __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const 
     const float ty = ly + (float)(bb.bc[1] - R.cy);
     const float tz = lz + (float)(bb.bc[2] - R.cz);
     const float4* sp = sloc + tile_first;
-    if (unit) {
+    if (unit && expand) {
+      // Expanded form: positive sources [0, np) into one set of sums, negative [np, nq) into another
+      // (the group straddling the boundary is split source by source on uniform branches);
+      // F = t (Sc+ - Sc-) + 0.5 (S+(-2s) - S-(-2s)).
+      // The whole workgroup takes this branch for this tile (the classification depends only on the
+      // block and the tile), so it stages the tile's kTile records together.
+      __syncthreads();  // every wave has finished reading the previous staged tile
+      if (threadIdx.x < kTile) {
+        const float4 q = sp[threadIdx.x];
+        stage[threadIdx.x] = make_float4(-2.f * q.x, -2.f * q.y, -2.f * q.z,
+                                         fmaf(q.x, q.x, fmaf(q.y, q.y, fmaf(q.z, q.z, kEps2Units))));
+      }
+      __syncthreads();
+      const float tt = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+      float pc = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;  // positive sources
+      float nc = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;  // negative sources
+#pragma unroll 8
+      for (int j = 0; j < ga; ++j) PairExpanded(tx, ty, tz, tt, stage[j], pc, sx, sy, sz);
+      for (int j = ga; j < gb; ++j) {  // the group straddling the sign boundary
+        if (j < np) PairExpanded(tx, ty, tz, tt, stage[j], pc, sx, sy, sz);
+        else if (j < nq) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
+      }
+#pragma unroll 8
+      for (int j = gb; j < gc; ++j) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
+      for (int j = gc; j < nq; ++j) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
+      const float dc = pc - nc;
+      px = fmaf(tx, dc, fmaf(0.5f, sx - ux, px));
+      py = fmaf(ty, dc, fmaf(0.5f, sy - uy, py));
+      pz = fmaf(tz, dc, fmaf(0.5f, sz - uz, pz));
+    } else if (unit) {
       for (int j0 = 0; j0 < ga; j0 += 8) {
         const Src8 g = LoadSrc8(sp, j0);
 #pragma unroll
