@@ -290,11 +290,10 @@ __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const 
     const float tz = lz + (float)(bb.bc[2] - R.cz);
     const float4* sp = sloc + tile_first;
     if (unit && expand) {
-      // Expanded form: positive sources [0, np) into one set of sums, negative [np, nq) into another
-      // (the group straddling the boundary is split source by source on uniform branches);
-      // F = t (Sc+ - Sc-) + 0.5 (S+(-2s) - S-(-2s)).
-      // The whole workgroup takes this branch for this tile (the classification depends only on the
-      // block and the tile), so it stages the tile's kTile records together.
+      // Expanded form over the staged records, by sign: the positive sources [0, np), then the
+      // negative ones [np, nq), each folded into F as +-(t sum c + 0.5 sum c (-2s)).  The whole
+      // workgroup takes this branch for this tile (the classification depends only on the block and
+      // the tile), so it stages the tile's kTile records together.
       __syncthreads();  // every wave has finished reading the previous staged tile
       if (threadIdx.x < kTile) {
         const float4 q = sp[threadIdx.x];
@@ -303,21 +302,18 @@ __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const 
       }
       __syncthreads();
       const float tt = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
-      float pc = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;  // positive sources
-      float nc = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;  // negative sources
+      float sc = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll 8
-      for (int j = 0; j < ga; ++j) PairExpanded(tx, ty, tz, tt, stage[j], pc, sx, sy, sz);
-      for (int j = ga; j < gb; ++j) {  // the group straddling the sign boundary
-        if (j < np) PairExpanded(tx, ty, tz, tt, stage[j], pc, sx, sy, sz);
-        else if (j < nq) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
-      }
+      for (int j = 0; j < np; ++j) PairExpanded(tx, ty, tz, tt, stage[j], sc, sx, sy, sz);
+      px = fmaf(tx, sc, fmaf(0.5f, sx, px));
+      py = fmaf(ty, sc, fmaf(0.5f, sy, py));
+      pz = fmaf(tz, sc, fmaf(0.5f, sz, pz));
+      sc = sx = sy = sz = 0.f;
 #pragma unroll 8
-      for (int j = gb; j < gc; ++j) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
-      for (int j = gc; j < nq; ++j) PairExpanded(tx, ty, tz, tt, stage[j], nc, ux, uy, uz);
-      const float dc = pc - nc;
-      px = fmaf(tx, dc, fmaf(0.5f, sx - ux, px));
-      py = fmaf(ty, dc, fmaf(0.5f, sy - uy, py));
-      pz = fmaf(tz, dc, fmaf(0.5f, sz - uz, pz));
+      for (int j = np; j < nq; ++j) PairExpanded(tx, ty, tz, tt, stage[j], sc, sx, sy, sz);
+      px = fmaf(-tx, sc, fmaf(-0.5f, sx, px));
+      py = fmaf(-ty, sc, fmaf(-0.5f, sy, py));
+      pz = fmaf(-tz, sc, fmaf(-0.5f, sz, pz));
     } else if (unit) {
       for (int j0 = 0; j0 < ga; j0 += 8) {
         const Src8 g = LoadSrc8(sp, j0);
