@@ -302,25 +302,36 @@ __global__ void AllPairsSortedReduceKernel(int n, int n_pad, int chunks, const d
 // phi of source tile R at one target, sum_j q_j / r in e/unit.  Pairs closer than the reference's
 // 1e-15 m floor (energy_analysis.cc via barnes-hut.cc:244-247), the target itself among them, are
 // skipped; they can only sit in near tiles, where the test costs a compare and a select per pair.
+// Far unit-charge tiles across a gap of at least a half extent take the expanded form on staged
+// records (4 VALU + rsq + 1 add per pair), the other far tiles the local form.
 __device__ __forceinline__ float TilePotential(const TileRec& R, int tile_first, const BlockBox& bb, const float4 th,
                                                const float4 tl, float lx, float ly, float lz,
-                                               const float4* __restrict__ shl, const float4* __restrict__ sloc) {
+                                               const float4* __restrict__ shl, const float4* __restrict__ sloc,
+                                               float4* __restrict__ stage) {
   const double gx = fmax(0.0, fabs(R.cx - bb.bc[0]) - R.hx - bb.bh[0]);
   const double gy = fmax(0.0, fabs(R.cy - bb.bc[1]) - R.hy - bb.bh[1]);
   const double gz = fmax(0.0, fabs(R.cz - bb.bc[2]) - R.hz - bb.bh[2]);
   const double size = fmax(fmax(fmax(R.hx, R.hy), R.hz), bb.bhmax);
   const double lim = 2.0 * kNearGap * size;
-  const int near = __builtin_amdgcn_readfirstlane((gx * gx + gy * gy + gz * gz) < lim * lim ? 1 : 0);
+  const double gap2 = gx * gx + gy * gy + gz * gz;
+  const int near = __builtin_amdgcn_readfirstlane(gap2 < lim * lim ? 1 : 0);
+  // the force sums' expanded-form rule: a positive gap of at least kExpandGap x the larger half extent
+  const int expand =
+      __builtin_amdgcn_readfirstlane(gap2 > 0.0 && gap2 >= (kExpandGap * size) * (kExpandGap * size) ? 1 : 0);
+  const int unit = __builtin_amdgcn_readfirstlane(R.unit);
+  const int np = __builtin_amdgcn_readfirstlane(R.npos);
   const int nq = __builtin_amdgcn_readfirstlane(R.npos + R.nneg);  // charged sources come first
   const int ge = (nq + 7) & ~7;
   float p = 0.f;
   if (near) {
+    // hi/lo sources in groups of 4 (32 SGPRs): groups of 8 spill SGPRs to VGPR lanes inside the loop
     const float4* sp = shl + 2 * tile_first;
-    for (int j0 = 0; j0 < ge; j0 += 8) {
-      const SrcHL8 g = LoadSrcHL8(sp, j0);
+    const int ge4 = (nq + 3) & ~3;
+    for (int j0 = 0; j0 < ge4; j0 += 4) {
+      const Src8 g = LoadSrc8(sp, 2 * j0);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const float4 sh = g.hi(u), sl = g.lo(u);
+      for (int u = 0; u < 4; ++u) {
+        const float4 sh = g[2 * u], sl = g[2 * u + 1];
         const float dx = (th.x - sh.x) + (tl.x - sl.x);
         const float dy = (th.y - sh.y) + (tl.y - sl.y);
         const float dz = (th.z - sh.z) + (tl.z - sl.z);
@@ -329,6 +340,20 @@ __device__ __forceinline__ float TilePotential(const TileRec& R, int tile_first,
         p = fmaf(w, __builtin_amdgcn_rsqf(r2), p);
       }
     }
+  } else if (unit && expand) {
+    // +e sources [0, np), then -e sources [np, nq), on the staged expanded-form records (the whole
+    // workgroup takes this branch for this tile)
+    const float tx = lx + (float)(bb.bc[0] - R.cx);
+    const float ty = ly + (float)(bb.bc[1] - R.cy);
+    const float tz = lz + (float)(bb.bc[2] - R.cz);
+    StageExpanded(sloc + tile_first, stage);
+    const float tt = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+    float pp = 0.f, pn = 0.f;
+#pragma unroll 8
+    for (int j = 0; j < np; ++j) pp += InvRExpanded(tx, ty, tz, tt, stage[j]);
+#pragma unroll 8
+    for (int j = np; j < nq; ++j) pn += InvRExpanded(tx, ty, tz, tt, stage[j]);
+    p = pp - pn;
   } else {
     const float tx = lx + (float)(bb.bc[0] - R.cx);
     const float ty = ly + (float)(bb.bc[1] - R.cy);
@@ -359,6 +384,7 @@ void AllPairsPotentialSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_
                                    const float4* __restrict__ sloc, const TileRec* __restrict__ tiles,
                                    double* __restrict__ part) {
   __shared__ int s_item;
+  __shared__ float4 s_stage[kTile];  // the expanded far-tile sums' source records
   const int n_tiles = n_pad / kTile;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
@@ -381,7 +407,7 @@ void AllPairsPotentialSortedKernel(int n, int n_pad, int tiles_per_chunk, int n_
     const int t1 = min(n_tiles, t0 + tiles_per_chunk);
     double acc = 0.0;
     for (int T = t0; T < t1; ++T)
-      acc += (double)TilePotential(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc);
+      acc += (double)TilePotential(tiles[T], T * kTile, bb, th, tl, lx, ly, lz, shl, sloc, s_stage);
     part[(size_t)chunk * n_pad + k] = acc;
   }
   if (threadIdx.x == 0) {

@@ -172,15 +172,6 @@ __device__ __forceinline__ Src8 LoadSrc8(const float4* tile, int j0) {
   const f32x16v* p = reinterpret_cast<const f32x16v*>(Group8(tile, j0));
   return Src8{p[0], p[1]};
 }
-// Eight hi/lo source pairs (16 float4) as four 16-dword vectors.
-struct SrcHL8 {
-  Src8 h0, h1;  // entries 0..7 and 8..15 of the interleaved hi/lo array
-  __device__ __forceinline__ float4 hi(int u) const { return u < 4 ? h0[2 * u] : h1[2 * (u - 4)]; }
-  __device__ __forceinline__ float4 lo(int u) const { return u < 4 ? h0[2 * u + 1] : h1[2 * (u - 4) + 1]; }
-};
-__device__ __forceinline__ SrcHL8 LoadSrcHL8(const float4* tile, int j0) {
-  return SrcHL8{LoadSrc8(tile, 2 * j0), LoadSrc8(tile, 2 * j0 + 8)};
-}
 
 // Far-tile pair: local coordinates (one subtraction per axis), same softening as PairF32.
 __device__ __forceinline__ void PairLocal(const float tx, const float ty, const float tz, const float4 s, float& px,
@@ -212,6 +203,24 @@ __device__ __forceinline__ void PairExpanded(const float tx, const float ty, con
 struct BlockBox {
   double bc[3], bh[3], bhmax;
 };
+
+// Stages the kTile local records of one source tile as expanded-form records (-2s, |s|^2 + eps^2).
+// Called by the whole workgroup; the first barrier keeps every wave's reads of the previously staged
+// tile ahead of the overwrite.
+__device__ __forceinline__ void StageExpanded(const float4* __restrict__ sp, float4* __restrict__ stage) {
+  __syncthreads();
+  if (threadIdx.x < kTile) {
+    const float4 q = sp[threadIdx.x];
+    stage[threadIdx.x] =
+        make_float4(-2.f * q.x, -2.f * q.y, -2.f * q.z, fmaf(q.x, q.x, fmaf(q.y, q.y, fmaf(q.z, q.z, kEps2Units))));
+  }
+  __syncthreads();
+}
+
+// 1/r of one expanded-form record at target t (tt = |t|^2), both local to the source tile centre.
+__device__ __forceinline__ float InvRExpanded(float tx, float ty, float tz, float tt, const float4 s) {
+  return __builtin_amdgcn_rsqf(fmaf(s.x, tx, fmaf(s.y, ty, fmaf(s.z, tz, tt + s.w))));
+}
 
 __device__ inline BlockBox MakeBlockBox(const TileRec& A, const TileRec& B, bool b_empty) {
   double blo[3] = {A.cx - A.hx, A.cy - A.hy, A.cz - A.hz}, bhi[3] = {A.cx + A.hx, A.cy + A.hy, A.cz + A.hz};
@@ -294,13 +303,7 @@ __device__ __forceinline__ void TileSum(const TileRec& R, int tile_first, const 
       // negative ones [np, nq), each folded into F as +-(t sum c + 0.5 sum c (-2s)).  The whole
       // workgroup takes this branch for this tile (the classification depends only on the block and
       // the tile), so it stages the tile's kTile records together.
-      __syncthreads();  // every wave has finished reading the previous staged tile
-      if (threadIdx.x < kTile) {
-        const float4 q = sp[threadIdx.x];
-        stage[threadIdx.x] = make_float4(-2.f * q.x, -2.f * q.y, -2.f * q.z,
-                                         fmaf(q.x, q.x, fmaf(q.y, q.y, fmaf(q.z, q.z, kEps2Units))));
-      }
-      __syncthreads();
+      StageExpanded(sp, stage);
       const float tt = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
       float sc = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll 8
