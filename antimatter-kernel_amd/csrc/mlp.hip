@@ -464,6 +464,35 @@ __device__ inline void ExactResidual(uint32_t i, const float4 th, const float4 t
   out[2] = fz - mz;
 }
 
+// The same residual summed by the whole wave, lane l taking particles b + l, b + l + 64, ...; the
+// monopole is subtracted in lane 0 only, so the wave's lanes add up to the residual.
+__device__ inline void ExactResidualWave(uint32_t i, const float4 th, const float4 tl, const WalkNode& nd,
+                                         const WalkNodeAux& nf, int sgn, const uint32_t* __restrict__ order,
+                                         const float4* __restrict__ src, int lane, float out[3]) {
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int k = nd.b + lane; k < nd.e; k += 64) {
+    const uint32_t j = order[k];
+    if (j == i) continue;
+    const float4 sh = src[2 * j];
+    if (sgn == 0 ? !(sh.w > 0.f) : !(sh.w < 0.f)) continue;
+    PairF32(th, tl, sh, src[2 * j + 1], fx, fy, fz);
+  }
+  if (lane == 0) {
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    PairF32(th, tl, sgn == 0 ? nf.p_hi : nf.n_hi, sgn == 0 ? nf.p_lo : nf.n_lo, mx, my, mz);
+    fx -= mx;
+    fy -= my;
+    fz -= mz;
+  }
+  out[0] = fx;
+  out[1] = fy;
+  out[2] = fz;
+}
+
+// Fallback rows over nodes of more than this many particles are summed by the whole wave: in one
+// lane, a node of S particles holds the wave for S pair iterations.
+constexpr int kFallbackLaneMax = 64;
+
 // Training rows: for sampled target k (sorted position tsel[k]) and its recorded node slot,
 // both signs: raw features [16] and the normalised label y (FP64 exact residual).
 __global__ void MlpDataKernel(int n_targets, const uint32_t* __restrict__ tsel, const uint32_t* __restrict__ rec,
@@ -598,8 +627,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void Ml
           const float moved = __shfl_xor(y2[o], 32);  // batch 2's row n + 32 is output in lane n
           y[o] = upper ? moved : y1[o];
         }
+        bool wave_fallback = false;
         if (ok) {
-          float corr[3];
+          float corr[3] = {0.f, 0.f, 0.f};
           const float p0 = y[0] + yb[0], p1 = y[1] + yb[1], p2 = y[2] + yb[2];
           const float a = hh / R;
           const float mag = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
@@ -609,13 +639,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void Ml
             corr[1] = p1 * sc;
             corr[2] = p2 * sc;
             ++n_model;
-          } else {
+          } else if (nodes[c].e - nodes[c].b <= kFallbackLaneMax) {
             ExactResidual(i, th, tl, nodes[c], nodes32[c], row & 1, order, src, corr);
             ++n_fallback;
+          } else {
+            wave_fallback = true;
           }
           cx += corr[0];
           cy += corr[1];
           cz += corr[2];
+        }
+        // the large-node fallback rows of this batch, one at a time over the whole wave
+        for (unsigned long long mask = __ballot(wave_fallback); mask; mask &= mask - 1) {
+          const int owner = __builtin_ctzll(mask);
+          const int fc = __builtin_amdgcn_readfirstlane(__shfl(c, owner));
+          const int fs = __builtin_amdgcn_readfirstlane(__shfl(row & 1, owner));
+          float corr[3];
+          ExactResidualWave(i, th, tl, nodes[fc], nodes32[fc], fs, order, src, l, corr);
+          cx += corr[0];
+          cy += corr[1];
+          cz += corr[2];
+          if (l == owner) ++n_fallback;
         }
       }
       for (int off = 32; off > 0; off >>= 1) {
