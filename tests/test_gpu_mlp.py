@@ -187,6 +187,26 @@ def test_fallback_everywhere_gives_the_direct_sum(ctx, small_cloud):
     net.close()
 
 
+def test_fallback_over_large_nodes_gives_the_direct_sum(ctx):
+    """Every row forced to the exact fallback on a 32,768-particle cloud at theta = 0.9: the accepted
+    nodes then include many of more than 64 particles, whose residuals the whole wave sums."""
+    pos, vel, sp = pm_cloud(32768, seed=5)
+    s = am.Store(ctx, len(sp))
+    s.upload(pos, vel, sp)
+    m, q = oracle.mass_charge(sp)
+    ref = oracle.allpairs_forces(pos, q)
+    net = mlp.Mlp(ctx, seed=1)
+    print("forced fallback: n=32768 theta=0.9", flush=True)
+    st = mlp.barnes_hut_learned_forces(s, 0.9, net, rec_cap=4096, ybound=-1.0, amax=10.0)
+    f = s.forces()
+    err = np.linalg.norm(f - ref) / np.linalg.norm(ref)
+    print(st, f"err {err:.2e}")
+    assert st["model_rows"] == 0 and st["fallback_rows"] > 0 and st["overflowed"] == 0
+    assert err < 1e-5
+    net.close()
+    s.close()
+
+
 def config2_cloud(n, seed, t_pbar=10.0, t_pos=100.0):
     """The learned-correction workload's cloud (the all-pairs workload's setup): antiprotons and
     positrons 50/50 in radii (1e-3, 1e-3, 4e-3) m."""
